@@ -1,0 +1,385 @@
+"""Seeded synthetic BEM inputs in depSolver's formats.
+
+Geometry builders (quadratic icospheres, square plate electrodes, the planar
+quadrupole trap, shelled particles) and writers for the reference's input
+files: ``input.bem``, ``nodes.bem``, ``elems.bem``, ``bcs.bem`` and internal
+points (formats of src/depSolver.c:98-252).
+
+Conventions (as in the reference's examples, e.g. examples/dep-trap-2):
+
+* tria6 node order is corner, mid, corner, mid, corner, mid
+  (src/shapeFunctions.c:68-87): corner 1 at L=(1,0), corner 2 at L=(0,1),
+  corner 3 at L=(0,0).
+* particle surfaces are oriented so that (X1-X3) x (X2-X3) points out of the
+  particle, and interfaces are listed as (outside material, inside material).
+* bc rows: 2N rows, Re block then Im block; conductors are type 1 with the
+  potential, interface nodes are type 6 (force wanted) or 0 with values
+  ``0 <interface id>``.
+"""
+from __future__ import annotations
+
+import dataclasses
+import os
+
+import numpy as np
+
+EPS0 = 8.854187817e-12
+
+
+@dataclasses.dataclass
+class Surface:
+    nodes: np.ndarray      # (n, 3) float64
+    elems: np.ndarray      # (e, 6) or (e, 3) int64, 0-based into ``nodes``
+
+
+@dataclasses.dataclass
+class Problem:
+    """One BEM problem: mesh, boundary conditions, materials, frequency.
+
+    ``conn`` is 0-based; ``bctype``/``bc`` have 2N rows (Re block, Im block)
+    like the reference's vBCType/mBC; ``ifaces`` holds 1-based material ids
+    (outside, inside) like vInterfaces; ``mats`` rows are (sigma, eps_r).
+    """
+    xyz: np.ndarray
+    conn: np.ndarray
+    bctype: np.ndarray
+    bc: np.ndarray
+    mats: np.ndarray
+    ifaces: np.ndarray
+    freq: float
+    body: np.ndarray | None = None   # per-element body id (0 = electrodes)
+    name: str = ""
+
+    @property
+    def n_nodes(self) -> int:
+        return int(self.xyz.shape[0])
+
+    @property
+    def n_elems(self) -> int:
+        return int(self.conn.shape[0])
+
+    @property
+    def npe(self) -> int:
+        return int(self.conn.shape[1])
+
+    def rhs(self) -> np.ndarray:
+        """vB as the reference assembles it (assembly_el.c:578-579)."""
+        n = self.n_nodes
+        return np.concatenate([self.bc[:n, 0], self.bc[n:, 0]]).astype(np.float64)
+
+
+# --------------------------------------------------------------------------
+# surfaces
+# --------------------------------------------------------------------------
+
+def _icosahedron():
+    t = (1.0 + 5.0 ** 0.5) / 2.0
+    v = np.array([(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0),
+                  (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+                  (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)], dtype=np.float64)
+    f = np.array([(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11),
+                  (1, 5, 9), (5, 11, 4), (11, 10, 2), (10, 7, 6), (7, 1, 8),
+                  (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9),
+                  (4, 9, 5), (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)], dtype=np.int64)
+    return v / np.linalg.norm(v, axis=1, keepdims=True), f
+
+
+def _edge_midpoints(faces, n_vert):
+    """Unique edges of a triangle list; returns (edge_index[f, 3], edges[m, 2])
+    with edge k of face f joining corners k and k+1."""
+    a = faces
+    pairs = np.stack([a[:, [0, 1]], a[:, [1, 2]], a[:, [2, 0]]], axis=1).reshape(-1, 2)
+    key = np.minimum(pairs[:, 0], pairs[:, 1]) * n_vert + np.maximum(pairs[:, 0], pairs[:, 1])
+    uniq, inv = np.unique(key, return_inverse=True)
+    edges = np.stack([uniq // n_vert, uniq % n_vert], axis=1)
+    return inv.reshape(-1, 3), edges
+
+
+def icosphere(level: int, radius: float = 1.0, center=(0.0, 0.0, 0.0),
+              quadratic: bool = True, inward: bool = False) -> Surface:
+    """Icosphere with 20*4**level triangles; quadratic midside nodes are
+    projected onto the sphere.  Outward orientation unless ``inward``."""
+    v, f = _icosahedron()
+    for _ in range(level):
+        eidx, edges = _edge_midpoints(f, len(v))
+        mid = v[edges[:, 0]] + v[edges[:, 1]]
+        mid /= np.linalg.norm(mid, axis=1, keepdims=True)
+        m = eidx + len(v)
+        v = np.concatenate([v, mid])
+        a, b, c = f[:, 0], f[:, 1], f[:, 2]
+        ab, bc, ca = m[:, 0], m[:, 1], m[:, 2]
+        f = np.concatenate([np.stack([a, ab, ca], 1), np.stack([b, bc, ab], 1),
+                            np.stack([c, ca, bc], 1), np.stack([ab, bc, ca], 1)])
+    if inward:
+        f = f[:, [1, 0, 2]]
+    if quadratic:
+        eidx, edges = _edge_midpoints(f, len(v))
+        mid = v[edges[:, 0]] + v[edges[:, 1]]
+        mid /= np.linalg.norm(mid, axis=1, keepdims=True)
+        m = eidx + len(v)
+        nodes = np.concatenate([v, mid])
+        # corners (c1, c2, c3) = (f0, f1, f2): mids c1c2 = edge 0, c2c3 = edge 1, c3c1 = edge 2
+        elems = np.stack([f[:, 0], m[:, 0], f[:, 1], m[:, 1], f[:, 2], m[:, 2]], axis=1)
+    else:
+        nodes, elems = v, f
+    return Surface(nodes * radius + np.asarray(center, dtype=np.float64), elems.astype(np.int64))
+
+
+def plate(n: int, size: float, center=(0.0, 0.0, 0.0), quadratic: bool = True,
+          normal_up: bool = True, size_y: float | None = None) -> Surface:
+    """Flat rectangular plate in the z = center[2] plane, an n x n grid of
+    squares each split along the same diagonal (2 n^2 triangles)."""
+    sy = size if size_y is None else size_y
+    q = 2 if quadratic else 1
+    m = q * n + 1
+    xs = np.linspace(-size / 2, size / 2, m)
+    ys = np.linspace(-sy / 2, sy / 2, m)
+    X, Y = np.meshgrid(xs, ys, indexing="xy")
+    nodes = np.stack([X.ravel(), Y.ravel(), np.zeros(m * m)], axis=1) + np.asarray(center)
+
+    def nid(i, j):           # i along x, j along y, on the fine grid
+        return j * m + i
+
+    el = []
+    for j in range(n):
+        for i in range(n):
+            i0, j0 = q * i, q * j
+            p00, p10 = (i0, j0), (i0 + q, j0)
+            p01, p11 = (i0, j0 + q), (i0 + q, j0 + q)
+            for tri in ((p00, p10, p11), (p00, p11, p01)):
+                c1, c2, c3 = tri
+                if quadratic:
+                    mid = lambda a, b: ((a[0] + b[0]) // 2, (a[1] + b[1]) // 2)
+                    el.append([nid(*c1), nid(*mid(c1, c2)), nid(*c2), nid(*mid(c2, c3)),
+                               nid(*c3), nid(*mid(c3, c1))])
+                else:
+                    el.append([nid(*c1), nid(*c2), nid(*c3)])
+    elems = np.asarray(el, dtype=np.int64)
+    if not normal_up:
+        elems = elems[:, [4, 3, 2, 1, 0, 5]] if quadratic else elems[:, [1, 0, 2]]
+    return Surface(nodes, elems)
+
+
+def _merge(surfaces):
+    nodes, elems, off = [], [], 0
+    for s in surfaces:
+        nodes.append(s.nodes)
+        elems.append(s.elems + off)
+        off += len(s.nodes)
+    return np.concatenate(nodes), np.concatenate(elems)
+
+
+def build_problem(conductors, particles, mats, freq, name=""):
+    """Assemble a Problem.
+
+    conductors: list of (Surface, potential_re)
+    particles:  list of (Surface, bc_type, interface_id)   interface ids 1-based
+    mats:       (M, 2) array of (sigma, eps_r); ifaces are inferred by caller
+    """
+    surfs = [s for s, _ in conductors] + [s for s, _, _ in particles]
+    xyz, conn = _merge(surfs)
+    n = len(xyz)
+    bctype = np.zeros(2 * n, dtype=np.uint32)
+    bc = np.zeros((2 * n, 3), dtype=np.float64)
+    body = np.zeros(len(conn), dtype=np.int32)
+    off, eoff = 0, 0
+    for s, v in conductors:
+        k = len(s.nodes)
+        bctype[off:off + k] = 1
+        bctype[n + off:n + off + k] = 1
+        bc[off:off + k, 0] = v
+        off += k
+        eoff += len(s.elems)
+    for b, (s, t, iid) in enumerate(particles, start=1):
+        k = len(s.nodes)
+        bctype[off:off + k] = t
+        bctype[n + off:n + off + k] = t
+        bc[off:off + k, 1] = iid
+        bc[n + off:n + off + k, 1] = iid
+        body[eoff:eoff + len(s.elems)] = b
+        off += k
+        eoff += len(s.elems)
+    return xyz, conn, bctype, bc, body
+
+
+# --------------------------------------------------------------------------
+# the benchmark configurations
+# --------------------------------------------------------------------------
+
+MEDIUM = (1e-3, 78.0)
+PARTICLE = (1e-4, 2.5)
+
+
+def sphere_between_plates(level: int, n_plate: int, freq: float = 1e6,
+                          quadratic: bool = True, name: str = "") -> Problem:
+    """Dielectric sphere (radius 1, centre origin) between two 10x10 plates at
+    z = +-3 held at +-1 V; medium eps_r 78 / 1e-3 S/m, particle 2.5 / 1e-4 S/m."""
+    top = plate(n_plate, 10.0, (0.0, 0.0, 3.0), quadratic, normal_up=False)
+    bot = plate(n_plate, 10.0, (0.0, 0.0, -3.0), quadratic, normal_up=True)
+    sph = icosphere(level, 1.0, (0.0, 0.0, 0.0), quadratic)
+    mats = np.array([MEDIUM, PARTICLE], dtype=np.float64)
+    xyz, conn, bctype, bc, body = build_problem([(top, 1.0), (bot, -1.0)], [(sph, 6, 1)], mats, freq)
+    return Problem(xyz, conn, bctype, bc, mats, np.array([[1, 2]], dtype=np.uint32), float(freq),
+                   body, name or f"sphere-L{level}-plates-{n_plate}")
+
+
+def config1(**kw) -> Problem:
+    """N = 1892: icosphere level 2 + 12x12-grid plates."""
+    return sphere_between_plates(2, 12, name="config1", **kw)
+
+
+def config2(**kw) -> Problem:
+    """N = 15044: icosphere level 4 + 24x24-grid plates."""
+    return sphere_between_plates(4, 24, name="config2", **kw)
+
+
+def config3_geometry(**kw) -> Problem:
+    """N = 4740: icosphere level 3 + 16x16-grid plates (frequency sweep)."""
+    return sphere_between_plates(3, 16, name="config3", **kw)
+
+
+def sweep_frequencies(n: int = 64, f0: float = 1e3, f1: float = 1e8) -> np.ndarray:
+    return np.logspace(np.log10(f0), np.log10(f1), n)
+
+
+def quadrupole_trap(level: int, radius: float, center, freq: float = 1e6,
+                    n_grid: int = 16, quadratic: bool = True, name: str = "") -> Problem:
+    """Four 4x4 coplanar electrodes at z=0, centres (+-3, +-3), alternating
+    +-1 V, each an n_grid x n_grid quadratic grid, plus one sphere."""
+    cond = []
+    for cx, cy, v in ((3.0, 3.0, 1.0), (-3.0, 3.0, -1.0), (-3.0, -3.0, 1.0), (3.0, -3.0, -1.0)):
+        cond.append((plate(n_grid, 4.0, (cx, cy, 0.0), quadratic), v))
+    sph = icosphere(level, radius, center, quadratic)
+    mats = np.array([MEDIUM, PARTICLE], dtype=np.float64)
+    xyz, conn, bctype, bc, body = build_problem(cond, [(sph, 6, 1)], mats, freq)
+    return Problem(xyz, conn, bctype, bc, mats, np.array([[1, 2]], dtype=np.uint32), float(freq),
+                   body, name or f"quadrupole-L{level}")
+
+
+def config4_cases(n: int = 256, seed: int = 20261203, levels=(2, 3, 4)):
+    """Seeded (level, radius, centre) draws of the uneven particle-position batch."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        lvl = int(rng.choice(levels))
+        r = float(rng.uniform(0.5, 1.5))
+        c = (float(rng.uniform(-2, 2)), float(rng.uniform(-2, 2)), float(rng.uniform(1, 4)))
+        out.append((lvl, r, c))
+    return out
+
+
+def shelled_trap(particles, freq: float, n_grid: int = 16, quadratic: bool = True) -> Problem:
+    """Quadrupole trap with shelled particles.  ``particles`` is a list of
+    (level, radius, centre).  Materials: 1 medium, 2 shell, 3 core; the outer
+    surface of each particle is interface 1 (medium|shell, bc 6) and the inner
+    one interface 2 (shell|core, bc 0)."""
+    cond = []
+    for cx, cy, v in ((3.0, 3.0, 1.0), (-3.0, 3.0, -1.0), (-3.0, -3.0, 1.0), (3.0, -3.0, -1.0)):
+        cond.append((plate(n_grid, 4.0, (cx, cy, 0.0), quadratic), v))
+    parts = []
+    for lvl, r, c in particles:
+        parts.append((icosphere(lvl, r * 1.05, c, quadratic), 6, 1))
+        parts.append((icosphere(lvl, r, c, quadratic), 0, 2))
+    mats = np.array([MEDIUM, (1e-6, 10.0), (0.5, 50.0)], dtype=np.float64)
+    xyz, conn, bctype, bc, body = build_problem(cond, parts, mats, freq)
+    # both surfaces of particle p carry body id p+1
+    body = np.where(body > 0, (body + 1) // 2, 0).astype(np.int32)
+    return Problem(xyz, conn, bctype, bc, mats, np.array([[1, 2], [2, 3]], dtype=np.uint32),
+                   float(freq), body, "shelled-trap")
+
+
+# --------------------------------------------------------------------------
+# writers (formats of src/depSolver.c:98-252)
+# --------------------------------------------------------------------------
+
+def write_bem(dirpath: str, p: Problem, solver: str = "gaussBksb", analysis: int = 4,
+              internal_points: np.ndarray | None = None, gmres_precond: int = 1,
+              force_points: np.ndarray | None = None, radius: float = 1.0,
+              reposition=None) -> None:
+    os.makedirs(dirpath, exist_ok=True)
+    et = "tria6" if p.npe == 6 else "tria3"
+    with open(os.path.join(dirpath, "nodes.bem"), "w") as f:
+        for i, (x, y, z) in enumerate(p.xyz, 1):
+            f.write(f"{i}\t{x:.17e}\t{y:.17e}\t{z:.17e}\n")
+    with open(os.path.join(dirpath, "elems.bem"), "w") as f:
+        for i, row in enumerate(p.conn + 1, 1):
+            f.write(f"{i}\t" + "\t".join(str(int(v)) for v in row) + "\n")
+    n = p.n_nodes
+    with open(os.path.join(dirpath, "bcs.bem"), "w") as f:
+        for j in range(2 * n):
+            t = int(p.bctype[j])
+            i = j % n + 1
+            if t == 1:
+                f.write(f"{i}\t1\t{p.bc[j, 0]:.17e}\n")
+            elif t in (0, 6):
+                f.write(f"{i}\t{t}\t{p.bc[j, 0]:.17e}\t{int(p.bc[j, 1])}\n")
+            else:
+                f.write(f"{i}\t{t}\t{p.bc[j, 0]:.17e}\t{p.bc[j, 1]:.17e}\t{p.bc[j, 2]:.17e}\n")
+    lines = [f"NODES\n{n}\nnodes.bem\n", f"ELEMS\n{p.n_elems}\n{et}\nelems.bem\n",
+             f"MATERIALS\n{len(p.mats)}\n"]
+    for i, (s, e) in enumerate(p.mats, 1):
+        lines.append(f"{i}\t{s:.17e}\t{e:.17e}\n")
+    lines.append(f"\nINTERFACES\n{len(p.ifaces)}\n")
+    for i, (a, b) in enumerate(p.ifaces, 1):
+        lines.append(f"{i}\t{int(a)}\t{int(b)}\n")
+    lines.append(f"\nPROBLEMS\n{p.freq:.17e}\nbcs.bem\n")
+    if reposition is not None:
+        pn, d = reposition
+        lines.append(f"\nREPOSITION\n{pn}\n{d[0]:.17e}\t{d[1]:.17e}\t{d[2]:.17e}\n")
+    if solver.lower() == "gmres":
+        lines.append(f"\nANALYSIS\ngmres {gmres_precond} 0\n{analysis}\n")
+    else:
+        lines.append(f"\nANALYSIS\ngaussBksb\n{analysis}\n")
+    if analysis > 4 and force_points is not None:
+        lines.append(f"{len(force_points)} {radius:.17e} fpoints.bem\n")
+        with open(os.path.join(dirpath, "fpoints.bem"), "w") as f:
+            for i, (x, y, z) in enumerate(force_points, 1):
+                f.write(f"{i}\t{x:.17e}\t{y:.17e}\t{z:.17e}\n")
+    if internal_points is not None and len(internal_points):
+        lines.append(f"\nINTERNALPOINTS\n{len(internal_points)} STD\ninternal.bem\n")
+        with open(os.path.join(dirpath, "internal.bem"), "w") as f:
+            for i, (x, y, z) in enumerate(internal_points, 1):
+                f.write(f"{i}\t{x:.17e}\t{y:.17e}\t{z:.17e}\n")
+    else:
+        lines.append("\nINTERNALPOINTS\n0\n")
+    lines.append("\nCOLUMNS\n0\n")
+    with open(os.path.join(dirpath, "input.bem"), "w") as f:
+        f.write("".join(lines))
+
+
+def read_bem_dir(dirpath: str) -> Problem:
+    """Read nodes/elems/bcs referenced by an input.bem (STD sections only)."""
+    with open(os.path.join(dirpath, "input.bem")) as f:
+        txt = "\n".join(line.split("//")[0] for line in f)
+    tok = txt.split()
+    it = iter(tok)
+    next(it); n = int(next(it)); nfile = next(it)
+    next(it); e = int(next(it)); etype = next(it).lower(); efile = next(it)
+    next(it); nm = int(next(it))
+    mats = []
+    for _ in range(nm):
+        next(it); mats.append((float(next(it)), float(next(it))))
+    next(it); ni = int(next(it))
+    ifaces = []
+    for _ in range(ni):
+        next(it); ifaces.append((int(next(it)), int(next(it))))
+    next(it); freq = float(next(it)); bfile = next(it)
+    npe = 6 if etype.startswith("tria6") else 3
+
+    def rows(fn):
+        with open(os.path.join(dirpath, fn)) as fh:
+            return [l.split("//")[0].split() for l in fh if l.split("//")[0].strip()]
+
+    xyz = np.array([[float(v) for v in r[1:4]] for r in rows(nfile)[:n]])
+    conn = np.array([[int(v) - 1 for v in r[1:1 + npe]] for r in rows(efile)[:e]], dtype=np.int64)
+    bctype = np.zeros(2 * n, dtype=np.uint32)
+    bc = np.zeros((2 * n, 3))
+    for j, r in enumerate(rows(bfile)[:2 * n]):
+        t = r[1]
+        t = int(t[1:]) + 5 if t[0] in "cC" else int(t)
+        bctype[j] = t
+        vals = [float(v) for v in r[2:]]
+        bc[j, :len(vals)] = vals[:3]
+    return Problem(xyz, conn, bctype, bc, np.array(mats, dtype=np.float64),
+                   np.array(ifaces, dtype=np.uint32).reshape(-1, 2), freq, None,
+                   os.path.basename(dirpath.rstrip("/")))

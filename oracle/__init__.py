@@ -1,0 +1,253 @@
+"""ctypes bindings for the oracle (oracle/liboracle.so) and, where it has been
+built, the reference harness (oracle/_ref/libdepref.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke entry point and by
+bench.py's cpu_baseline leg, always as the checker.  The product never imports
+this package.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_ORC = os.path.join(HERE, "liboracle.so")
+_REF = os.path.join(HERE, "_ref", "libdepref.so")
+REF_SRC = "/root/reference/src"
+
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_c_int, _c_dbl = ctypes.c_int, ctypes.c_double
+
+
+def build(reference: bool = True) -> None:
+    """Compile liboracle.so and, when the reference sources are present,
+    oracle/_ref/ (make is incremental)."""
+    subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], check=True)
+    if reference and os.path.isdir(REF_SRC):
+        subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
+
+
+def _c64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _c32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _cu32(a):
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+class _Lib:
+    """Common wrappers over the two libraries (same calling conventions)."""
+
+    prefix = ""
+
+    def __init__(self, path):
+        self.path = path
+        self.lib = ctypes.CDLL(path)
+        p = self.prefix
+        L = self.lib
+        getattr(L, p + "intG").argtypes = [_c_int, _f64p, _f64p, _f64p]
+        getattr(L, p + "intH").argtypes = [_c_int, _c_int, _f64p, _f64p, _f64p, _f64p]
+        getattr(L, p + "intSingularG").argtypes = [_c_int, _c_int, _f64p, _f64p, _f64p]
+        getattr(L, p + "intSingularH").argtypes = [_c_int, _c_int, _c_int, _f64p, _f64p, _f64p, _f64p]
+
+    def intG(self, X, xe):
+        X = _c64(X); npe = X.shape[0]
+        out = np.zeros(npe)
+        getattr(self.lib, self.prefix + "intG")(npe, X, _c64(xe), out)
+        return out
+
+    def intH(self, X, xe, n, flag=0):
+        X = _c64(X); npe = X.shape[0]
+        out = np.zeros(npe * (3 if flag else 1))
+        getattr(self.lib, self.prefix + "intH")(npe, flag, X, _c64(xe), out, _c64(n))
+        return out
+
+    def intSingularG(self, X, xe, sin_node):
+        X = _c64(X); npe = X.shape[0]
+        out = np.zeros(npe)
+        getattr(self.lib, self.prefix + "intSingularG")(npe, sin_node, X, _c64(xe), out)
+        return out
+
+    def intSingularH(self, X, xe, n, sin_node, flag=0):
+        X = _c64(X); npe = X.shape[0]
+        out = np.zeros(npe * (3 if flag else 1))
+        getattr(self.lib, self.prefix + "intSingularH")(npe, flag, sin_node, X, _c64(xe), out, _c64(n))
+        return out
+
+
+class Oracle(_Lib):
+    prefix = "orc_"
+
+    def __init__(self, path=_ORC):
+        super().__init__(path)
+        L = self.lib
+        L.orc_node_normal.argtypes = [_c_int, _c_int, _f64p, _c_int, _i32p, _f64p]
+        L.orc_iface_weights.argtypes = [_c_int, _u32p, _f64p, _c_dbl, _f64p, _f64p]
+        L.orc_assemble.argtypes = [_c_int, _c_int, _c_int, _f64p, _c_int, _i32p, _u32p, _f64p,
+                                   _c_int, _u32p, _f64p, _c_dbl, _f64p, _f64p, _c_int, _c_int]
+        L.orc_assemble.restype = _c_int
+        L.orc_gauss_bksb.argtypes = [_c_int, _f64p, _f64p, _c_int]
+        L.orc_gauss_bksb.restype = _c_int
+        L.orc_gmres_el.argtypes = [_c_int, _c_int, ctypes.c_void_p, _f64p, _f64p, _c_dbl, _c_int,
+                                   ctypes.c_char_p, ctypes.POINTER(_c_int)]
+        L.orc_gmres_el.restype = _c_int
+        L.orc_potential.argtypes = [_c_int, _c_int, _f64p, _c_int, _i32p, _f64p, _f64p, _f64p]
+        L.orc_field.argtypes = [_c_int, _c_int, _f64p, _c_int, _i32p, _f64p, _f64p, _f64p]
+        L.orc_force_mst.argtypes = [_c_int, _c_int, _f64p, _c_int, _i32p, _u32p, _f64p, _c_dbl,
+                                    _f64p, _f64p, _f64p, _f64p, _c_int]
+        L.orc_force_mst.restype = _c_int
+        L.orc_force_multipole.argtypes = [_c_int, _c_int, _c_dbl, _f64p, _c_int, _f64p, _c_int,
+                                          _i32p, _f64p, _f64p, _c_dbl, _f64p]
+
+    def node_normal(self, p, node):
+        n = np.zeros(3)
+        self.lib.orc_node_normal(p.npe, node, _c64(p.xyz), p.n_elems, _c32(p.conn), n)
+        return n
+
+    def iface_weights(self, p, freq=None):
+        nI = len(p.ifaces)
+        A, B = np.zeros(max(nI, 1)), np.zeros(max(nI, 1))
+        self.lib.orc_iface_weights(nI, _cu32(p.ifaces.reshape(-1)), _c64(p.mats), p.freq if freq is None else freq, A, B)
+        return A[:nI], B[:nI]
+
+    def assemble(self, p, full=True, rows=None):
+        N = p.n_nodes
+        mA = np.zeros((2 * N if full else N, 2 * N))
+        vB = np.zeros(2 * N)
+        r0, r1 = (0, -1) if rows is None else rows
+        rc = self.lib.orc_assemble(p.npe, int(full), N, _c64(p.xyz), p.n_elems, _c32(p.conn),
+                                   _cu32(p.bctype), _c64(p.bc), len(p.ifaces),
+                                   _cu32(p.ifaces.reshape(-1)), _c64(p.mats), p.freq, mA, vB, r0, r1)
+        if rc != 0:
+            raise RuntimeError("orc_assemble failed")
+        return mA, vB
+
+    def gauss_bksb(self, A, B, max_pivots=-1):
+        A = _c64(A).copy(); B = _c64(B).copy()
+        rc = self.lib.orc_gauss_bksb(A.shape[0], A, B, max_pivots)
+        if rc != 0:
+            raise RuntimeError("singular system")
+        return B
+
+    def gmres_el(self, A, B, precond=1, tol=1e-6, max_iters=600, log_path=None, x0=None):
+        A = _c64(A).copy(); B = _c64(B).copy()
+        it = _c_int(0)
+        x0p = None if x0 is None else _c64(x0).ctypes.data
+        rc = self.lib.orc_gmres_el(A.shape[0], precond, x0p, A, B, tol, max_iters,
+                                   None if log_path is None else log_path.encode(), ctypes.byref(it))
+        return rc, B, it.value
+
+    def potential(self, p, sigma, x):
+        out = np.zeros(2)
+        self.lib.orc_potential(p.npe, p.n_nodes, _c64(p.xyz), p.n_elems, _c32(p.conn), _c64(sigma), _c64(x), out)
+        return out
+
+    def field(self, p, sigma, x):
+        out = np.zeros(6)
+        self.lib.orc_field(p.npe, p.n_nodes, _c64(p.xyz), p.n_elems, _c32(p.conn), _c64(sigma), _c64(x), out)
+        return out
+
+    def force_mst(self, p, sigma, eps=None, max_field_points=-1):
+        E = p.n_elems
+        Fce, Xce, Fcm, Xcm = np.zeros((E, 3)), np.zeros((E, 3)), np.zeros(3), np.zeros(3)
+        eps = p.mats[0, 1] * 8.854187817e-12 if eps is None else eps
+        self.lib.orc_force_mst(p.npe, p.n_nodes, _c64(p.xyz), E, _c32(p.conn), _cu32(p.bctype),
+                               _c64(sigma), eps, Fce, Xce, Fcm, Xcm, max_field_points)
+        return Fce, Xce, Fcm, Xcm
+
+    def force_multipole(self, p, sigma, x, R, order=3):
+        F = np.zeros(3)
+        self.lib.orc_force_multipole(p.npe, order, R, _c64(x), p.n_nodes, _c64(p.xyz), p.n_elems,
+                                     _c32(p.conn), _c64(sigma), _c64(p.mats), p.freq, F)
+        return F
+
+
+class Reference(_Lib):
+    """The reference's own functions (oracle/_ref/libdepref.so)."""
+    prefix = "ref_"
+
+    def __init__(self, path=_REF):
+        super().__init__(path)
+        L = self.lib
+        L.ref_node_normal.argtypes = [_c_int, _c_int, _c_int, _f64p, _c_int, _i32p, _f64p]
+        L.ref_assemble.argtypes = [_c_int, _c_int, _c_int, _f64p, _c_int, _i32p, _u32p, _f64p,
+                                   _c_int, _u32p, _c_int, _f64p, _c_dbl, _f64p, _f64p]
+        L.ref_gauss_bksb.argtypes = [_c_int, _f64p, _f64p]
+        L.ref_gmres.argtypes = [_c_int, _c_int, _f64p, _f64p]
+        L.ref_potential.argtypes = [_c_int, _c_int, _f64p, _c_int, _i32p, _f64p, _f64p, _f64p]
+        L.ref_field.argtypes = [_c_int, _c_int, _f64p, _c_int, _i32p, _f64p, _f64p, _f64p]
+        L.ref_force_mst.argtypes = [_c_int, _c_int, _f64p, _c_int, _i32p, _u32p, _f64p, _c_dbl,
+                                    _f64p, _f64p, _f64p, _f64p]
+        L.ref_force_multipole.argtypes = [_c_int, _c_int, _c_dbl, _f64p, _c_int, _f64p, _c_int,
+                                          _i32p, _f64p, _c_int, _f64p, _c_dbl, _f64p]
+
+    def node_normal(self, p, node):
+        n = np.zeros(3)
+        self.lib.ref_node_normal(p.npe, node, p.n_nodes, _c64(p.xyz), p.n_elems, _c32(p.conn), n)
+        return n
+
+    def assemble(self, p, full=True):
+        N = p.n_nodes
+        mA = np.zeros((2 * N if full else N, 2 * N))
+        vB = np.zeros(2 * N)
+        self.lib.ref_assemble(p.npe, int(full), N, _c64(p.xyz), p.n_elems, _c32(p.conn),
+                              _cu32(p.bctype), _c64(p.bc), len(p.ifaces), _cu32(p.ifaces.reshape(-1)),
+                              len(p.mats), _c64(p.mats), p.freq, mA, vB)
+        return mA, vB
+
+    def gauss_bksb(self, A, B):
+        A = _c64(A).copy(); B = _c64(B).copy()
+        self.lib.ref_gauss_bksb(A.shape[0], A, B)
+        return B
+
+    def gmres(self, A, B, precond=1):
+        A = _c64(A).copy(); B = _c64(B).copy()
+        self.lib.ref_gmres(A.shape[0], precond, A, B)
+        return B
+
+    def potential(self, p, sigma, x):
+        out = np.zeros(2)
+        self.lib.ref_potential(p.npe, p.n_nodes, _c64(p.xyz), p.n_elems, _c32(p.conn), _c64(sigma), _c64(x), out)
+        return out
+
+    def field(self, p, sigma, x):
+        out = np.zeros(6)
+        self.lib.ref_field(p.npe, p.n_nodes, _c64(p.xyz), p.n_elems, _c32(p.conn), _c64(sigma), _c64(x), out)
+        return out
+
+    def force_mst(self, p, sigma, eps=None):
+        E = p.n_elems
+        Fce, Xce, Fcm, Xcm = np.zeros((E, 3)), np.zeros((E, 3)), np.zeros(3), np.zeros(3)
+        eps = p.mats[0, 1] * 8.854187817e-12 if eps is None else eps
+        self.lib.ref_force_mst(p.npe, p.n_nodes, _c64(p.xyz), E, _c32(p.conn), _cu32(p.bctype),
+                               _c64(sigma), eps, Fce, Xce, Fcm, Xcm)
+        return Fce, Xce, Fcm, Xcm
+
+    def force_multipole(self, p, sigma, x, R, order=3):
+        F = np.zeros(3)
+        self.lib.ref_force_multipole(p.npe, order, R, _c64(x), p.n_nodes, _c64(p.xyz), p.n_elems,
+                                     _c32(p.conn), _c64(sigma), len(p.mats), _c64(p.mats), p.freq, F)
+        return F
+
+
+def oracle() -> Oracle:
+    if not os.path.exists(_ORC):
+        build(reference=False)
+    return Oracle()
+
+
+def reference_available() -> bool:
+    return os.path.exists(_REF)
+
+
+def reference() -> Reference:
+    return Reference()

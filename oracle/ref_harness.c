@@ -1,0 +1,247 @@
+/*
+ * ref_harness.c -- flat-array entry points onto the reference build.
+ *
+ * TEST INFRASTRUCTURE ONLY.  oracle/Makefile compiles the reference's own
+ * sources where they lie (never copied here) and links them with this file
+ * into oracle/_ref/libdepref.so.  The tests use it to pin the oracle
+ * (oracle.c) and to write golden vectors; nothing on the product path loads
+ * it.
+ *
+ * The reference keeps its sizes in globals defined by src/depSolver.c:46-50
+ * and its quadrature tables in src/gaussData.h (a header that defines data);
+ * both are provided here because depSolver.c's main() is not linked in.
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "gaussData.h"
+#include "assembly_el.h"
+#include "field.h"
+#include "force.h"
+#include "gaussBksb.h"
+#include "gmres_el.h"
+#include "kernelIntegrals.h"
+#include "normals.h"
+#include "potential.h"
+
+FILE *file_log;
+char cElemType[6];
+unsigned int nDim, nElems, nElemType, nFPoints, nInternalPoints, nInterfaces, nMats, nNodes,
+    nNodesAux, nNodesInElem, nCols, nColType, NX, NY, NZ;
+double deltaX, deltaY, deltaZ;
+
+static double **rows_of(double *flat, long nrows, long ncols)
+{
+    double **r = (double **)malloc(sizeof(double *) * (nrows > 0 ? nrows : 1));
+    for (long i = 0; i < nrows; i++) r[i] = flat + i * ncols;
+    return r;
+}
+
+static double **copy_rows(const double *flat, long nrows, long ncols)
+{
+    double **r = (double **)malloc(sizeof(double *) * (nrows > 0 ? nrows : 1));
+    for (long i = 0; i < nrows; i++) {
+        r[i] = (double *)malloc(sizeof(double) * ncols);
+        memcpy(r[i], flat + i * ncols, sizeof(double) * ncols);
+    }
+    return r;
+}
+
+static unsigned **conn1(const int *conn0, long E, int npe)
+{
+    unsigned **r = (unsigned **)malloc(sizeof(unsigned *) * (E > 0 ? E : 1));
+    for (long e = 0; e < E; e++) {
+        r[e] = (unsigned *)malloc(sizeof(unsigned) * npe);
+        for (int k = 0; k < npe; k++) r[e][k] = (unsigned)(conn0[e * npe + k] + 1);
+    }
+    return r;
+}
+
+static void free_rows(void **r, long n)
+{
+    for (long i = 0; i < n; i++) free(r[i]);
+    free(r);
+}
+
+static void set_sizes(int N, int E, int nI)
+{
+    nNodes = (unsigned)N;
+    nElems = (unsigned)E;
+    nInterfaces = (unsigned)nI;
+    if (!file_log) file_log = fopen("/dev/null", "w");
+}
+
+void ref_intG(int npe, const double *X, const double *xe, double *out)
+{
+    double Xl[6][3], x[3] = {xe[0], xe[1], xe[2]};
+    memcpy(Xl, X, sizeof(double) * 3 * npe);
+    if (npe == 3) intG_tria3((double(*)[3])Xl, x, out);
+    else intG_tria6(Xl, x, out);
+}
+
+void ref_intH(int npe, int flag, const double *X, const double *xe, double *out, const double *n)
+{
+    double Xl[6][3], x[3] = {xe[0], xe[1], xe[2]}, nn[3] = {n[0], n[1], n[2]};
+    memcpy(Xl, X, sizeof(double) * 3 * npe);
+    if (npe == 3) intH_tria3((unsigned)flag, (double(*)[3])Xl, x, out, nn);
+    else intH_tria6((unsigned)flag, Xl, x, out, nn);
+}
+
+void ref_intSingularG(int npe, int sn, const double *X, const double *xe, double *out)
+{
+    double Xl[6][3], x[3] = {xe[0], xe[1], xe[2]};
+    memcpy(Xl, X, sizeof(double) * 3 * npe);
+    if (npe == 3) intSingularG_tria3((unsigned)sn, (double(*)[3])Xl, x, out);
+    else intSingularG_tria6((unsigned)sn, Xl, x, out);
+}
+
+void ref_intSingularH(int npe, int flag, int sn, const double *X, const double *xe, double *out,
+                      const double *n)
+{
+    double Xl[6][3], x[3] = {xe[0], xe[1], xe[2]}, nn[3] = {n[0], n[1], n[2]};
+    memcpy(Xl, X, sizeof(double) * 3 * npe);
+    if (npe == 3) intSingularH_tria3((unsigned)flag, (unsigned)sn, (double(*)[3])Xl, x, out, nn);
+    else intSingularH_tria6((unsigned)flag, (unsigned)sn, Xl, x, out, nn);
+}
+
+void ref_node_normal(int npe, int node0, int N, const double *xyz, int E, const int *conn0,
+                     double *n)
+{
+    set_sizes(N, E, 0);
+    double **nodes = copy_rows(xyz, N, 3);
+    unsigned **el = conn1(conn0, E, npe);
+    if (npe == 3) getNormal_tria3((unsigned)(node0 + 1), nodes, el, n);
+    else getNormal_tria6((unsigned)(node0 + 1), nodes, el, n);
+    free_rows((void **)nodes, N);
+    free_rows((void **)el, E);
+}
+
+/* mA must be zeroed by the caller: 2N x 2N (full) or N x 2N (GMRES layout) */
+void ref_assemble(int npe, int full, int N, const double *xyz, int E, const int *conn0,
+                  const unsigned *bctype, const double *bc, int nI, const unsigned *ifaces, int nM,
+                  const double *mats, double freq, double *mA, double *vB)
+{
+    set_sizes(N, E, nI);
+    double **nodes = copy_rows(xyz, N, 3);
+    unsigned **el = conn1(conn0, E, npe);
+    double **mbc = copy_rows(bc, 2L * N, 3);
+    double **mat = copy_rows(mats, nM, 2);
+    unsigned *bct = (unsigned *)malloc(sizeof(unsigned) * 2 * N);
+    memcpy(bct, bctype, sizeof(unsigned) * 2 * N);
+    unsigned **ifc = NULL;
+    if (nI > 0) {
+        ifc = (unsigned **)malloc(sizeof(unsigned *) * nI);
+        for (int i = 0; i < nI; i++) {
+            ifc[i] = (unsigned *)malloc(sizeof(unsigned) * 2);
+            ifc[i][0] = ifaces[2 * i];
+            ifc[i][1] = ifaces[2 * i + 1];
+        }
+    }
+    double prob[1] = {freq};
+    double **A = rows_of(mA, full ? 2L * N : (long)N, 2L * N);
+    if (npe == 3) {
+        if (full) electricFormFullA_tria3(nodes, el, bct, ifc, mat, prob, A, mbc, vB);
+        else electricFormA_tria3(nodes, el, bct, ifc, mat, prob, A, mbc, vB);
+    } else {
+        if (full) electricFormFullA_tria6(nodes, el, bct, ifc, mat, prob, A, mbc, vB);
+        else electricFormA_tria6(nodes, el, bct, ifc, mat, prob, A, mbc, vB);
+    }
+    free(A);
+    free(bct);
+    if (ifc) free_rows((void **)ifc, nI);
+    free_rows((void **)nodes, N);
+    free_rows((void **)el, E);
+    free_rows((void **)mbc, 2L * N);
+    free_rows((void **)mat, nM);
+}
+
+void ref_gauss_bksb(int n, double *A, double *B)
+{
+    double **r = rows_of(A, n, n);
+    gaussBksb((unsigned)n, r, B);
+    free(r);
+}
+
+/* Runs the reference GMRES (tolerance 1e-6, writes ./gmres.log) on the
+ * N x 2N layout. */
+void ref_gmres(int N, int precond, double *A, double *B)
+{
+    set_sizes(N, 0, 0);
+    double **r = rows_of(A, N, 2L * N);
+    solverGMRES_el((unsigned)precond, 0, r, B);
+    free(r);
+}
+
+void ref_potential(int npe, int N, const double *xyz, int E, const int *conn0,
+                   const double *sigma, const double *x, double *pot)
+{
+    set_sizes(N, E, 0);
+    double **nodes = copy_rows(xyz, N, 3);
+    unsigned **el = conn1(conn0, E, npe);
+    double xi[3] = {x[0], x[1], x[2]};
+    double *s = (double *)malloc(sizeof(double) * 2 * N);
+    memcpy(s, sigma, sizeof(double) * 2 * N);
+    if (npe == 3) potential_tria3(xi, nodes, el, s, pot);
+    else potential_tria6(xi, nodes, el, s, pot);
+    free(s);
+    free_rows((void **)nodes, N);
+    free_rows((void **)el, E);
+}
+
+void ref_field(int npe, int N, const double *xyz, int E, const int *conn0, const double *sigma,
+               const double *x, double *ef)
+{
+    set_sizes(N, E, 0);
+    double **nodes = copy_rows(xyz, N, 3);
+    unsigned **el = conn1(conn0, E, npe);
+    double xi[3] = {x[0], x[1], x[2]};
+    double *s = (double *)malloc(sizeof(double) * 2 * N);
+    memcpy(s, sigma, sizeof(double) * 2 * N);
+    if (npe == 3) field_tria3(xi, nodes, el, s, ef);
+    else field_tria6(xi, nodes, el, s, ef);
+    free(s);
+    free_rows((void **)nodes, N);
+    free_rows((void **)el, E);
+}
+
+void ref_force_mst(int npe, int N, const double *xyz, int E, const int *conn0,
+                   const unsigned *bctype, const double *sigma, double eps, double *Fce,
+                   double *Xce, double *Fcm, double *Xcm)
+{
+    set_sizes(N, E, 0);
+    double **nodes = copy_rows(xyz, N, 3);
+    unsigned **el = conn1(conn0, E, npe);
+    unsigned *bct = (unsigned *)malloc(sizeof(unsigned) * 2 * N);
+    memcpy(bct, bctype, sizeof(unsigned) * 2 * N);
+    double *s = (double *)malloc(sizeof(double) * 2 * N);
+    memcpy(s, sigma, sizeof(double) * 2 * N);
+    double **fce = rows_of(Fce, E, 3), **xce = rows_of(Xce, E, 3);
+    if (npe == 3) forceMST_tria3(nodes, el, bct, s, eps, fce, xce, Fcm, Xcm);
+    else forceMST_tria6(nodes, el, bct, s, eps, fce, xce, Fcm, Xcm);
+    free(fce);
+    free(xce);
+    free(s);
+    free(bct);
+    free_rows((void **)nodes, N);
+    free_rows((void **)el, E);
+}
+
+void ref_force_multipole(int npe, int order, double R, const double *x, int N, const double *xyz,
+                         int E, const int *conn0, const double *sigma, int nM, const double *mats,
+                         double freq, double *F)
+{
+    set_sizes(N, E, 0);
+    double **nodes = copy_rows(xyz, N, 3);
+    unsigned **el = conn1(conn0, E, npe);
+    double **mat = copy_rows(mats, nM, 2);
+    double xi[3] = {x[0], x[1], x[2]}, prob[1] = {freq};
+    double *s = (double *)malloc(sizeof(double) * 2 * N);
+    memcpy(s, sigma, sizeof(double) * 2 * N);
+    if (npe == 3) forceMultipole_tria3((unsigned)order, R, xi, nodes, el, s, mat, prob, F);
+    else forceMultipole_tria6((unsigned)order, R, xi, nodes, el, s, mat, prob, F);
+    free(s);
+    free_rows((void **)nodes, N);
+    free_rows((void **)el, E);
+    free_rows((void **)mat, nM);
+}
