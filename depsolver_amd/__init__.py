@@ -1,0 +1,237 @@
+"""depsolver_amd -- MI355X (gfx950) BEM hot path of depSolver.
+
+The compute path is libdepsolver_amd.so (HIP kernels behind a C ABI, see
+include/depsolver_amd.h).  This package is a thin ctypes mirror of that ABI
+for tests, benchmarks and scripting; it never computes on the CPU.  Importing
+it does not touch the GPU; creating a :class:`Context` does, and fails loudly
+when the native library is missing.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from . import mesh  # noqa: F401
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdepsolver_amd.so")
+
+DEP_OK = 0
+ERRORS = {-1: "DEP_ERR_ARG", -2: "DEP_ERR_HIP", -3: "DEP_ERR_NOMEM", -4: "DEP_ERR_SINGULAR",
+          -5: "DEP_ERR_NOCONV", -6: "DEP_ERR_STATE", -7: "DEP_ERR_ADMISSION"}
+PREC = {"fp64": 0, "fp32": 1, "fp16": 2, "bf16": 3}
+
+
+class DepError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class SolveInfo(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("inner_iterations", ctypes.c_int),
+                ("fallback", ctypes.c_int), ("converged", ctypes.c_int),
+                ("residual", ctypes.c_double), ("min_pivot", ctypes.c_double),
+                ("factor_ms", ctypes.c_double), ("solve_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+_p = ctypes.c_void_p
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+
+
+def lib():
+    """Load libdepsolver_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() "
+                           "(make -C depsolver_amd/csrc) first")
+    L = ctypes.CDLL(LIB_PATH)
+    i, d = ctypes.c_int, ctypes.c_double
+    sig = {
+        "dep_ctx_create": ([i, ctypes.POINTER(_p)], i),
+        "dep_ctx_destroy": ([_p], i),
+        "dep_last_error": ([_p], ctypes.c_char_p),
+        "dep_ctx_synchronize": ([_p], i),
+        "dep_ctx_device_bytes": ([_p], ctypes.c_size_t),
+        "dep_geom_create": ([_p, i, _f64p, i, i, _i32p, _u32p, _i32p, ctypes.POINTER(_p)], i),
+        "dep_geom_destroy": ([_p], i),
+        "dep_geom_assemble": ([_p], i),
+        "dep_geom_get_blocks": ([_p, _f64p], i),
+        "dep_sys_create": ([_p, i, _f64p, i, _u32p, d, _f64p, ctypes.POINTER(_p)], i),
+        "dep_sys_destroy": ([_p], i),
+        "dep_sys_set_frequency": ([_p, d], i),
+        "dep_sys_get_matrix": ([_p, i, _f64p], i),
+        "dep_factor": ([_p, i], i),
+        "dep_solve": ([_p, d, i, _p, ctypes.POINTER(SolveInfo)], i),
+        "dep_gmres": ([_p, i, d, i, _p, _p, ctypes.POINTER(SolveInfo), ctypes.c_char_p], i),
+        "dep_get_solution": ([_p, _f64p], i),
+        "dep_set_solution": ([_p, _f64p], i),
+        "dep_post_potential": ([_p, i, _f64p, _f64p], i),
+        "dep_post_field": ([_p, i, _f64p, _f64p], i),
+        "dep_post_mst": ([_p, d, _f64p, _f64p, _f64p, _f64p], i),
+        "dep_post_multipole": ([_p, i, d, i, _f64p, _f64p], i),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def _c64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Context:
+    """One device, one stream (dep_ctx)."""
+
+    def __init__(self, device: int = 0):
+        self._h = _p()
+        rc = lib().dep_ctx_create(device, ctypes.byref(self._h))
+        if rc != DEP_OK:
+            raise DepError(rc, f"dep_ctx_create(device={device}) failed")
+
+    def check(self, rc, what=""):
+        if rc != DEP_OK:
+            raise DepError(rc, f"{what}: {lib().dep_last_error(self._h).decode()}")
+
+    @property
+    def device_bytes(self) -> int:
+        return int(lib().dep_ctx_device_bytes(self._h))
+
+    def synchronize(self):
+        self.check(lib().dep_ctx_synchronize(self._h), "synchronize")
+
+    def close(self):
+        if self._h:
+            lib().dep_ctx_destroy(self._h)
+            self._h = _p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class Geometry:
+    """Device-resident mesh and geometric blocks (dep_geom)."""
+
+    def __init__(self, ctx: Context, problem):
+        self.ctx = ctx
+        self.problem = problem
+        n = problem.n_nodes
+        bct = np.ascontiguousarray(problem.bctype[:n], dtype=np.uint32)
+        iface = np.ascontiguousarray(problem.bc[:n, 1].astype(np.int64) - 1, dtype=np.int32)
+        iface[(bct != 0) & (bct != 6)] = -1
+        self._h = _p()
+        ctx.check(lib().dep_geom_create(ctx._h, n, _c64(problem.xyz), problem.n_elems, problem.npe,
+                                        np.ascontiguousarray(problem.conn, dtype=np.int32), bct,
+                                        iface, ctypes.byref(self._h)), "dep_geom_create")
+
+    def assemble(self):
+        self.ctx.check(lib().dep_geom_assemble(self._h), "dep_geom_assemble")
+
+    def blocks(self) -> np.ndarray:
+        n = self.problem.n_nodes
+        P = np.zeros((n, n))
+        self.ctx.check(lib().dep_geom_get_blocks(self._h, P), "dep_geom_get_blocks")
+        return P
+
+    def close(self):
+        if self._h:
+            lib().dep_geom_destroy(self._h)
+            self._h = _p()
+
+
+class System:
+    """Materials + frequency + RHS on a geometry; factors and solution (dep_sys)."""
+
+    def __init__(self, geom: Geometry, freq: float | None = None, rhs=None):
+        p = geom.problem
+        self.geom, self.ctx, self.n = geom, geom.ctx, p.n_nodes
+        rhs = p.rhs() if rhs is None else _c64(rhs)
+        self._h = _p()
+        self.ctx.check(lib().dep_sys_create(geom._h, len(p.mats), _c64(p.mats), len(p.ifaces),
+                                            np.ascontiguousarray(p.ifaces.reshape(-1), dtype=np.uint32),
+                                            float(p.freq if freq is None else freq), rhs,
+                                            ctypes.byref(self._h)), "dep_sys_create")
+
+    def set_frequency(self, f):
+        self.ctx.check(lib().dep_sys_set_frequency(self._h, float(f)), "dep_sys_set_frequency")
+
+    def matrix(self, layout=0) -> np.ndarray:
+        n = self.n
+        A = np.zeros((2 * n if layout == 0 else n, 2 * n))
+        self.ctx.check(lib().dep_sys_get_matrix(self._h, layout, A), "dep_sys_get_matrix")
+        return A
+
+    def factor(self, precision="fp64"):
+        self.ctx.check(lib().dep_factor(self._h, PREC[precision]), "dep_factor")
+
+    def solve(self, tol=1e-12, max_steps=30):
+        sigma = np.zeros(2 * self.n)
+        info = SolveInfo()
+        self.ctx.check(lib().dep_solve(self._h, tol, max_steps, sigma.ctypes.data, ctypes.byref(info)),
+                       "dep_solve")
+        return sigma, info.as_dict()
+
+    def gmres(self, precond=1, tol=1e-6, max_iters=600, x0=None, log_path=None):
+        sigma = np.zeros(2 * self.n)
+        info = SolveInfo()
+        x0p = None if x0 is None else _c64(x0).ctypes.data
+        self.ctx.check(lib().dep_gmres(self._h, precond, tol, max_iters, x0p, sigma.ctypes.data,
+                                       ctypes.byref(info), None if log_path is None else log_path.encode()),
+                       "dep_gmres")
+        return sigma, info.as_dict()
+
+    def solution(self):
+        s = np.zeros(2 * self.n)
+        self.ctx.check(lib().dep_get_solution(self._h, s), "dep_get_solution")
+        return s
+
+    def set_solution(self, sigma):
+        self.ctx.check(lib().dep_set_solution(self._h, _c64(sigma)), "dep_set_solution")
+
+    def potential(self, pts):
+        pts = _c64(pts).reshape(-1, 3)
+        out = np.zeros((len(pts), 2))
+        self.ctx.check(lib().dep_post_potential(self._h, len(pts), pts, out), "dep_post_potential")
+        return out
+
+    def field(self, pts):
+        pts = _c64(pts).reshape(-1, 3)
+        out = np.zeros((len(pts), 6))
+        self.ctx.check(lib().dep_post_field(self._h, len(pts), pts, out), "dep_post_field")
+        return out
+
+    def mst(self, eps=None):
+        p = self.geom.problem
+        E = p.n_elems
+        eps = p.mats[0, 1] * 8.854187817e-12 if eps is None else eps
+        Fce, Xce, Fcm, Xcm = np.zeros((E, 3)), np.zeros((E, 3)), np.zeros(3), np.zeros(3)
+        self.ctx.check(lib().dep_post_mst(self._h, eps, Fce, Xce, Fcm, Xcm), "dep_post_mst")
+        return Fce, Xce, Fcm, Xcm
+
+    def multipole(self, pts, radius, order=3):
+        pts = _c64(pts).reshape(-1, 3)
+        F = np.zeros((len(pts), 3))
+        self.ctx.check(lib().dep_post_multipole(self._h, order, radius, len(pts), pts, F),
+                       "dep_post_multipole")
+        return F
+
+    def close(self):
+        if self._h:
+            lib().dep_sys_destroy(self._h)
+            self._h = _p()

@@ -1,0 +1,964 @@
+// dep_api.cpp -- the C ABI of libdepsolver_amd.so (include/depsolver_amd.h).
+//
+// Host side of the hot path: input conversion (0-based ids, Morton node order,
+// incidence and patch lists), device object lifetime, launch sequencing and
+// the FP64 refinement loop.  All arithmetic on matrices runs in the kernels.
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <numeric>
+#include <unordered_map>
+
+#include "dep_common.h"
+#include "dep_internal.h"
+#include "dep_post.h"
+
+static const double kC = 1.0 / (DEP_PI4 * DEP_EPS0);
+
+int dep_fail(dep_ctx* c, int code, const char* fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+int dep_malloc(dep_ctx* c, void** p, size_t bytes)
+{
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *p = nullptr;
+        return dep_fail(c, DEP_ERR_NOMEM, "hipMalloc(%zu bytes) failed: %s", bytes,
+                        hipGetErrorString(e));
+    }
+    c->bytes += bytes;
+    return DEP_OK;
+}
+
+void dep_free(dep_ctx* c, void* p, size_t bytes)
+{
+    if (!p) return;
+    (void)hipFree(p);
+    c->bytes -= bytes;
+}
+
+template <class Obj>
+static void free_all(Obj* o)
+{
+    dep_ctx* c = o->ctx_of();
+    for (auto& a : o->allocs) dep_free(c, a.first, a.second);
+    o->allocs.clear();
+}
+
+template <class T>
+static int upload(dep_ctx* c, T* dst, const T* src, size_t count)
+{
+    DEP_HIP(c, hipMemcpyAsync(dst, src, sizeof(T) * count, hipMemcpyHostToDevice, c->stream));
+    return DEP_OK;
+}
+
+template <class T>
+static int download(dep_ctx* c, T* dst, const T* src, size_t count)
+{
+    DEP_HIP(c, hipMemcpyAsync(dst, src, sizeof(T) * count, hipMemcpyDeviceToHost, c->stream));
+    DEP_HIP(c, hipStreamSynchronize(c->stream));
+    return DEP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+
+extern "C" int dep_ctx_create(int device, dep_ctx** out)
+{
+    if (!out) return DEP_ERR_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return DEP_ERR_HIP;
+    if (device < 0 || device >= n) return DEP_ERR_ARG;
+    dep_ctx* c = new dep_ctx();
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        delete c;
+        return DEP_ERR_HIP;
+    }
+    *out = c;
+    return DEP_OK;
+}
+
+extern "C" int dep_ctx_destroy(dep_ctx* c)
+{
+    if (!c) return DEP_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return DEP_OK;
+}
+
+extern "C" const char* dep_last_error(const dep_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+extern "C" int dep_ctx_synchronize(dep_ctx* c)
+{
+    if (!c) return DEP_ERR_ARG;
+    DEP_HIP(c, hipStreamSynchronize(c->stream));
+    return DEP_OK;
+}
+
+extern "C" size_t dep_ctx_device_bytes(const dep_ctx* c) { return c ? c->bytes : 0; }
+
+// ---------------------------------------------------------------------------
+// geometry
+// ---------------------------------------------------------------------------
+
+static uint64_t spread3(uint64_t v)
+{
+    v &= 0x1fffff;
+    v = (v | v << 32) & 0x1f00000000ffffULL;
+    v = (v | v << 16) & 0x1f0000ff0000ffULL;
+    v = (v | v << 8) & 0x100f00f00f00f00fULL;
+    v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
+    v = (v | v << 2) & 0x1249249249249249ULL;
+    return v;
+}
+
+// Morton order of the node coordinates: consecutive internal ids are close in
+// space, so a column patch of DEP_TC ids touches few elements.
+static void morton_order(int N, const double* xyz, std::vector<int>& perm)
+{
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int i = 0; i < N; i++)
+        for (int c = 0; c < 3; c++) {
+            lo[c] = std::min(lo[c], xyz[3 * i + c]);
+            hi[c] = std::max(hi[c], xyz[3 * i + c]);
+        }
+    double ext = 0.0;
+    for (int c = 0; c < 3; c++) ext = std::max(ext, hi[c] - lo[c]);
+    if (ext <= 0.0) ext = 1.0;
+    std::vector<uint64_t> code(N);
+    for (int i = 0; i < N; i++) {
+        uint64_t m = 0;
+        for (int c = 0; c < 3; c++) {
+            const double f = (xyz[3 * i + c] - lo[c]) / ext;
+            const uint64_t q = (uint64_t)std::min(2097151.0, std::max(0.0, f * 2097151.0));
+            m |= spread3(q) << c;
+        }
+        code[i] = m;
+    }
+    perm.resize(N);
+    std::iota(perm.begin(), perm.end(), 0);
+    std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return code[a] < code[b]; });
+}
+
+extern "C" int dep_geom_create(dep_ctx* c, int N, const double* xyz, int E, int npe,
+                               const int32_t* conn, const uint32_t* bctype, const int32_t* iface,
+                               dep_geom** out)
+{
+    if (!c || !out || !xyz || !conn || !bctype || N <= 0 || E <= 0 || (npe != 3 && npe != 6))
+        return dep_fail(c, DEP_ERR_ARG, "dep_geom_create: bad arguments (N=%d E=%d npe=%d)", N, E, npe);
+    for (long t = 0; t < (long)E * npe; t++)
+        if (conn[t] < 0 || conn[t] >= N)
+            return dep_fail(c, DEP_ERR_ARG, "dep_geom_create: element %ld references node %d outside [0,%d)",
+                            t / npe, conn[t], N);
+    for (int i = 0; i < N; i++)
+        if (!std::isfinite(xyz[3 * i]) || !std::isfinite(xyz[3 * i + 1]) || !std::isfinite(xyz[3 * i + 2]))
+            return dep_fail(c, DEP_ERR_ARG, "dep_geom_create: node %d has non-finite coordinates", i);
+    DEP_HIP(c, hipSetDevice(c->device));
+    dep_geom* g = new dep_geom();
+    g->ctx = c;
+    g->N = N;
+    g->E = E;
+    g->npe = npe;
+    morton_order(N, xyz, g->perm);
+    g->iperm.resize(N);
+    for (int i = 0; i < N; i++) g->iperm[g->perm[i]] = i;
+    g->hxyz.resize(3 * (size_t)N);
+    g->hkind.resize(N);
+    g->hiface.resize(N);
+    g->hbc6.resize(N);
+    for (int i = 0; i < N; i++) {
+        const int o = g->perm[i];
+        for (int k = 0; k < 3; k++) g->hxyz[3 * i + k] = xyz[3 * o + k];
+        const uint32_t t = bctype[o];
+        g->hkind[i] = (t == 0 || t == 6) ? 1 : 0;
+        g->hiface[i] = (iface && g->hkind[i]) ? iface[o] : -1;
+        g->hbc6[i] = (t == 6);
+    }
+    g->hconn.resize((size_t)E * npe);
+    for (long t = 0; t < (long)E * npe; t++) g->hconn[t] = g->iperm[conn[t]];
+
+    // node -> (element, first position) in element order
+    std::vector<int> cnt(N + 1, 0);
+    for (int e = 0; e < E; e++)
+        for (int k = 0; k < npe; k++) {
+            const int v = g->hconn[(size_t)e * npe + k];
+            bool first = true;
+            for (int q = 0; q < k; q++) first &= g->hconn[(size_t)e * npe + q] != v;
+            if (first) cnt[v + 1]++;
+        }
+    for (int i = 0; i < N; i++) cnt[i + 1] += cnt[i];
+    g->hinc_ptr = cnt;
+    std::vector<int>& inc_ek = g->hinc_ek;
+    inc_ek.assign(cnt[N], 0);
+    {
+        std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+        for (int e = 0; e < E; e++)
+            for (int k = 0; k < npe; k++) {
+                const int v = g->hconn[(size_t)e * npe + k];
+                bool first = true;
+                for (int q = 0; q < k; q++) first &= g->hconn[(size_t)e * npe + q] != v;
+                if (first) inc_ek[fill[v]++] = e * 8 + k;
+            }
+    }
+    // column patches
+    g->n_patch = dep_cdiv(N, DEP_TC);
+    std::vector<int> pptr(g->n_patch + 1, 0), pel;
+    std::vector<int8_t> pslot;
+    {
+        std::vector<int> mark(E, -1);
+        for (int p = 0; p < g->n_patch; p++) {
+            const int c0 = p * DEP_TC, c1 = std::min(N, c0 + DEP_TC);
+            std::vector<int> els;
+            for (int v = c0; v < c1; v++)
+                for (int t = cnt[v]; t < cnt[v + 1]; t++) {
+                    const int e = inc_ek[t] >> 3;
+                    if (mark[e] != p) {
+                        mark[e] = p;
+                        els.push_back(e);
+                    }
+                }
+            std::sort(els.begin(), els.end());
+            for (int e : els) {
+                pel.push_back(e);
+                for (int k = 0; k < npe; k++) {
+                    const int v = g->hconn[(size_t)e * npe + k];
+                    pslot.push_back((v >= c0 && v < c1) ? (int8_t)(v - c0) : (int8_t)-1);
+                }
+            }
+            pptr[p + 1] = (int)pel.size();
+        }
+    }
+    int r = DEP_OK;
+#define GA(ptr, n)                                  \
+    if (r == DEP_OK) r = dep_obj_alloc(g, &(ptr), (n));
+    GA(g->xyz, 3 * (size_t)N);
+    GA(g->conn, (size_t)E * npe);
+    GA(g->kind, (size_t)N);
+    GA(g->normal, 3 * (size_t)N);
+    GA(g->q, (size_t)E * 7 * 4);
+    GA(g->inc_ptr, (size_t)N + 1);
+    GA(g->inc_ek, inc_ek.size());
+    GA(g->patch_ptr, pptr.size());
+    GA(g->patch_elem, pel.size());
+    GA(g->patch_slot, pslot.size());
+#undef GA
+    if (r == DEP_OK) r = upload(c, g->xyz, g->hxyz.data(), g->hxyz.size());
+    if (r == DEP_OK) r = upload(c, g->conn, g->hconn.data(), g->hconn.size());
+    if (r == DEP_OK) r = upload(c, g->kind, g->hkind.data(), g->hkind.size());
+    if (r == DEP_OK) r = upload(c, g->inc_ptr, cnt.data(), cnt.size());
+    if (r == DEP_OK) r = upload(c, g->inc_ek, inc_ek.data(), inc_ek.size());
+    if (r == DEP_OK) r = upload(c, g->patch_ptr, pptr.data(), pptr.size());
+    if (r == DEP_OK) r = upload(c, g->patch_elem, pel.data(), pel.size());
+    if (r == DEP_OK) r = upload(c, g->patch_slot, pslot.data(), pslot.size());
+    if (r == DEP_OK) {
+        hipError_t e = launch_quad_points(g);
+        if (e == hipSuccess) e = launch_node_normals(g);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) r = dep_fail(c, DEP_ERR_HIP, "geometry kernels: %s", hipGetErrorString(e));
+    }
+    if (r != DEP_OK) {
+        free_all(g);
+        delete g;
+        return r;
+    }
+    *out = g;
+    return DEP_OK;
+}
+
+extern "C" int dep_geom_destroy(dep_geom* g)
+{
+    if (!g) return DEP_OK;
+    (void)hipStreamSynchronize(g->ctx->stream);
+    free_all(g);
+    delete g;
+    return DEP_OK;
+}
+
+extern "C" int dep_geom_assemble(dep_geom* g)
+{
+    if (!g) return DEP_ERR_ARG;
+    dep_ctx* c = g->ctx;
+    if (g->assembled) return DEP_OK;
+    if (!g->P) {
+        g->ldP = ((long)g->N + 63) / 64 * 64;
+        DEP_CHECK(dep_obj_alloc(g, &g->P, (size_t)g->N * g->ldP));
+        DEP_CHECK(dep_obj_alloc(g, &g->S, (size_t)g->npe * g->E * g->npe));
+        DEP_HIP(c, hipMemsetAsync(g->P, 0, sizeof(double) * g->N * g->ldP, c->stream));
+    }
+    DEP_HIP(c, launch_assemble(g));
+    DEP_HIP(c, hipStreamSynchronize(c->stream));
+    g->assembled = true;
+    return DEP_OK;
+}
+
+extern "C" int dep_geom_get_blocks(dep_geom* g, double* P)
+{
+    if (!g || !P) return DEP_ERR_ARG;
+    DEP_CHECK(dep_geom_assemble(g));
+    const int N = g->N;
+    std::vector<double> h((size_t)N * g->ldP);
+    DEP_CHECK(download(g->ctx, h.data(), g->P, h.size()));
+    for (int i = 0; i < N; i++)
+        for (int j = 0; j < N; j++) P[(size_t)g->perm[i] * N + g->perm[j]] = h[(size_t)i * g->ldP + j];
+    return DEP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// systems
+// ---------------------------------------------------------------------------
+
+// per-interface weights A_k, B_k of src/assembly_el.c:496-512
+static void iface_weights(const dep_sys* s, double freq, std::vector<double>& A, std::vector<double>& B)
+{
+    const double W = freq * DEP_PI2 * DEP_EPS0;
+    A.assign(s->nI, 0.0);
+    B.assign(s->nI, 0.0);
+    for (int i = 0; i < s->nI; i++) {
+        const int m1 = (int)s->ifaces[2 * i] - 1, m2 = (int)s->ifaces[2 * i + 1] - 1;
+        const double SigTot = s->mats[2 * m1] + s->mats[2 * m2];
+        const double SigDif = s->mats[2 * m1] - s->mats[2 * m2];
+        const double EpsTot = W * (s->mats[2 * m1 + 1] + s->mats[2 * m2 + 1]);
+        const double EpsDif = W * (s->mats[2 * m1 + 1] - s->mats[2 * m2 + 1]);
+        const double D = kC / (SigTot * SigTot + EpsTot * EpsTot);
+        A[i] = D * (SigTot * SigDif + EpsTot * EpsDif);
+        B[i] = D * (SigDif * EpsTot - SigTot * EpsDif);
+    }
+}
+
+static int set_weights(dep_sys* s, double freq)
+{
+    dep_geom* g = s->g;
+    std::vector<double> A, B;
+    iface_weights(s, freq, A, B);
+    s->hwr.assign(g->N, 0.0);
+    s->hwi.assign(g->N, 0.0);
+    s->hshift.assign(g->N, 0.0);
+    for (int i = 0; i < g->N; i++) {
+        if (g->hkind[i] == 0) {
+            s->hwr[i] = kC;
+        } else {
+            const int k = g->hiface[i];
+            if (k < 0 || k >= s->nI)
+                return dep_fail(g->ctx, DEP_ERR_ARG, "interface node %d has interface id %d outside [0,%d)",
+                                g->perm[i], k, s->nI);
+            // Z = F - iG with F = A_k P, G = B_k P (assembly_el.c:565-573)
+            s->hwr[i] = A[k];
+            s->hwi[i] = -B[k];
+            s->hshift[i] = -0.5 / DEP_EPS0;
+        }
+    }
+    s->freq = freq;
+    DEP_CHECK(upload(g->ctx, s->wr, s->hwr.data(), g->N));
+    DEP_CHECK(upload(g->ctx, s->wi, s->hwi.data(), g->N));
+    DEP_CHECK(upload(g->ctx, s->shift, s->hshift.data(), g->N));
+    s->factored = false;
+    return DEP_OK;
+}
+
+extern "C" int dep_sys_create(dep_geom* g, int nM, const double* mats, int nI, const uint32_t* ifaces,
+                              double freq, const double* rhs, dep_sys** out)
+{
+    if (!g || !out || (nM > 0 && !mats) || (nI > 0 && !ifaces) || !rhs)
+        return dep_fail(g ? g->ctx : nullptr, DEP_ERR_ARG, "dep_sys_create: bad arguments");
+    for (int i = 0; i < 2 * nI; i++)
+        if (ifaces[i] < 1 || (int)ifaces[i] > nM)
+            return dep_fail(g->ctx, DEP_ERR_ARG, "interface material id %u outside [1,%d]", ifaces[i], nM);
+    DEP_CHECK(dep_geom_assemble(g));
+    dep_sys* s = new dep_sys();
+    s->g = g;
+    s->nM = nM;
+    s->nI = nI;
+    s->mats.assign(mats, mats + 2 * nM);
+    s->ifaces.assign(ifaces, ifaces + 2 * nI);
+    const int N = g->N;
+    int r = DEP_OK;
+    if (r == DEP_OK) r = dep_obj_alloc(s, &s->wr, N);
+    if (r == DEP_OK) r = dep_obj_alloc(s, &s->wi, N);
+    if (r == DEP_OK) r = dep_obj_alloc(s, &s->shift, N);
+    if (r == DEP_OK) r = dep_obj_alloc(s, &s->b, 2 * (size_t)N);
+    if (r == DEP_OK) r = dep_obj_alloc(s, &s->x, 2 * (size_t)N);
+    if (r == DEP_OK) r = dep_obj_alloc(s, &s->work, 8 * 2 * (size_t)N);
+    if (r == DEP_OK) r = dep_obj_alloc(s, &s->dwork, 65536);
+    if (r == DEP_OK) r = dep_obj_alloc(s, &s->qs, (size_t)g->E * 7 * 2);
+    if (r == DEP_OK) r = set_weights(s, freq);
+    if (r == DEP_OK) {
+        std::vector<double> hb(2 * (size_t)N);
+        for (int i = 0; i < N; i++) {
+            hb[i] = rhs[g->perm[i]];
+            hb[N + i] = rhs[N + g->perm[i]];
+        }
+        r = upload(g->ctx, s->b, hb.data(), hb.size());
+        if (r == DEP_OK) {
+            hipError_t e = hipMemsetAsync(s->x, 0, sizeof(double) * 2 * N, g->ctx->stream);
+            if (e != hipSuccess) r = dep_fail(g->ctx, DEP_ERR_HIP, "memset: %s", hipGetErrorString(e));
+        }
+    }
+    if (r != DEP_OK) {
+        free_all(s);
+        delete s;
+        return r;
+    }
+    *out = s;
+    return DEP_OK;
+}
+
+extern "C" int dep_sys_destroy(dep_sys* s)
+{
+    if (!s) return DEP_OK;
+    (void)hipStreamSynchronize(s->g->ctx->stream);
+    free_all(s);
+    delete s;
+    return DEP_OK;
+}
+
+extern "C" int dep_sys_set_frequency(dep_sys* s, double freq)
+{
+    if (!s || !(freq >= 0.0)) return DEP_ERR_ARG;
+    return set_weights(s, freq);
+}
+
+extern "C" int dep_sys_get_matrix(dep_sys* s, int layout, double* mA)
+{
+    if (!s || !mA || (layout != 0 && layout != 1)) return DEP_ERR_ARG;
+    dep_geom* g = s->g;
+    const int N = g->N;
+    std::vector<double> h((size_t)N * g->ldP);
+    DEP_CHECK(download(g->ctx, h.data(), g->P, h.size()));
+    const size_t ld = 2 * (size_t)N;
+    memset(mA, 0, sizeof(double) * ld * (layout == 0 ? 2 * (size_t)N : (size_t)N));
+    for (int i = 0; i < N; i++) {
+        const size_t ro = g->perm[i];
+        const double wr = s->hwr[i], G = -s->hwi[i];
+        for (int j = 0; j < N; j++) {
+            const size_t co = g->perm[j];
+            const double p = h[(size_t)i * g->ldP + j];
+            double F = wr * p;
+            if (i == j) F += s->hshift[i];
+            const double Gv = G * p;
+            mA[ro * ld + co] = F;
+            mA[ro * ld + co + N] = Gv;
+            if (layout == 0) {
+                mA[(ro + N) * ld + co] = -Gv;
+                mA[(ro + N) * ld + co + N] = F;
+            }
+        }
+        if (layout == 1 && g->hkind[i] != 0) {
+            // the GMRES layout carries the jump term on the real diagonal only
+        }
+    }
+    return DEP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// factor / solve
+// ---------------------------------------------------------------------------
+
+static int ensure_fp64_factor_storage(dep_sys* s)
+{
+    dep_geom* g = s->g;
+    if (s->Zr) return DEP_OK;
+    s->ldZ = ((long)g->N + 63) / 64 * 64;
+    DEP_CHECK(dep_obj_alloc(s, &s->Zr, (size_t)g->N * s->ldZ));
+    DEP_CHECK(dep_obj_alloc(s, &s->Zi, (size_t)g->N * s->ldZ));
+    const size_t nblk = dep_cdiv(g->N, 64);
+    double* inv = nullptr;
+    DEP_CHECK(dep_obj_alloc(s, &inv, nblk * 4 * 64 * 64));
+    s->opA = inv;
+    return DEP_OK;
+}
+
+extern "C" int dep_factor(dep_sys* s, int precision)
+{
+    if (!s) return DEP_ERR_ARG;
+    dep_ctx* c = s->g->ctx;
+    if (precision != DEP_PREC_FP64)
+        return dep_fail(c, DEP_ERR_ARG, "precision %d is not available in this build", precision);
+    DEP_CHECK(ensure_fp64_factor_storage(s));
+    DEP_HIP(c, hipEventRecord(c->ev0, c->stream));
+    DEP_HIP(c, launch_weight_fp64(s));
+    double mp = 0.0;
+    DEP_HIP(c, launch_lu_fp64(s, &mp));
+    DEP_HIP(c, hipEventRecord(c->ev1, c->stream));
+    DEP_HIP(c, hipEventSynchronize(c->ev1));
+    float ms = 0.0f;
+    DEP_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    s->factor_ms = ms;
+    s->min_pivot = mp;
+    if (!(mp > 0.0) || !std::isfinite(mp))
+        return dep_fail(c, DEP_ERR_SINGULAR, "zero or non-finite pivot in the LU (min |pivot| = %g)", mp);
+    s->prec = precision;
+    s->factored = true;
+    return DEP_OK;
+}
+
+static double host_norm(const std::vector<double>& v)
+{
+    double a = 0.0;
+    for (double x : v) a += x * x;
+    return sqrt(a);
+}
+
+extern "C" int dep_solve(dep_sys* s, double tol, int max_steps, double* sigma, dep_solve_info* info)
+{
+    if (!s) return DEP_ERR_ARG;
+    dep_geom* g = s->g;
+    dep_ctx* c = g->ctx;
+    if (!s->factored) return dep_fail(c, DEP_ERR_STATE, "dep_solve: system is not factored");
+    const int N = g->N;
+    double* r = s->work;
+    double* d = s->work + 2 * (size_t)N;
+    DEP_HIP(c, hipEventRecord(c->ev0, c->stream));
+    DEP_HIP(c, launch_trsv_fp64(s, s->b, s->x));
+    std::vector<double> hb(2 * (size_t)N), hr(2 * (size_t)N);
+    DEP_CHECK(download(c, hb.data(), s->b, hb.size()));
+    const double bn = host_norm(hb);
+    int steps = 0;
+    double res = 0.0;
+    for (;;) {
+        DEP_HIP(c, launch_residual(s, s->x, r));
+        DEP_CHECK(download(c, hr.data(), r, hr.size()));
+        res = bn > 0 ? host_norm(hr) / bn : host_norm(hr);
+        if (!(res > tol) || steps >= max_steps) break;
+        DEP_HIP(c, launch_trsv_fp64(s, r, d));
+        // x += d
+        std::vector<double> hx(2 * (size_t)N), hd(2 * (size_t)N);
+        DEP_CHECK(download(c, hx.data(), s->x, hx.size()));
+        DEP_CHECK(download(c, hd.data(), d, hd.size()));
+        for (size_t k = 0; k < hx.size(); k++) hx[k] += hd[k];
+        DEP_CHECK(upload(c, s->x, hx.data(), hx.size()));
+        steps++;
+    }
+    DEP_HIP(c, hipEventRecord(c->ev1, c->stream));
+    DEP_HIP(c, hipEventSynchronize(c->ev1));
+    float ms = 0.0f;
+    DEP_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    if (info) {
+        memset(info, 0, sizeof(*info));
+        info->iterations = steps;
+        info->residual = res;
+        info->converged = !(res > tol);
+        info->min_pivot = s->min_pivot;
+        info->factor_ms = s->factor_ms;
+        info->solve_ms = ms;
+    }
+    if (sigma) {
+        DEP_CHECK(dep_get_solution(s, sigma));
+    }
+    if (!std::isfinite(res)) return dep_fail(c, DEP_ERR_SINGULAR, "non-finite residual after the solve");
+    return DEP_OK;
+}
+
+extern "C" int dep_get_solution(dep_sys* s, double* sigma)
+{
+    if (!s || !sigma) return DEP_ERR_ARG;
+    dep_geom* g = s->g;
+    const int N = g->N;
+    std::vector<double> hx(2 * (size_t)N);
+    DEP_CHECK(download(g->ctx, hx.data(), s->x, hx.size()));
+    for (int i = 0; i < N; i++) {
+        sigma[g->perm[i]] = hx[i];
+        sigma[N + g->perm[i]] = hx[N + i];
+    }
+    return DEP_OK;
+}
+
+extern "C" int dep_set_solution(dep_sys* s, const double* sigma)
+{
+    if (!s || !sigma) return DEP_ERR_ARG;
+    dep_geom* g = s->g;
+    const int N = g->N;
+    std::vector<double> hx(2 * (size_t)N);
+    for (int i = 0; i < N; i++) {
+        hx[i] = sigma[g->perm[i]];
+        hx[N + i] = sigma[N + g->perm[i]];
+    }
+    return upload(g->ctx, s->x, hx.data(), hx.size());
+}
+
+// ---------------------------------------------------------------------------
+// post-processing
+// ---------------------------------------------------------------------------
+
+
+struct XyzKey {
+    double x, y, z;
+    bool operator==(const XyzKey& o) const { return x == o.x && y == o.y && z == o.z; }
+};
+struct XyzHash {
+    size_t operator()(const XyzKey& k) const
+    {
+        uint64_t a, b, c;
+        memcpy(&a, &k.x, 8);
+        memcpy(&b, &k.y, 8);
+        memcpy(&c, &k.z, 8);
+        return (size_t)(a * 0x9E3779B97F4A7C15ULL ^ (b + 0x7F4A7C15ULL) * 0xC2B2AE3D27D4EB4FULL ^ c);
+    }
+};
+
+// Prepare an evaluation-point set.  An element is singular for a point when
+// one of its nodes has exactly the point's coordinates (potential.c:155,
+// field.c:171); its position is the first such node in the element.
+int dep_points_prepare(dep_geom* g, int m, const double* pts, dep_points* P)
+{
+    dep_ctx* c = g->ctx;
+    *P = dep_points();
+    P->ctx = c;
+    P->m = m;
+    if (m <= 0) return DEP_OK;
+    std::unordered_map<XyzKey, std::vector<int>, XyzHash> at;
+    at.reserve(2 * (size_t)g->N);
+    for (int i = 0; i < g->N; i++) at[{g->hxyz[3 * i], g->hxyz[3 * i + 1], g->hxyz[3 * i + 2]}].push_back(i);
+    std::vector<int> ptr(m + 1, 0), ex, pp, pek;
+    std::vector<std::pair<int, int>> hits;
+    for (int p = 0; p < m; p++) {
+        hits.clear();
+        auto it = at.find({pts[3 * p], pts[3 * p + 1], pts[3 * p + 2]});
+        if (it != at.end())
+            for (int v : it->second)
+                for (int t = g->hinc_ptr[v]; t < g->hinc_ptr[v + 1]; t++)
+                    hits.emplace_back(g->hinc_ek[t] >> 3, g->hinc_ek[t] & 7);
+        std::sort(hits.begin(), hits.end());
+        int last = -1;
+        for (auto& h : hits) {
+            if (h.first == last) continue;   // keep the first coincident position
+            last = h.first;
+            ex.push_back(h.first);
+            pp.push_back(p);
+            pek.push_back(h.first * 8 + h.second);
+        }
+        ptr[p + 1] = (int)ex.size();
+    }
+    P->npairs = (int)ex.size();
+    const int waves = dep_cdiv(m, 64);
+    P->nchunk = std::max(1, std::min(std::min(64, g->E), dep_cdiv(4096, waves)));
+    struct Tmp {
+        dep_ctx* c;
+        std::vector<std::pair<void*, size_t>> allocs;
+        dep_ctx* ctx_of() { return c; }
+    } t{c, {}};
+    int r = DEP_OK;
+    if (r == DEP_OK) r = dep_obj_alloc(&t, &P->pts, 3 * (size_t)m);
+    if (r == DEP_OK) r = dep_obj_alloc(&t, &P->excl_ptr, (size_t)m + 1);
+    if (r == DEP_OK) r = dep_obj_alloc(&t, &P->excl_e, ex.size());
+    if (r == DEP_OK) r = dep_obj_alloc(&t, &P->pair_pt, pp.size());
+    if (r == DEP_OK) r = dep_obj_alloc(&t, &P->pair_ek, pek.size());
+    if (r == DEP_OK) r = dep_obj_alloc(&t, &P->partial, (size_t)P->nchunk * m * 6);
+    if (r == DEP_OK) r = dep_obj_alloc(&t, &P->sing, ex.size() * 6);
+    if (r == DEP_OK) r = dep_obj_alloc(&t, &P->out, (size_t)m * 6);
+    if (r == DEP_OK) r = upload(c, P->pts, pts, 3 * (size_t)m);
+    if (r == DEP_OK) r = upload(c, P->excl_ptr, ptr.data(), ptr.size());
+    if (r == DEP_OK && !ex.empty()) r = upload(c, P->excl_e, ex.data(), ex.size());
+    if (r == DEP_OK && !pp.empty()) r = upload(c, P->pair_pt, pp.data(), pp.size());
+    if (r == DEP_OK && !pek.empty()) r = upload(c, P->pair_ek, pek.data(), pek.size());
+    P->bytes = 0;
+    for (auto& a : t.allocs) P->bytes += a.second;
+    P->allocs = t.allocs;
+    if (r != DEP_OK) dep_points_release(P);
+    return r;
+}
+
+void dep_points_release(dep_points* P)
+{
+    if (!P->ctx) return;
+    (void)hipStreamSynchronize(P->ctx->stream);
+    for (auto& a : P->allocs) dep_free(P->ctx, a.first, a.second);
+    P->allocs.clear();
+}
+
+static int eval_points(dep_sys* s, int mode, int m, const double* pts, double* out)
+{
+    if (!s || m < 0 || (m > 0 && (!pts || !out))) return DEP_ERR_ARG;
+    if (m == 0) return DEP_OK;
+    dep_ctx* c = s->g->ctx;
+    dep_points P;
+    DEP_CHECK(dep_points_prepare(s->g, m, pts, &P));
+    hipError_t e = launch_sigma_at_points(s);
+    if (e == hipSuccess) e = launch_points(s, mode, &P, P.out);
+    int r = DEP_OK;
+    if (e != hipSuccess) r = dep_fail(c, DEP_ERR_HIP, "post kernels: %s", hipGetErrorString(e));
+    if (r == DEP_OK) r = download(c, out, P.out, (size_t)m * (mode ? 6 : 2));
+    dep_points_release(&P);
+    return r;
+}
+
+extern "C" int dep_post_potential(dep_sys* s, int m, const double* pts, double* pot)
+{
+    return eval_points(s, 0, m, pts, pot);
+}
+
+extern "C" int dep_post_field(dep_sys* s, int m, const double* pts, double* ef)
+{
+    return eval_points(s, 1, m, pts, ef);
+}
+
+extern "C" int dep_post_mst(dep_sys* s, double eps, double* Fce, double* Xce, double* Fcm, double* Xcm)
+{
+    if (!s || !Fce || !Xce || !Fcm || !Xcm) return DEP_ERR_ARG;
+    dep_geom* g = s->g;
+    dep_ctx* c = g->ctx;
+    const int N = g->N, E = g->E, npe = g->npe;
+    // bc-6 nodes (internal ids) and their coordinates
+    std::vector<int> nodes;
+    std::vector<double> pts;
+    for (int i = 0; i < N; i++)
+        if (g->hbc6[i]) {
+            nodes.push_back(i);
+            for (int k = 0; k < 3; k++) pts.push_back(g->hxyz[3 * i + k]);
+        }
+    const int m = (int)nodes.size();
+    if (m == 0) return dep_fail(c, DEP_ERR_ARG, "dep_post_mst: no node has bc type 6");
+    // Xcm: mean of the bc-6 nodes in the caller's node order (force.c:493-507)
+    Xcm[0] = Xcm[1] = Xcm[2] = 0.0;
+    for (int o = 0; o < N; o++) {
+        const int i = g->iperm[o];
+        if (!g->hbc6[i]) continue;
+        for (int k = 0; k < 3; k++) Xcm[k] += g->hxyz[3 * i + k];
+    }
+    for (int k = 0; k < 3; k++) Xcm[k] = Xcm[k] / m;
+    // force elements: first node has bc type 6 (force.c:519)
+    std::vector<int> fel;
+    for (int e = 0; e < E; e++)
+        if (g->hbc6[g->hconn[(size_t)e * npe]]) fel.push_back(e);
+    const int nf = (int)fel.size();
+    dep_points P;
+    DEP_CHECK(dep_points_prepare(g, m, pts.data(), &P));
+    struct Tmp {
+        dep_ctx* c;
+        std::vector<std::pair<void*, size_t>> allocs;
+        dep_ctx* ctx_of() { return c; }
+    } t{c, {}};
+    int* dnodes = nullptr;
+    int* dfel = nullptr;
+    double *Efn = nullptr, *dF = nullptr, *dX = nullptr, *dcm = nullptr;
+    int r = DEP_OK;
+    if (r == DEP_OK) r = dep_obj_alloc(&t, &dnodes, (size_t)m);
+    if (r == DEP_OK) r = dep_obj_alloc(&t, &dfel, (size_t)std::max(nf, 1));
+    if (r == DEP_OK) r = dep_obj_alloc(&t, &Efn, 3 * (size_t)N);
+    if (r == DEP_OK) r = dep_obj_alloc(&t, &dF, 3 * (size_t)std::max(nf, 1));
+    if (r == DEP_OK) r = dep_obj_alloc(&t, &dX, 3 * (size_t)std::max(nf, 1));
+    if (r == DEP_OK) r = dep_obj_alloc(&t, &dcm, 3);
+    if (r == DEP_OK) r = upload(c, dnodes, nodes.data(), nodes.size());
+    if (r == DEP_OK && nf) r = upload(c, dfel, fel.data(), fel.size());
+    if (r == DEP_OK) {
+        hipError_t e = hipMemsetAsync(Efn, 0, sizeof(double) * 3 * N, c->stream);
+        if (e == hipSuccess) e = launch_sigma_at_points(s);
+        if (e == hipSuccess) e = launch_points(s, 1, &P, P.out);
+        if (e == hipSuccess) e = launch_mst(s, &P, P.out, dnodes, nf, dfel, 0.5 * eps, Efn, dF, dX, dcm);
+        if (e != hipSuccess) r = dep_fail(c, DEP_ERR_HIP, "MST kernels: %s", hipGetErrorString(e));
+    }
+    std::vector<double> hF(3 * (size_t)std::max(nf, 1)), hX(3 * (size_t)std::max(nf, 1));
+    if (r == DEP_OK) r = download(c, hF.data(), dF, hF.size());
+    if (r == DEP_OK) r = download(c, hX.data(), dX, hX.size());
+    if (r == DEP_OK) r = download(c, Fcm, dcm, 3);
+    if (r == DEP_OK) {
+        memset(Fce, 0, sizeof(double) * 3 * E);
+        memset(Xce, 0, sizeof(double) * 3 * E);
+        for (int k = 0; k < nf; k++)
+            for (int q = 0; q < 3; q++) {
+                Fce[3 * (size_t)fel[k] + q] = hF[3 * k + q];
+                Xce[3 * (size_t)fel[k] + q] = hX[3 * k + q];
+            }
+    }
+    for (auto& a : t.allocs) dep_free(c, a.first, a.second);
+    dep_points_release(&P);
+    return r;
+}
+
+extern "C" int dep_post_multipole(dep_sys* s, int order, double radius, int m, const double* pts,
+                                  double* F)
+{
+    (void)order; (void)radius; (void)m; (void)pts; (void)F;
+    return dep_fail(s ? s->g->ctx : nullptr, DEP_ERR_STATE, "dep_post_multipole: not available yet");
+}
+
+// GMRES (src/gmres_el.c:66-354): unrestarted, Givens rotations on the host,
+// Krylov vectors and products on the device.
+extern "C" int dep_gmres(dep_sys* s, int precond, double tol_rel, int max_iters, const double* x0,
+                         double* sigma, dep_solve_info* info, const char* log_path)
+{
+    if (!s || max_iters <= 0 || !(tol_rel > 0.0)) return DEP_ERR_ARG;
+    dep_geom* g = s->g;
+    dep_ctx* c = g->ctx;
+    const int N = g->N;
+    const long n2 = 2L * N;
+    // Jacobi scaling: row i (both halves) and b[i] by 1/A[i][i], not b[i+N] (gmres_el.c:87-93)
+    std::vector<double> d(N, 1.0);
+    if (precond == 1) {
+        if (s->dense) return dep_fail(c, DEP_ERR_ARG, "dep_gmres: scale a dense system before the call");
+        std::vector<double> pd(N);
+        DEP_HIP(c, hipMemcpy2DAsync(pd.data(), sizeof(double), g->P, sizeof(double) * (g->ldP + 1),
+                                    sizeof(double), N, hipMemcpyDeviceToHost, c->stream));
+        DEP_HIP(c, hipStreamSynchronize(c->stream));
+        for (int i = 0; i < N; i++) d[i] = 1.0 / (s->hwr[i] * pd[i] + s->hshift[i]);
+    }
+    struct Tmp {
+        dep_ctx* c;
+        std::vector<std::pair<void*, size_t>> allocs;
+        dep_ctx* ctx_of() { return c; }
+        ~Tmp() { for (auto& a : allocs) dep_free(c, a.first, a.second); }
+    } t{c, {}};
+    double *gw = nullptr, *gwi = nullptr, *gsh = nullptr, *b = nullptr, *x = nullptr, *V = nullptr,
+           *w = nullptr, *h = nullptr;
+    DEP_CHECK(dep_obj_alloc(&t, &gw, N));
+    DEP_CHECK(dep_obj_alloc(&t, &gwi, N));
+    DEP_CHECK(dep_obj_alloc(&t, &gsh, N));
+    DEP_CHECK(dep_obj_alloc(&t, &b, n2));
+    DEP_CHECK(dep_obj_alloc(&t, &x, n2));
+    DEP_CHECK(dep_obj_alloc(&t, &w, n2));
+    DEP_CHECK(dep_obj_alloc(&t, &h, 2 * (size_t)(max_iters + 2)));
+    DEP_CHECK(dep_obj_alloc(&t, &V, (size_t)(max_iters + 1) * n2));
+    std::vector<double> hb(n2);
+    DEP_CHECK(download(c, hb.data(), s->b, n2));
+    if (!s->dense) {
+        std::vector<double> a(N), bi(N), sh(N);
+        for (int i = 0; i < N; i++) {
+            a[i] = d[i] * s->hwr[i];
+            bi[i] = d[i] * s->hwi[i];
+            sh[i] = d[i] * s->hshift[i];
+            hb[i] = hb[i] * d[i];
+        }
+        DEP_CHECK(upload(c, gw, a.data(), N));
+        DEP_CHECK(upload(c, gwi, bi.data(), N));
+        DEP_CHECK(upload(c, gsh, sh.data(), N));
+    }
+    DEP_CHECK(upload(c, b, hb.data(), n2));
+    auto matvec = [&](const double* in, const double* rhs, double* out) -> hipError_t {
+        if (s->dense) return launch_dense_zmatvec(c, N, s->Zr, s->Zi, s->ldZ, in, rhs, out);
+        return launch_zmatvec_w(s, gw, gwi, gsh, in, rhs, out);
+    };
+    auto norm = [&](const double* v, double* out) -> int {
+        DEP_HIP(c, launch_dots(c, 1, n2, v, v, h));
+        double hv = 0.0;
+        DEP_CHECK(download(c, &hv, h, 1));
+        *out = sqrt(hv);
+        return DEP_OK;
+    };
+    double bnorm = 0.0;
+    for (double v : hb) bnorm += v * v;
+    const double tol = sqrt(bnorm) * tol_rel;
+    FILE* fg = log_path ? fopen(log_path, "w") : nullptr;
+    if (fg) {
+        fprintf(fg, "\n\nRELATIVE TOLERANCE = %10.4e\n", tol);
+        fprintf(fg, "\n%10s%21s\n", "ITER #    ", "RELATIVE RESIDUAL");
+        fprintf(fg, "%10s%21s\n", "***************", "*******************");
+    }
+    DEP_HIP(c, hipEventRecord(c->ev0, c->stream));
+    // initial residual (gmres_el.c:272-294)
+    double xn = 0.0;
+    if (x0) {
+        std::vector<double> hx(n2);
+        for (int i = 0; i < N; i++) {
+            hx[i] = x0[g->perm[i]];
+            hx[N + i] = x0[N + g->perm[i]];
+        }
+        DEP_CHECK(upload(c, x, hx.data(), n2));
+        for (double v : hx) xn += v * v;
+    } else {
+        DEP_HIP(c, hipMemsetAsync(x, 0, sizeof(double) * n2, c->stream));
+    }
+    if (xn != 0.0) DEP_HIP(c, matvec(x, b, V));
+    else DEP_HIP(c, hipMemcpyAsync(V, b, sizeof(double) * n2, hipMemcpyDeviceToDevice, c->stream));
+    double beta = 0.0;
+    DEP_CHECK(norm(V, &beta));
+    DEP_HIP(c, launch_scale(c, n2, V, 1.0 / beta, V));
+    const int M = max_iters;
+    std::vector<double> H((size_t)(M + 1) * (M + 1), 0.0), cs(M + 1, 0.0), sn(M + 1, 0.0),
+        res(M + 2, 0.0), hh(M + 2);
+#define HH(r, col) H[(size_t)(r) * (M + 1) + (col)]
+    res[0] = beta;
+    double err0 = beta, err1 = beta;
+    int iters = 0, rc = 1;
+    if (err0 < tol) rc = 0;
+    for (int i = 0; rc == 1 && i < M; i++) {
+        iters++;
+        double* vi = V + (size_t)i * n2;
+        DEP_HIP(c, matvec(vi, nullptr, w));
+        double wn = 0.0;
+        DEP_CHECK(norm(w, &wn));
+        // CGS2
+        std::vector<double> hsum(i + 1, 0.0);
+        for (int pass = 0; pass < 2; pass++) {
+            DEP_HIP(c, launch_dots(c, i + 1, n2, V, w, h));
+            DEP_HIP(c, launch_update(c, i + 1, n2, V, h, w));
+            DEP_CHECK(download(c, hh.data(), h, i + 1));
+            for (int j = 0; j <= i; j++) hsum[j] += hh[j];
+        }
+        for (int j = 0; j <= i; j++) HH(j, i) = hsum[j];
+        double hn = 0.0;
+        DEP_CHECK(norm(w, &hn));
+        HH(i + 1, i) = hn;
+        if (hn != 0.0) DEP_HIP(c, launch_scale(c, n2, w, 1.0 / hn, V + (size_t)(i + 1) * n2));
+        for (int j = 0; j <= i - 1; j++) {
+            const double h1 = cs[j] * HH(j, i) + sn[j] * HH(j + 1, i);
+            const double h2 = -sn[j] * HH(j, i) + cs[j] * HH(j + 1, i);
+            HH(j, i) = h1;
+            HH(j + 1, i) = h2;
+        }
+        const double htmp = HH(i + 1, i) / HH(i, i);
+        if (wn != 0.0) {
+            cs[i] = 1.0 / sqrt(1.0 + htmp * htmp);
+            sn[i] = cs[i] * htmp;
+            HH(i, i) = cs[i] * HH(i, i) + sn[i] * HH(i + 1, i);
+            HH(i + 1, i) = 0.0;
+            res[i + 1] = -sn[i] * res[i];
+            res[i] = cs[i] * res[i];
+        }
+        err1 = fabs(res[i + 1]);
+        if (fg) fprintf(fg, "%8d%25.4e\n", iters, err1);
+        if (err1 < tol) rc = 0;
+        else if (fabs((err1 - err0) / err0) < 1.0e-6 * tol) rc = 2;
+        else err0 = err1;
+    }
+    if (rc != 2 && iters > 0) {
+        for (int i = iters - 1; i >= 0; i--) {
+            for (int j = i + 1; j <= iters; j++) res[i] -= HH(i, j) * res[j];
+            res[i] = res[i] / HH(i, i);
+        }
+        DEP_CHECK(upload(c, h, res.data(), iters));
+        DEP_HIP(c, launch_combine(c, iters, n2, V, h, x));
+    }
+#undef HH
+    DEP_HIP(c, hipEventRecord(c->ev1, c->stream));
+    DEP_HIP(c, hipEventSynchronize(c->ev1));
+    float ms = 0.0f;
+    DEP_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    if (fg) {
+        if (rc == 0) fprintf(fg, "\n\n***PROBLEM CONVERGES IN %d ITERATIONS***\n\n", iters);
+        else fprintf(fg, "Problem fails to vonverge to given tolerance in %d iterations\n\n", M);
+        fclose(fg);
+    }
+    if (rc == 0) {
+        DEP_HIP(c, hipMemcpyAsync(s->x, x, sizeof(double) * n2, hipMemcpyDeviceToDevice, c->stream));
+        if (sigma) DEP_CHECK(dep_get_solution(s, sigma));
+    }
+    if (info) {
+        memset(info, 0, sizeof(*info));
+        info->iterations = iters;
+        info->converged = rc == 0;
+        info->residual = (bnorm > 0 ? err1 / sqrt(bnorm) : err1);
+        info->solve_ms = ms;
+    }
+    if (rc == 2) return dep_fail(c, DEP_ERR_NOCONV, "GMRES: CONVERGENCE RATE TOO SLOW after %d iterations", iters);
+    if (rc == 1) return dep_fail(c, DEP_ERR_NOCONV, "GMRES: no convergence in %d iterations", M);
+    return DEP_OK;
+}

@@ -1,0 +1,263 @@
+// dep_assemble.hip -- FP64 collocation assembly of the geometric blocks P.
+//
+// Replaces the (row node i) x (element j) double loop of
+// electricForm[Full]A_tria{3,6} (src/assembly_el.c:515-575) and the calls it
+// makes into intG/intH (regular) and intSingularG/intSingularH (singular).
+//
+// P[i][c] = sum over elements e containing node c of the integral over e of
+//           N_{k(e,c)} * G(x_i, y)           (Dirichlet row i), or
+//           -N_{k(e,c)} * (d.n_i)/r^3        (interface row i, n_i = node normal)
+// with G = 1/r and d = x_i - y.  Frequency and materials enter only through
+// per-row weights applied later (dep_solve.hip), so P is assembled once per
+// geometry and reused across frequencies.
+//
+// Regular pairs: a workgroup owns TR consecutive rows (one per thread) and a
+// column patch of DEP_TC consecutive nodes.  It walks the elements touching
+// the patch (in element order), evaluates the 7-point rule for its row from
+// the precomputed quadrature points (wave-uniform loads), and accumulates
+// into an LDS tile owned row-by-row by the threads, so every sum has a fixed
+// order (no atomics).  Elements straddling patches are evaluated once per
+// patch they touch; the Morton node order keeps that overhead small.
+//
+// Singular pairs (node i is a node of e): one thread per (e, position) pair
+// evaluates intSingularG/intSingularH; a scatter kernel then adds them to the
+// rows, one thread per row.
+#include "dep_internal.h"
+#include "dep_quad.h"
+
+namespace {
+
+// quadrature points: q[(e*7+g)] = (x, y, z, W*J) with the reference's map
+template <int NPE>
+__global__ void k_quad_points(int E, const double* __restrict__ xyz, const int* __restrict__ conn,
+                              double4* __restrict__ q)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= E * 7) return;
+    const int e = t / 7, g = t % 7;
+    double X[NPE][3];
+#pragma unroll
+    for (int p = 0; p < NPE; p++) {
+        const int v = conn[e * NPE + p];
+        X[p][0] = xyz[3 * v];
+        X[p][1] = xyz[3 * v + 1];
+        X[p][2] = xyz[3 * v + 2];
+    }
+    double N[NPE], xl[3];
+    const double a = kT7[g][0], b = kT7[g][1];
+    shape_fn<NPE>(a, b, N);
+    const double J = map_point<NPE>(X, a, b, N, xl);
+    q[t] = make_double4(xl[0], xl[1], xl[2], kT7[g][2] * J);
+}
+
+template <int NPE>
+__global__ void k_node_normals(int N, const double* __restrict__ xyz, const int* __restrict__ conn,
+                               const int* __restrict__ inc_ptr, const int* __restrict__ inc_ek,
+                               double* __restrict__ normal)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const int p0 = inc_ptr[i], p1 = inc_ptr[i + 1];
+    double n[3] = {0.0, 0.0, 0.0};
+    if (p1 > p0) node_normal<NPE>(xyz, conn, inc_ek + p0, p1 - p0, n);
+    normal[3 * i] = n[0];
+    normal[3 * i + 1] = n[1];
+    normal[3 * i + 2] = n[2];
+}
+
+// Shape-function values at the 7 regular points, folded at compile time
+template <int NPE>
+struct ShapeTable {
+    double v[7][NPE];
+    constexpr ShapeTable() : v{}
+    {
+        for (int g = 0; g < 7; g++) {
+            const double a = kT7[g][0], b = kT7[g][1], c = 1.0 - a - b;
+            if (NPE == 3) {
+                v[g][0] = a;
+                v[g][1] = b;
+                v[g][2 % NPE] = c;
+            } else {
+                v[g][0] = a * (2.0 * a - 1.0);
+                v[g][1] = 4.0 * a * b;
+                v[g][2 % NPE] = b * (2.0 * b - 1.0);
+                v[g][3 % NPE] = 4.0 * b * c;
+                v[g][4 % NPE] = c * (2.0 * c - 1.0);
+                v[g][5 % NPE] = 4.0 * a * c;
+            }
+        }
+    }
+};
+
+template <int NPE, int TR>
+__global__ __launch_bounds__(TR) void k_assemble_regular(
+    int N, const double* __restrict__ xyz, const int* __restrict__ conn,
+    const int8_t* __restrict__ kind, const double* __restrict__ normal,
+    const double4* __restrict__ q, const int* __restrict__ patch_ptr,
+    const int* __restrict__ patch_elem, const int8_t* __restrict__ patch_slot,
+    double* __restrict__ P, long ldP)
+{
+    constexpr ShapeTable<NPE> SF{};
+    __shared__ double acc[DEP_TC][TR + 1];
+    const int patch = blockIdx.x;
+    const int t = threadIdx.x;
+    const int row = blockIdx.y * TR + t;
+    const bool active = row < N;
+#pragma unroll 8
+    for (int c = 0; c < DEP_TC; c++) acc[c][t] = 0.0;
+
+    double xe0 = 0.0, xe1 = 0.0, xe2 = 0.0, n0 = 0.0, n1 = 0.0, n2 = 0.0;
+    bool iface = false;
+    if (active) {
+        xe0 = xyz[3 * row];
+        xe1 = xyz[3 * row + 1];
+        xe2 = xyz[3 * row + 2];
+        iface = kind[row] != 0;
+        if (iface) {
+            n0 = normal[3 * row];
+            n1 = normal[3 * row + 1];
+            n2 = normal[3 * row + 2];
+        }
+    }
+    const int j0 = patch_ptr[patch], j1 = patch_ptr[patch + 1];
+    for (int j = j0; j < j1; j++) {
+        const int e = patch_elem[j];
+        bool sing = false;
+#pragma unroll
+        for (int k = 0; k < NPE; k++) sing |= (conn[e * NPE + k] == row);
+        double sub[NPE];
+#pragma unroll
+        for (int k = 0; k < NPE; k++) sub[k] = 0.0;
+        if (active && !sing) {
+#pragma unroll
+            for (int g = 0; g < 7; g++) {
+                const double4 p = q[e * 7 + g];
+                const double dx = xe0 - p.x, dy = xe1 - p.y, dz = xe2 - p.z;
+                const double rsq = dx * dx + dy * dy + dz * dz;
+                const double r = sqrt(rsq);
+                if (!iface) {
+                    const double A = p.w / r;
+#pragma unroll
+                    for (int k = 0; k < NPE; k++) sub[k] += A * SF.v[g][k];
+                } else {
+                    double A = p.w / (rsq * r);
+                    A = A * (dx * n0 + dy * n1 + dz * n2);
+#pragma unroll
+                    for (int k = 0; k < NPE; k++) sub[k] -= A * SF.v[g][k];
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NPE; k++) {
+            const int s = patch_slot[j * NPE + k];
+            if (s >= 0) acc[s][t] += sub[k];
+        }
+    }
+    __syncthreads();
+    // coalesced write-out of the TR x DEP_TC tile
+    const int c0 = patch * DEP_TC;
+    constexpr int RPP = TR / DEP_TC;   // rows per pass
+    const int cc = t % DEP_TC, rr = t / DEP_TC;
+    for (int r = rr; r < TR; r += RPP) {
+        const int gi = blockIdx.y * TR + r, gc = c0 + cc;
+        if (gi < N && gc < N) P[(long)gi * ldP + gc] = acc[cc][r];
+    }
+}
+
+// one thread per (position k, element e): the singular integral of row conn[e][k]
+template <int NPE>
+__global__ __launch_bounds__(64) void k_assemble_singular(int E, const double* __restrict__ xyz,
+                                    const int* __restrict__ conn, const int8_t* __restrict__ kind,
+                                    const double* __restrict__ normal, double* __restrict__ S)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= NPE * E) return;
+    const int k = t / E, e = t % E;
+    double X[NPE][3];
+#pragma unroll
+    for (int p = 0; p < NPE; p++) {
+        const int v = conn[e * NPE + p];
+        X[p][0] = xyz[3 * v];
+        X[p][1] = xyz[3 * v + 1];
+        X[p][2] = xyz[3 * v + 2];
+    }
+    const int row = conn[e * NPE + k];
+    const double xe[3] = {xyz[3 * row], xyz[3 * row + 1], xyz[3 * row + 2]};
+    double out[NPE];
+    if (kind[row] == 0) {
+        singular_G<NPE>(k + 1, X, xe, out);
+    } else {
+        const double n[3] = {normal[3 * row], normal[3 * row + 1], normal[3 * row + 2]};
+        singular_H<NPE, 0>(k + 1, X, xe, n, out);
+    }
+#pragma unroll
+    for (int m = 0; m < NPE; m++) S[(long)t * NPE + m] = out[m];
+}
+
+// one thread per row: add the singular blocks of the incident elements
+template <int NPE>
+__global__ void k_scatter_singular(int N, int E, const int* __restrict__ conn,
+                                   const int* __restrict__ inc_ptr, const int* __restrict__ inc_ek,
+                                   const double* __restrict__ S, double* __restrict__ P, long ldP)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    double* Pi = P + (long)i * ldP;
+    for (int t = inc_ptr[i]; t < inc_ptr[i + 1]; t++) {
+        const int e = inc_ek[t] >> 3, k = inc_ek[t] & 7;
+        const double* s = S + ((long)k * E + e) * NPE;
+#pragma unroll
+        for (int m = 0; m < NPE; m++) Pi[conn[e * NPE + m]] += s[m];
+    }
+}
+
+template <int NPE>
+hipError_t assemble_impl(dep_geom* g)
+{
+    hipStream_t st = g->ctx->stream;
+    constexpr int TR = 128;
+    dim3 grid(g->n_patch, dep_cdiv(g->N, TR));
+    hipLaunchKernelGGL((k_assemble_regular<NPE, TR>), grid, dim3(TR), 0, st, g->N, g->xyz, g->conn,
+                       g->kind, g->normal, (const double4*)g->q, g->patch_ptr, g->patch_elem,
+                       g->patch_slot, g->P, g->ldP);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const int np = NPE * g->E;
+    hipLaunchKernelGGL((k_assemble_singular<NPE>), dim3(dep_cdiv(np, 64)), dim3(64), 0, st, g->E,
+                       g->xyz, g->conn, g->kind, g->normal, g->S);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_scatter_singular<NPE>), dim3(dep_cdiv(g->N, 128)), dim3(128), 0, st, g->N,
+                       g->E, g->conn, g->inc_ptr, g->inc_ek, g->S, g->P, g->ldP);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_quad_points(dep_geom* g)
+{
+    const int n = g->E * 7;
+    if (g->npe == 3)
+        hipLaunchKernelGGL(k_quad_points<3>, dim3(dep_cdiv(n, 256)), dim3(256), 0, g->ctx->stream,
+                           g->E, g->xyz, g->conn, (double4*)g->q);
+    else
+        hipLaunchKernelGGL(k_quad_points<6>, dim3(dep_cdiv(n, 256)), dim3(256), 0, g->ctx->stream,
+                           g->E, g->xyz, g->conn, (double4*)g->q);
+    return hipGetLastError();
+}
+
+hipError_t launch_node_normals(dep_geom* g)
+{
+    if (g->npe == 3)
+        hipLaunchKernelGGL(k_node_normals<3>, dim3(dep_cdiv(g->N, 128)), dim3(128), 0,
+                           g->ctx->stream, g->N, g->xyz, g->conn, g->inc_ptr, g->inc_ek, g->normal);
+    else
+        hipLaunchKernelGGL(k_node_normals<6>, dim3(dep_cdiv(g->N, 128)), dim3(128), 0,
+                           g->ctx->stream, g->N, g->xyz, g->conn, g->inc_ptr, g->inc_ek, g->normal);
+    return hipGetLastError();
+}
+
+hipError_t launch_assemble(dep_geom* g)
+{
+    return g->npe == 3 ? assemble_impl<3>(g) : assemble_impl<6>(g);
+}

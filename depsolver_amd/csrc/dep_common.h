@@ -1,0 +1,132 @@
+// dep_common.h -- shared definitions for the depSolver MI355X hot path.
+//
+// Element library and quadrature data used by every kernel.  The quadrature
+// tables are the reference's decimals (src/gaussData.h:38-74) and the
+// constants are src/constants.h:42-55; they have to be the same numbers for
+// the FP64 path to reproduce the reference to rounding.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define DEP_EPS0 8.854187817E-12
+#define DEP_PI2 6.283185307179586E+00
+#define DEP_PI4 1.256637061435917E+01
+
+#define DEP_NGAUSS 7   // regular triangle rule (TNGAUSS)
+#define DEP_NSUB 8     // subdivision levels of the strongly singular rule
+
+// 7-point degree-5 triangle rule (L1, L2, W), weights sum to 1/2
+constexpr double kT7[7][3] = {
+    {3.33333333333333E-01, 3.33333333333333E-01, 1.1250000000000E-01},
+    {4.70142064105115E-01, 4.70142064105115E-01, 6.6197076394253E-02},
+    {5.97158717897700E-02, 4.70142064105115E-01, 6.6197076394253E-02},
+    {4.70142064105115E-01, 5.97158717897700E-02, 6.6197076394253E-02},
+    {1.01286507323456E-01, 1.01286507323456E-01, 6.2969590272414E-02},
+    {7.97426985353088E-01, 1.01286507323456E-01, 6.2969590272414E-02},
+    {1.01286507323456E-01, 7.97426985353088E-01, 6.2969590272414E-02}};
+
+// 8-point Gauss-Legendre (x, w) and Gauss-Jacobi with weight (1+x) (x, w)
+constexpr double kGL8[8][2] = {
+    {-9.60289856497536E-01, 1.01228536290370E-01}, {-7.96666477413627E-01, 2.22381034453376E-01},
+    {-5.25532409916329E-01, 3.13706645877887E-01}, {-1.83434642495650E-01, 3.62683783378362E-01},
+    {1.83434642495650E-01, 3.62683783378362E-01},  {5.25532409916329E-01, 3.13706645877887E-01},
+    {7.96666477413627E-01, 2.22381034453376E-01},  {9.60289856497536E-01, 1.01228536290370E-01}};
+constexpr double kGJ8[8][2] = {
+    {-9.10732089420060E-01, 1.31807657689955E-02}, {-7.11267485915709E-01, 7.13716106239446E-02},
+    {-4.26350485711139E-01, 1.81757278018796E-01}, {-9.03733696068533E-02, 3.16798397969277E-01},
+    {2.56135670833455E-01, 4.24189437743720E-01},  {5.71383041208738E-01, 4.50023197883551E-01},
+    {8.17352784200412E-01, 3.64476094545495E-01},  {9.64440169705273E-01, 1.78203217446225E-01}};
+
+// ---------------------------------------------------------------------------
+// Element library (device).  X is npe x 3, node order of the reference.
+// ---------------------------------------------------------------------------
+
+template <int NPE>
+__device__ __forceinline__ void shape_fn(double a, double b, double* N)
+{
+    if constexpr (NPE == 3) {
+        N[0] = a;
+        N[1] = b;
+        N[2] = 1.0 - a - b;
+    } else {
+        const double c = 1.0 - a - b;
+        N[0] = a * (2.0 * a - 1.0);
+        N[1] = 4.0 * a * b;
+        N[2] = b * (2.0 * b - 1.0);
+        N[3] = 4.0 * b * c;
+        N[4] = c * (2.0 * c - 1.0);
+        N[5] = 4.0 * a * c;
+    }
+}
+
+// Tangents dx/dL1, dx/dL2 at (a, b)
+template <int NPE>
+__device__ __forceinline__ void tangents(const double (*X)[3], double a, double b, double* t1,
+                                         double* t2)
+{
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        if constexpr (NPE == 3) {
+            t1[c] = X[0][c] - X[2][c];
+            t2[c] = X[1][c] - X[2][c];
+        } else {
+            const double s = X[1][c] - X[3][c] + X[4][c] - X[5][c];
+            t1[c] = 4.0 * (a * (X[0][c] + X[4][c] - 2.0 * X[5][c]) + b * s + X[5][c]) - X[0][c] -
+                    3.0 * X[4][c];
+            t2[c] = 4.0 * (a * s + b * (X[2][c] + X[4][c] - 2.0 * X[3][c]) + X[3][c]) - X[2][c] -
+                    3.0 * X[4][c];
+        }
+    }
+}
+
+// x(L) and the surface Jacobian |t1 x t2|
+template <int NPE>
+__device__ __forceinline__ double map_point(const double (*X)[3], double a, double b,
+                                            const double* N, double* xl)
+{
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        double s = N[0] * X[0][c];
+#pragma unroll
+        for (int k = 1; k < NPE; k++) s += N[k] * X[k][c];
+        xl[c] = s;
+    }
+    double t1[3], t2[3];
+    tangents<NPE>(X, a, b, t1, t2);
+    const double j1 = t1[1] * t2[2] - t2[1] * t1[2];
+    const double j2 = t2[0] * t1[2] - t1[0] * t2[2];
+    const double j3 = t1[0] * t2[1] - t2[0] * t1[1];
+    return sqrt(j1 * j1 + j2 * j2 + j3 * j3);
+}
+
+template <int NPE>
+__device__ __forceinline__ void local_normal(const double (*X)[3], double a, double b, double* n)
+{
+    double t1[3], t2[3];
+    tangents<NPE>(X, a, b, t1, t2);
+    n[0] = t1[1] * t2[2] - t2[1] * t1[2];
+    n[1] = t2[0] * t1[2] - t1[0] * t2[2];
+    n[2] = t1[0] * t2[1] - t2[0] * t1[1];
+    const double inv = 1.0 / sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    n[0] *= inv;
+    n[1] *= inv;
+    n[2] *= inv;
+}
+
+// Local coordinates of node k (normals.c:244-267)
+template <int NPE>
+__device__ __forceinline__ void node_local(int k, double* a, double* b)
+{
+    if constexpr (NPE == 3) {
+        *a = (k == 0) ? 1.0 : 0.0;
+        *b = (k == 1) ? 1.0 : 0.0;
+    } else {
+        const double la[6] = {1.0, 0.5, 0.0, 0.0, 0.0, 0.5};
+        const double lb[6] = {0.0, 0.5, 1.0, 0.5, 0.0, 0.0};
+        *a = la[k];
+        *b = lb[k];
+    }
+}
+
+static inline int dep_cdiv(long a, long b) { return (int)((a + b - 1) / b); }

@@ -1,0 +1,141 @@
+// dep_internal.h -- object layouts and kernel entry points shared by the
+// translation units of libdepsolver_amd.so.
+//
+// Device data layout (all node-indexed data are in the library's internal
+// node order, a Morton order of the node coordinates; `perm[new] = old`):
+//   xyz      N x 3 FP64
+//   conn     E x npe int32 (internal node ids; elements keep their order)
+//   q        E x 7 quadrature points, 4 doubles each: x, y, z, W*J
+//   P        N x ldP FP64 row-major geometric blocks: row i holds the
+//            collocation sums of node i (G rows for Dirichlet nodes, dG/dn
+//            rows with node i's normal for interface nodes), no weights
+//   Zr, Zi   N x ldZ split-plane complex working matrix / LU factors
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/depsolver_amd.h"
+
+#define DEP_TC 64   // columns per assembly patch
+
+struct dep_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    size_t bytes = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+struct dep_geom {
+    dep_ctx* ctx = nullptr;
+    int N = 0, E = 0, npe = 0;
+    std::vector<int> perm, iperm;     // perm[new] = old, iperm[old] = new
+    std::vector<int> hconn;           // E x npe internal ids (host copy)
+    std::vector<int8_t> hkind;        // N: 0 Dirichlet, 1 interface (internal order)
+    std::vector<int> hiface;          // N: interface id (internal order)
+    std::vector<uint8_t> hbc6;        // N: bc type 6 (internal order)
+    std::vector<double> hxyz;         // N x 3 (internal order)
+    std::vector<int> hinc_ptr, hinc_ek;  // host copy of the incidence lists
+    // device
+    double* xyz = nullptr;
+    int* conn = nullptr;
+    int8_t* kind = nullptr;
+    double* normal = nullptr;         // N x 3 averaged node normals
+    double* q = nullptr;              // E x 7 x 4
+    int* inc_ptr = nullptr;           // node -> incident (element, position), by element
+    int* inc_ek = nullptr;            // packed e*8 + k
+    int* patch_ptr = nullptr;         // column patch -> elements touching it
+    int* patch_elem = nullptr;
+    int8_t* patch_slot = nullptr;     // per patch element: npe column slots (-1 outside)
+    int n_patch = 0;
+    double* S = nullptr;              // npe*E*npe singular integrals
+    double* P = nullptr;
+    long ldP = 0;
+    bool assembled = false;
+    std::vector<std::pair<void*, size_t>> allocs;
+    dep_ctx* ctx_of() { return ctx; }
+};
+
+struct dep_sys {
+    dep_geom* g = nullptr;
+    int nI = 0, nM = 0;
+    std::vector<double> mats;         // M x 2
+    std::vector<uint32_t> ifaces;     // I x 2
+    double freq = 0.0;
+    // per-row complex weight (wr + i wi) and diagonal shift, internal order
+    double *wr = nullptr, *wi = nullptr, *shift = nullptr;
+    std::vector<double> hwr, hwi, hshift;
+    double* b = nullptr;              // 2N RHS (re, im), internal order
+    double* x = nullptr;              // 2N solution, internal order
+    bool dense = false;               // Zr/Zi hold a caller-supplied dense matrix (no P)
+    // factor
+    int prec = -1;
+    bool factored = false;
+    long ldZ = 0;
+    double *Zr = nullptr, *Zi = nullptr;    // FP64 factors
+    float *Zr32 = nullptr, *Zi32 = nullptr; // FP32 factors
+    float *rs = nullptr, *cs = nullptr;     // equilibration (row, column)
+    void *opA = nullptr, *opB = nullptr;    // low-precision GEMM operand panels
+    double min_pivot = 0.0;
+    double factor_ms = 0.0;
+    // solve workspace
+    double* work = nullptr;           // 8 vectors of 2N
+    double* dwork = nullptr;          // reduction scratch / pivots
+    double* qs = nullptr;             // E x 7 complex sigma at quadrature points
+    size_t bytes = 0;                 // device bytes owned by the system
+    std::vector<std::pair<void*, size_t>> allocs;
+    dep_ctx* ctx_of() { return g->ctx; }
+};
+
+// ---- error helpers ---------------------------------------------------------
+int dep_fail(dep_ctx* c, int code, const char* fmt, ...);
+#define DEP_HIP(ctx, call)                                                          \
+    do {                                                                            \
+        hipError_t e_ = (call);                                                     \
+        if (e_ != hipSuccess)                                                       \
+            return dep_fail((ctx), DEP_ERR_HIP, "%s failed: %s (%s:%d)", #call,     \
+                            hipGetErrorString(e_), __FILE__, __LINE__);             \
+    } while (0)
+#define DEP_CHECK(expr)             \
+    do {                            \
+        int r_ = (expr);            \
+        if (r_ != DEP_OK) return r_; \
+    } while (0)
+
+int dep_malloc(dep_ctx* c, void** p, size_t bytes);
+void dep_free(dep_ctx* c, void* p, size_t bytes);
+// allocate and remember in an object's allocation list (freed with the object)
+template <class Obj, class T>
+int dep_obj_alloc(Obj* o, T** p, size_t count)
+{
+    void* v = nullptr;
+    const size_t bytes = count * sizeof(T) > 0 ? count * sizeof(T) : 16;
+    int r = dep_malloc(o->ctx_of(), &v, bytes);
+    if (r != DEP_OK) return r;
+    o->allocs.emplace_back(v, bytes);
+    *p = (T*)v;
+    return DEP_OK;
+}
+
+// ---- launchers (return hipError_t of the launch) ---------------------------
+hipError_t launch_quad_points(dep_geom* g);
+hipError_t launch_node_normals(dep_geom* g);
+hipError_t launch_assemble(dep_geom* g);
+hipError_t launch_weight_fp64(dep_sys* s);            // Z = w * P + shift (FP64 planes)
+hipError_t launch_lu_fp64(dep_sys* s, double* min_pivot);
+hipError_t launch_trsv_fp64(dep_sys* s, const double* rhs, double* out);
+hipError_t launch_residual(dep_sys* s, const double* x, double* r);   // r = b - Z x (FP64, via P)
+hipError_t launch_matvec(dep_sys* s, const double* x, double* y);     // y = Z x
+hipError_t launch_zmatvec_w(dep_sys* s, const double* wr, const double* wi, const double* shift,
+                            const double* x, const double* b, double* y);
+hipError_t launch_dots(dep_ctx* c, int nvec, long n2, const double* V, const double* w, double* h);
+hipError_t launch_update(dep_ctx* c, int nvec, long n2, const double* V, const double* h, double* w);
+hipError_t launch_scale(dep_ctx* c, long n2, const double* w, double alpha, double* out);
+hipError_t launch_combine(dep_ctx* c, int nvec, long n2, const double* V, const double* y, double* x);
+hipError_t launch_dense_zmatvec(dep_ctx* c, int N, const double* Zr, const double* Zi, long ldZ,
+                                const double* x, const double* b, double* y);

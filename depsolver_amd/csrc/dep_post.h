@@ -1,0 +1,34 @@
+// dep_post.h -- evaluation-point sets for post-processing.
+#pragma once
+
+#include "dep_internal.h"
+
+// A prepared set of evaluation points (device arrays owned by the set).
+// Elements with a node at exactly a point's coordinates are that point's
+// singular elements: listed per point in element order (excl_*), and as
+// (point, element*8 + first coincident position) pairs in the same order.
+struct dep_points {
+    dep_ctx* ctx = nullptr;
+    int m = 0;
+    int nchunk = 1;
+    int npairs = 0;
+    double* pts = nullptr;       // m x 3
+    int* excl_ptr = nullptr;     // m + 1
+    int* excl_e = nullptr;       // npairs
+    int* pair_pt = nullptr;      // npairs
+    int* pair_ek = nullptr;      // npairs
+    double* partial = nullptr;   // nchunk x m x 6
+    double* sing = nullptr;      // npairs x 6
+    double* out = nullptr;       // m x 6
+    size_t bytes = 0;
+    std::vector<std::pair<void*, size_t>> allocs;
+};
+
+int dep_points_prepare(dep_geom* g, int m, const double* pts_internal_coords, dep_points* P);
+void dep_points_release(dep_points* P);
+
+hipError_t launch_sigma_at_points(dep_sys* s);
+hipError_t launch_points(dep_sys* s, int mode, const dep_points* P, double* out);
+hipError_t launch_mst(dep_sys* s, const dep_points* P, const double* ef, const int* nodes, int nf,
+                      const int* felems, double half_eps, double* Efn, double* Fce, double* Xce,
+                      double* Fcm);

@@ -1,0 +1,362 @@
+// dep_post.hip -- potential, field and Maxwell-stress force after the solve.
+//
+// Replaces potential_tria{3,6} (src/potential.c:55-179), field_tria{3,6}
+// (src/field.c:57-200) and forceMST_tria{3,6} (src/force.c:348-552 with
+// intF_tria{3,6}, src/forceIntegrals.c:632-723).
+//
+// For a point x the reference sums, over every element e,
+//   phi(x) = C sum_k sigma_k int_e N_k / r                    (C = 1/(4 pi eps0))
+//   E(x)   = C sum_k sigma_k int_e N_k (x - y) / r^3  (+ jump term on a node)
+// With the 7-point rule this is a sum over all element quadrature points y_g
+// of wj_g K(x, y_g) q_g where q_g = sum_k N_k(g) sigma_k is the complex charge
+// density interpolated at y_g, precomputed once per solution.  Points are
+// one per thread; the element range is split into chunks over the grid's y
+// dimension so small point sets still fill the chip, and the chunk partials
+// are reduced in a fixed order.  Elements that have a node at exactly the
+// evaluation point (the reference's coordinate-equality test, potential.c:155,
+// field.c:171) are skipped there and evaluated with the singular rules in a
+// separate kernel, including the reference's jump term (field.c:189-195):
+// B sigma(first node of e) n, once per such element, with n the node normal.
+#include "dep_internal.h"
+#include "dep_post.h"
+#include "dep_quad.h"
+
+namespace {
+
+template <int NPE>
+__global__ void k_sigma_at_points(int E, int N, const int* __restrict__ conn,
+                                  const double* __restrict__ x, double2* __restrict__ qs)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= E * 7) return;
+    const int e = t / 7, g = t % 7;
+    double Nf[NPE];
+    shape_fn<NPE>(kT7[g][0], kT7[g][1], Nf);
+    double sr = 0.0, si = 0.0;
+#pragma unroll
+    for (int k = 0; k < NPE; k++) {
+        const int v = conn[e * NPE + k];
+        sr += Nf[k] * x[v];
+        si += Nf[k] * x[N + v];
+    }
+    qs[t] = make_double2(sr, si);
+}
+
+// MODE 0: potential (2 outputs), MODE 1: field (6 outputs: ReEx ImEx ReEy ImEy ReEz ImEz)
+template <int MODE>
+__global__ __launch_bounds__(256) void k_points_regular(
+    int M, const double* __restrict__ pts, int E, int chunk, const double4* __restrict__ q,
+    const double2* __restrict__ qs, const int* __restrict__ excl_ptr,
+    const int* __restrict__ excl_e, double* __restrict__ partial)
+{
+    constexpr int NO = MODE ? 6 : 2;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = blockIdx.y;
+    const int e0 = s * chunk, e1 = min(E, e0 + chunk);
+    double x0 = 0.0, x1 = 0.0, x2 = 0.0;
+    int xp = 0, xend = 0;
+    if (p < M) {
+        x0 = pts[3 * p];
+        x1 = pts[3 * p + 1];
+        x2 = pts[3 * p + 2];
+        xp = excl_ptr[p];
+        xend = excl_ptr[p + 1];
+        while (xp < xend && excl_e[xp] < e0) xp++;
+    }
+    double acc[NO];
+#pragma unroll
+    for (int o = 0; o < NO; o++) acc[o] = 0.0;
+    for (int e = e0; e < e1; e++) {
+        bool skip = false;
+        if (xp < xend && excl_e[xp] == e) {
+            skip = true;
+            xp++;
+        }
+        if (skip || p >= M) continue;
+#pragma unroll
+        for (int g = 0; g < 7; g++) {
+            const double4 y = q[e * 7 + g];
+            const double2 sg = qs[e * 7 + g];
+            const double dx = x0 - y.x, dy = x1 - y.y, dz = x2 - y.z;
+            const double rsq = dx * dx + dy * dy + dz * dz;
+            const double r = sqrt(rsq);
+            if constexpr (MODE == 0) {
+                const double A = y.w / r;
+                acc[0] += A * sg.x;
+                acc[1] += A * sg.y;
+            } else {
+                const double A = y.w / (rsq * r);
+                const double ar = A * sg.x, ai = A * sg.y;
+                acc[0] += ar * dx;
+                acc[1] += ai * dx;
+                acc[2] += ar * dy;
+                acc[3] += ai * dy;
+                acc[4] += ar * dz;
+                acc[5] += ai * dz;
+            }
+        }
+    }
+    if (p < M)
+#pragma unroll
+        for (int o = 0; o < NO; o++) partial[((long)s * M + p) * NO + o] = acc[o];
+}
+
+// one thread per coincident (point, element) pair
+template <int NPE, int MODE>
+__global__ __launch_bounds__(64) void k_points_singular(int npairs, const int* __restrict__ pair_pt,
+                                  const int* __restrict__ pair_ek, const double* __restrict__ pts,
+                                  int N, const double* __restrict__ xyz, const int* __restrict__ conn,
+                                  const double* __restrict__ normal, const double* __restrict__ x,
+                                  double* __restrict__ out)
+{
+    constexpr int NO = MODE ? 6 : 2;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= npairs) return;
+    const int p = pair_pt[t], e = pair_ek[t] >> 3, k = pair_ek[t] & 7;
+    double X[NPE][3];
+#pragma unroll
+    for (int a = 0; a < NPE; a++) {
+        const int v = conn[e * NPE + a];
+        X[a][0] = xyz[3 * v];
+        X[a][1] = xyz[3 * v + 1];
+        X[a][2] = xyz[3 * v + 2];
+    }
+    const double xe[3] = {pts[3 * p], pts[3 * p + 1], pts[3 * p + 2]};
+    double res[NO];
+    if constexpr (MODE == 0) {
+        double sub[NPE];
+        singular_G<NPE>(k + 1, X, xe, sub);
+        double sr = 0.0, si = 0.0;
+#pragma unroll
+        for (int a = 0; a < NPE; a++) {
+            const int v = conn[e * NPE + a];
+            sr += sub[a] * x[v];
+            si += sub[a] * x[N + v];
+        }
+        res[0] = sr;
+        res[1] = si;
+    } else {
+        const int node = conn[e * NPE + k];
+        const double n[3] = {normal[3 * node], normal[3 * node + 1], normal[3 * node + 2]};
+        double sub[3 * NPE];
+        singular_H<NPE, 1>(k + 1, X, xe, n, sub);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            double sr = 0.0, si = 0.0;
+#pragma unroll
+            for (int a = 0; a < NPE; a++) {
+                const int v = conn[e * NPE + a];
+                sr += sub[3 * a + c] * x[v];
+                si += sub[3 * a + c] * x[N + v];
+            }
+            res[2 * c] = sr;
+            res[2 * c + 1] = si;
+        }
+        // jump term (field.c:189-195): sigma of the element's first node, times 2 pi,
+        // because the sum is scaled by 1/(4 pi eps0) and the jump is 1/(2 eps0)
+        const int v0 = conn[e * NPE];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            res[2 * c] += DEP_PI2 * x[v0] * n[c];
+            res[2 * c + 1] += DEP_PI2 * x[N + v0] * n[c];
+        }
+    }
+#pragma unroll
+    for (int o = 0; o < NO; o++) out[(long)t * NO + o] = res[o];
+}
+
+// out[p] = C * (sum_s partial[s][p] + sum of p's singular pairs), fixed order
+template <int MODE>
+__global__ void k_points_reduce(int M, int nchunk, const double* __restrict__ partial,
+                                const int* __restrict__ excl_ptr, const double* __restrict__ sing,
+                                double scale, double* __restrict__ out)
+{
+    constexpr int NO = MODE ? 6 : 2;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= M) return;
+    double acc[NO];
+#pragma unroll
+    for (int o = 0; o < NO; o++) acc[o] = 0.0;
+    for (int s = 0; s < nchunk; s++)
+#pragma unroll
+        for (int o = 0; o < NO; o++) acc[o] += partial[((long)s * M + p) * NO + o];
+    for (int t = excl_ptr[p]; t < excl_ptr[p + 1]; t++)
+#pragma unroll
+        for (int o = 0; o < NO; o++) acc[o] += sing[(long)t * NO + o];
+#pragma unroll
+    for (int o = 0; o < NO; o++) out[(long)p * NO + o] = scale * acc[o];
+}
+
+// Maxwell stress integral over one force element (intF_tria{3,6}); Ef is Re E at nodes
+template <int NPE>
+__global__ __launch_bounds__(64) void k_mst_elements(int nf, const int* __restrict__ elems, const double* __restrict__ xyz,
+                               const int* __restrict__ conn, const double* __restrict__ Ef,
+                               double half_eps, double* __restrict__ Fce, double* __restrict__ Xce)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nf) return;
+    const int e = elems[t];
+    double X[NPE][3], EE[NPE][3];
+#pragma unroll
+    for (int a = 0; a < NPE; a++) {
+        const int v = conn[e * NPE + a];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            X[a][c] = xyz[3 * v + c];
+            EE[a][c] = Ef[3 * v + c];
+        }
+    }
+    double F0 = 0.0, F1 = 0.0, F2 = 0.0;
+#pragma unroll
+    for (int g = 0; g < 7; g++) {
+        const double a = kT7[g][0], b = kT7[g][1];
+        double Nf[NPE], xl[3], n[3];
+        shape_fn<NPE>(a, b, Nf);
+        const double J = map_point<NPE>(X, a, b, Nf, xl);
+        local_normal<NPE>(X, a, b, n);
+        double ev[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            double s = EE[0][c] * Nf[0];
+#pragma unroll
+            for (int k = 1; k < NPE; k++) s += EE[k][c] * Nf[k];
+            ev[c] = s;
+        }
+        const double NE = ev[0] * n[0] + ev[1] * n[1] + ev[2] * n[2];
+        const double Esq = ev[0] * ev[0] + ev[1] * ev[1] + ev[2] * ev[2];
+        const double A = J * kT7[g][2];
+        F0 += A * (ev[0] * NE - 0.5 * Esq * n[0]);
+        F1 += A * (ev[1] * NE - 0.5 * Esq * n[1]);
+        F2 += A * (ev[2] * NE - 0.5 * Esq * n[2]);
+    }
+    Fce[3 * t] = half_eps * F0;
+    Fce[3 * t + 1] = half_eps * F1;
+    Fce[3 * t + 2] = half_eps * F2;
+    double Nc[NPE], xc[3];
+    shape_fn<NPE>(0.25, 0.25, Nc);
+    map_point<NPE>(X, 0.25, 0.25, Nc, xc);
+    Xce[3 * t] = xc[0];
+    Xce[3 * t + 1] = xc[1];
+    Xce[3 * t + 2] = xc[2];
+}
+
+// scatter Re E of the evaluated points to node slots: Ef[node] = (ReEx, ReEy, ReEz)
+__global__ void k_field_to_nodes(int m, const int* __restrict__ nodes, const double* __restrict__ ef,
+                                 double* __restrict__ Ef)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m) return;
+    const int v = nodes[t];
+    Ef[3 * v] = ef[6 * t];
+    Ef[3 * v + 1] = ef[6 * t + 2];
+    Ef[3 * v + 2] = ef[6 * t + 4];
+}
+
+// single-workgroup fixed-order sum of the element forces
+__global__ __launch_bounds__(256) void k_sum3(int n, const double* __restrict__ v, double* __restrict__ out)
+{
+    __shared__ double sh[3][256];
+    const int t = threadIdx.x;
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (int i = t; i < n; i += 256) {
+        a += v[3 * i];
+        b += v[3 * i + 1];
+        c += v[3 * i + 2];
+    }
+    sh[0][t] = a;
+    sh[1][t] = b;
+    sh[2][t] = c;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) {
+            sh[0][t] += sh[0][t + w];
+            sh[1][t] += sh[1][t + w];
+            sh[2][t] += sh[2][t + w];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        out[0] = sh[0][0];
+        out[1] = sh[1][0];
+        out[2] = sh[2][0];
+    }
+}
+
+}  // namespace
+
+hipError_t launch_sigma_at_points(dep_sys* s)
+{
+    dep_geom* g = s->g;
+    const int n = g->E * 7;
+    if (g->npe == 3)
+        hipLaunchKernelGGL(k_sigma_at_points<3>, dim3(dep_cdiv(n, 256)), dim3(256), 0, g->ctx->stream,
+                           g->E, g->N, g->conn, s->x, (double2*)s->qs);
+    else
+        hipLaunchKernelGGL(k_sigma_at_points<6>, dim3(dep_cdiv(n, 256)), dim3(256), 0, g->ctx->stream,
+                           g->E, g->N, g->conn, s->x, (double2*)s->qs);
+    return hipGetLastError();
+}
+
+// evaluate potential (mode 0) or field (mode 1) at a prepared point set
+hipError_t launch_points(dep_sys* s, int mode, const dep_points* P, double* out)
+{
+    dep_geom* g = s->g;
+    hipStream_t st = g->ctx->stream;
+    const int M = P->m;
+    if (M == 0) return hipSuccess;
+    const int NO = mode ? 6 : 2;
+    const int nchunk = P->nchunk, chunk = dep_cdiv(g->E, nchunk);
+    dim3 grid(dep_cdiv(M, 256), nchunk);
+    if (mode == 0)
+        hipLaunchKernelGGL(k_points_regular<0>, grid, dim3(256), 0, st, M, P->pts, g->E, chunk,
+                           (const double4*)g->q, (const double2*)s->qs, P->excl_ptr, P->excl_e,
+                           P->partial);
+    else
+        hipLaunchKernelGGL(k_points_regular<1>, grid, dim3(256), 0, st, M, P->pts, g->E, chunk,
+                           (const double4*)g->q, (const double2*)s->qs, P->excl_ptr, P->excl_e,
+                           P->partial);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (P->npairs > 0) {
+        dim3 gs(dep_cdiv(P->npairs, 64));
+#define SING(NPE_, MODE_)                                                                         \
+    hipLaunchKernelGGL((k_points_singular<NPE_, MODE_>), gs, dim3(64), 0, st, P->npairs,         \
+                       P->pair_pt, P->pair_ek, P->pts, g->N, g->xyz, g->conn, g->normal, s->x, \
+                       P->sing)
+        if (g->npe == 3) {
+            if (mode == 0) SING(3, 0); else SING(3, 1);
+        } else {
+            if (mode == 0) SING(6, 0); else SING(6, 1);
+        }
+#undef SING
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    const double C = 1.0 / (DEP_PI4 * DEP_EPS0);
+    if (mode == 0)
+        hipLaunchKernelGGL(k_points_reduce<0>, dim3(dep_cdiv(M, 256)), dim3(256), 0, st, M, nchunk,
+                           P->partial, P->excl_ptr, P->sing, C, out);
+    else
+        hipLaunchKernelGGL(k_points_reduce<1>, dim3(dep_cdiv(M, 256)), dim3(256), 0, st, M, nchunk,
+                           P->partial, P->excl_ptr, P->sing, C, out);
+    (void)NO;
+    return hipGetLastError();
+}
+
+hipError_t launch_mst(dep_sys* s, const dep_points* P, const double* ef, const int* nodes,
+                      int nf, const int* felems, double half_eps, double* Efn, double* Fce,
+                      double* Xce, double* Fcm)
+{
+    dep_geom* g = s->g;
+    hipStream_t st = g->ctx->stream;
+    hipLaunchKernelGGL(k_field_to_nodes, dim3(dep_cdiv(P->m, 256)), dim3(256), 0, st, P->m, nodes,
+                       ef, Efn);
+    if (g->npe == 3)
+        hipLaunchKernelGGL(k_mst_elements<3>, dim3(dep_cdiv(nf, 64)), dim3(64), 0, st, nf, felems,
+                           g->xyz, g->conn, Efn, half_eps, Fce, Xce);
+    else
+        hipLaunchKernelGGL(k_mst_elements<6>, dim3(dep_cdiv(nf, 64)), dim3(64), 0, st, nf, felems,
+                           g->xyz, g->conn, Efn, half_eps, Fce, Xce);
+    hipLaunchKernelGGL(k_sum3, dim3(1), dim3(256), 0, st, nf, Fce, Fcm);
+    return hipGetLastError();
+}

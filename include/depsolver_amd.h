@@ -1,0 +1,149 @@
+/*
+ * depsolver_amd.h -- C ABI of the MI355X BEM hot path (libdepsolver_amd.so).
+ *
+ * Two layers:
+ *
+ *  1. The native API (dep_*): opaque context, device-resident geometry,
+ *     systems, factors and solutions.  Plain pointers and sizes only; every
+ *     call returns DEP_OK (0) or a negative DEP_ERR_* code and never exits the
+ *     process; dep_last_error() gives the message.  Host buffers are owned by
+ *     the caller and only read/written during the call.  Contexts are
+ *     independent; one context must not be used by two threads at once.
+ *
+ *  2. Drop-in shims with the reference's exact signatures (bottom of this
+ *     file): electricForm[Full]A_tria{3,6}, gaussBksb, solverGMRES_el and
+ *     [vtk]postProcess_tria{3,6} as src/depSolver.c calls them, reading the
+ *     same globals (nNodes, nElems, nInterfaces, ...) from the executable.
+ *
+ * Conventions of the native API
+ *   - node ids in `conn` are 0-based; material and interface ids in
+ *     `ifaces` are 1-based as in the reference's INTERFACES section;
+ *   - complex node vectors (RHS, solution sigma) are 2N doubles, the real
+ *     block then the imaginary block, as the reference's vB;
+ *   - the unknown is the surface charge density sigma; Z sigma = b with
+ *     Z = F - iG in the reference's notation (src/assembly_el.c:565-573).
+ */
+#ifndef DEPSOLVER_AMD_H
+#define DEPSOLVER_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define DEP_OK 0
+#define DEP_ERR_ARG (-1)        /* bad argument or size */
+#define DEP_ERR_HIP (-2)        /* HIP runtime error */
+#define DEP_ERR_NOMEM (-3)      /* device or host allocation failed */
+#define DEP_ERR_SINGULAR (-4)   /* zero or non-finite pivot */
+#define DEP_ERR_NOCONV (-5)     /* iterative solver did not converge */
+#define DEP_ERR_STATE (-6)      /* call out of order (e.g. solve before factor) */
+#define DEP_ERR_ADMISSION (-7)  /* request does not fit the device memory budget */
+
+/* factor precisions */
+#define DEP_PREC_FP64 0   /* FP64 LU on FP64 MFMA: the plain path */
+#define DEP_PREC_FP32 1   /* FP32 LU on FP32 MFMA + FP64 refinement */
+#define DEP_PREC_FP16 2   /* FP32 panels, FP16 MFMA trailing update (FP32 accumulate) + FP64 refinement */
+#define DEP_PREC_BF16 3   /* as FP16 with BF16 operands */
+
+typedef struct dep_ctx dep_ctx;
+typedef struct dep_geom dep_geom;
+typedef struct dep_sys dep_sys;
+
+typedef struct dep_solve_info {
+    int iterations;        /* GMRES iterations, or refinement steps */
+    int inner_iterations;  /* GMRES iterations inside GMRES-IR */
+    int fallback;          /* 0 none, 1 GMRES-IR used, 2 FP64 LU used */
+    int converged;         /* 1 if the FP64 residual reached the tolerance */
+    double residual;       /* ||b - Z x|| / ||b|| in FP64 */
+    double min_pivot;      /* smallest |U_ii| seen by the factorization */
+    double factor_ms;      /* device time of the last factorization */
+    double solve_ms;       /* device time of the last solve */
+} dep_solve_info;
+
+/* ---- context ------------------------------------------------------------ */
+int dep_ctx_create(int device, dep_ctx **out);
+int dep_ctx_destroy(dep_ctx *ctx);
+const char *dep_last_error(const dep_ctx *ctx);
+int dep_ctx_synchronize(dep_ctx *ctx);
+/* device memory in use by this context, bytes */
+size_t dep_ctx_device_bytes(const dep_ctx *ctx);
+
+/* ---- geometry: replaces the geometric half of electricForm[Full]A_tria{3,6}
+ *      (src/assembly_el.c:470-589, :329-448, :197-307, :63-174) ------------ */
+/* bctype: N bc types of the real rows (1 Dirichlet, 0/6 interface, 6 = force
+ * wanted); iface: N 0-based interface ids (ignored for Dirichlet rows). */
+int dep_geom_create(dep_ctx *ctx, int n_nodes, const double *xyz, int n_elems, int npe,
+                    const int32_t *conn, const uint32_t *bctype, const int32_t *iface,
+                    dep_geom **out);
+int dep_geom_destroy(dep_geom *g);
+/* FP64 geometric integral blocks on the device: the collocation sums of
+ * kernelIntegrals.c (intG/intH regular, intSingularG/H singular) without the
+ * material/frequency weights.  Idempotent. */
+int dep_geom_assemble(dep_geom *g);
+/* Copy the geometric matrix (N x N, row-major, reference node order). */
+int dep_geom_get_blocks(dep_geom *g, double *P);
+
+/* ---- systems ------------------------------------------------------------ */
+/* mats: M x 2 (sigma, eps_r); ifaces: I x 2 1-based material ids; rhs: 2N */
+int dep_sys_create(dep_geom *g, int n_mats, const double *mats, int n_ifaces,
+                   const uint32_t *ifaces, double freq, const double *rhs, dep_sys **out);
+int dep_sys_destroy(dep_sys *s);
+/* re-weight the device-resident geometric blocks for a new frequency */
+int dep_sys_set_frequency(dep_sys *s, double freq);
+/* The assembled matrix in the reference's layouts (src/depSolver.c:276-277):
+ * layout 0: 2N x 2N [[F, G], [-G, F]] (gaussBksb); layout 1: N x 2N [F, G]
+ * (solverGMRES_el). */
+int dep_sys_get_matrix(dep_sys *s, int layout, double *mA);
+
+/* ---- dense solve: replaces gaussBksb (src/gaussBksb.c:39-108) ----------- */
+int dep_factor(dep_sys *s, int precision);
+/* FP64 iterative refinement on the factor to ||b - Zx|| <= tol ||b||
+ * (a single triangular solve for DEP_PREC_FP64), GMRES-IR and then the FP64
+ * LU as fallbacks.  sigma (2N) may be NULL. */
+int dep_solve(dep_sys *s, double tol, int max_steps, double *sigma, dep_solve_info *info);
+
+/* ---- GMRES: replaces solverGMRES_el (src/gmres_el.c:66-128) ------------- */
+/* Unrestarted GMRES in FP64 on the N x 2N system, tolerance tol*||b|| (the
+ * reference hard-codes 1e-6), optional Jacobi scaling (precond = 1) applied
+ * as the reference does, optional initial guess x0 (2N), residual history in
+ * the reference's gmres.log format when log_path is not NULL. */
+int dep_gmres(dep_sys *s, int precond, double tol, int max_iters, const double *x0,
+              double *sigma, dep_solve_info *info, const char *log_path);
+
+int dep_get_solution(dep_sys *s, double *sigma);
+int dep_set_solution(dep_sys *s, const double *sigma);
+
+/* ---- post-processing: replaces potential_*, field_*, forceMST_*,
+ *      forceMultipole_* (src/potential.c, src/field.c, src/force.c) ------- */
+int dep_post_potential(dep_sys *s, int m, const double *pts, double *pot /* m x 2 */);
+int dep_post_field(dep_sys *s, int m, const double *pts, double *ef /* m x 6 */);
+/* Fce, Xce: E x 3 (rows of elements not on the force surface are zero) */
+int dep_post_mst(dep_sys *s, double eps_medium, double *Fce, double *Xce, double *Fcm,
+                 double *Xcm);
+int dep_post_multipole(dep_sys *s, int order, double radius, int m, const double *pts,
+                       double *F /* m x 3 */);
+
+/* ---- reference-signature shims (src/depSolver.c:284-344) ----------------- */
+int electricFormA_tria3(double **mNodes, unsigned int **mElems, unsigned int *vBCType,
+                        unsigned int **vInterfaces, double **vMatParam, double *vProbParam,
+                        double **mA, double **mBC, double *vB);
+int electricFormA_tria6(double **mNodes, unsigned int **mElems, unsigned int *vBCType,
+                        unsigned int **vInterfaces, double **vMatParam, double *vProbParam,
+                        double **mA, double **mBC, double *vB);
+int electricFormFullA_tria3(double **mNodes, unsigned int **mElems, unsigned int *vBCType,
+                            unsigned int **vInterfaces, double **vMatParam, double *vProbParam,
+                            double **mA, double **mBC, double *vB);
+int electricFormFullA_tria6(double **mNodes, unsigned int **mElems, unsigned int *vBCType,
+                            unsigned int **vInterfaces, double **vMatParam, double *vProbParam,
+                            double **mA, double **mBC, double *vB);
+int gaussBksb(unsigned int N, double **A, double *B);
+void solverGMRES_el(unsigned int preCond, unsigned int nInit, double **A, double *B);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,102 @@
+"""Write the golden vectors under tests/golden/ from the reference build.
+
+Runs the reference's own functions (oracle/_ref/libdepref.so, compiled from
+/root/reference/src by oracle/Makefile) on seeded synthetic inputs and stores
+inputs and outputs as .npz fixtures.  Also copies the shipped example outputs
+that the tests compare against (examples/dep-trap-2).
+
+    python tests/golden/make_golden.py
+"""
+import os
+import shutil
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+import oracle  # noqa: E402
+from depsolver_amd import mesh  # noqa: E402
+
+EX = "/root/reference/examples"
+
+
+def tria3_problem():
+    top = mesh.plate(4, 10.0, (0, 0, 3), quadratic=False, normal_up=False)
+    bot = mesh.plate(4, 10.0, (0, 0, -3), quadratic=False)
+    sph = mesh.icosphere(2, 1.0, quadratic=False)
+    xyz, conn, bct, bc, body = mesh.build_problem([(top, 1.0), (bot, -1.0)], [(sph, 6, 1)], None, 1e6)
+    return mesh.Problem(xyz, conn, bct, bc, np.array([mesh.MEDIUM, mesh.PARTICLE]),
+                        np.array([[1, 2]], dtype=np.uint32), 1e6, body, "tiny-tria3")
+
+
+def problem_arrays(p):
+    return dict(xyz=p.xyz, conn=p.conn.astype(np.int32), bctype=p.bctype, bc=p.bc, mats=p.mats,
+                ifaces=p.ifaces, freq=np.float64(p.freq))
+
+
+def solve_case(r, p, name, npts=24, seed=1):
+    A, b = r.assemble(p, True)
+    sigma = r.gauss_bksb(A, b)
+    rng = np.random.default_rng(seed)
+    pts = np.concatenate([rng.uniform([-2, -2, -2], [2, 2, 2], (npts, 3)),
+                          p.xyz[rng.choice(p.n_nodes, 6, replace=False)]])
+    pot = np.array([r.potential(p, sigma, x) for x in pts])
+    fld = np.array([r.field(p, sigma, x) for x in pts])
+    Fce, Xce, Fcm, Xcm = r.force_mst(p, sigma)
+    xf = np.array([[0.2, 0.1, 0.3], [-0.4, 0.3, 0.1]])
+    mp = np.array([[r.force_multipole(p, sigma, x, 1.0, o) for o in (1, 2, 3)] for x in xf])
+    np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **problem_arrays(p),
+                        row_sum=A.sum(1), row_abs=np.abs(A).sum(1), A_diag=np.diag(A).copy(),
+                        A_row0=A[0].copy(), A_rowN=A[p.n_nodes].copy(), b=b, sigma=sigma, pts=pts,
+                        pot=pot, fld=fld, Fce=Fce, Xce=Xce, Fcm=Fcm, Xcm=Xcm, xf=xf, multipole=mp)
+
+
+def kernel_case(r):
+    """Per-pair quadrature vectors: regular, singular at every position, both flags."""
+    rng = np.random.default_rng(7)
+    out = {}
+    for npe, p in ((6, mesh.config1()), (3, tria3_problem())):
+        els = rng.choice(p.n_elems, 12, replace=False)
+        X = p.xyz[p.conn[els]]
+        xe = rng.uniform(-2, 2, (12, 3))
+        nrm = rng.normal(size=(12, 3))
+        nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+        out[f"X{npe}"], out[f"xe{npe}"], out[f"n{npe}"] = X, xe, nrm
+        out[f"G{npe}"] = np.array([r.intG(X[i], xe[i]) for i in range(12)])
+        out[f"H0_{npe}"] = np.array([r.intH(X[i], xe[i], nrm[i], 0) for i in range(12)])
+        out[f"H1_{npe}"] = np.array([r.intH(X[i], xe[i], nrm[i], 1) for i in range(12)])
+        out[f"SG{npe}"] = np.array([[r.intSingularG(X[i], X[i][s - 1], s) for s in range(1, npe + 1)]
+                                    for i in range(12)])
+        out[f"SH0_{npe}"] = np.array([[r.intSingularH(X[i], X[i][s - 1], nrm[i], s, 0)
+                                       for s in range(1, npe + 1)] for i in range(12)])
+        out[f"SH1_{npe}"] = np.array([[r.intSingularH(X[i], X[i][s - 1], nrm[i], s, 1)
+                                       for s in range(1, npe + 1)] for i in range(12)])
+        out[f"normal_nodes{npe}"] = np.arange(0, p.n_nodes, max(1, p.n_nodes // 40))
+        out[f"normal{npe}"] = np.array([r.node_normal(p, i) for i in out[f"normal_nodes{npe}"]])
+    np.savez_compressed(os.path.join(HERE, "kernels.npz"), **out)
+
+
+def copy_examples():
+    d = os.path.join(HERE, "dep-trap-2")
+    os.makedirs(d, exist_ok=True)
+    for f in ("input.bem", "nodes.bem", "elems.bem", "bcs.bem", "force-mst.dat", "solution.dat",
+              "gmres.log"):
+        shutil.copy(os.path.join(EX, "dep-trap-2", f), os.path.join(d, f))
+        os.chmod(os.path.join(d, f), 0o644)
+
+
+def main():
+    oracle.build()
+    r = oracle.reference()
+    kernel_case(r)
+    solve_case(r, mesh.sphere_between_plates(1, 4, name="tiny-tria6"), "tiny_tria6")
+    solve_case(r, tria3_problem(), "tiny_tria3")
+    solve_case(r, mesh.config1(), "config1", npts=12)
+    copy_examples()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,46 @@
+"""The oracle is bitwise equal to the reference build (oracle/_ref) on the
+same inputs, function by function (skipped where the reference sources are
+absent)."""
+import numpy as np
+import pytest
+
+from depsolver_amd import mesh
+
+
+def tria3():
+    top = mesh.plate(3, 10.0, (0, 0, 3), quadratic=False, normal_up=False)
+    bot = mesh.plate(3, 10.0, (0, 0, -3), quadratic=False)
+    sph = mesh.icosphere(1, 1.0, quadratic=False)
+    xyz, conn, bct, bc, body = mesh.build_problem([(top, 1.0), (bot, -1.0)], [(sph, 6, 1)], None, 1e6)
+    return mesh.Problem(xyz, conn, bct, bc, np.array([mesh.MEDIUM, mesh.PARTICLE]),
+                        np.array([[1, 2]], dtype=np.uint32), 1e6, body)
+
+
+CASES = {"tria6": lambda: mesh.sphere_between_plates(1, 3), "tria3": tria3,
+         "shelled": lambda: mesh.shelled_trap([(1, 0.8, (0.5, -0.3, 2.0))], 3e5, n_grid=3)}
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_bitwise(orc, ref, case, tmp_path, monkeypatch):
+    p = CASES[case]()
+    for full in (True, False):
+        a, b = orc.assemble(p, full)
+        c, d = ref.assemble(p, full)
+        assert np.array_equal(a, c) and np.array_equal(b, d)
+    A, b = orc.assemble(p, True)
+    x = orc.gauss_bksb(A, b)
+    assert np.array_equal(x, ref.gauss_bksb(A, b))
+    Ael, _ = orc.assemble(p, False)
+    monkeypatch.chdir(tmp_path)
+    xr = ref.gmres(Ael, b, 1)
+    rc, xo, it = orc.gmres_el(Ael, b, 1, 1e-6, 600, str(tmp_path / "o.log"))
+    assert rc == 0 and np.array_equal(xr, xo)
+    assert open(tmp_path / "gmres.log").read() == open(tmp_path / "o.log").read()
+    rng = np.random.default_rng(3)
+    for pt in list(rng.uniform(-2, 2, (4, 3))) + [p.xyz[i] for i in rng.choice(p.n_nodes, 4)]:
+        assert np.array_equal(orc.potential(p, x, pt), ref.potential(p, x, pt))
+        assert np.array_equal(orc.field(p, x, pt), ref.field(p, x, pt))
+    for a_, b_ in zip(orc.force_mst(p, x), ref.force_mst(p, x)):
+        assert np.array_equal(a_, b_)
+    for node in range(0, p.n_nodes, 23):
+        assert np.array_equal(orc.node_normal(p, node), ref.node_normal(p, node))
