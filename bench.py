@@ -1,0 +1,218 @@
+"""Benchmark: solves/sec of the full BEM hot path at N = 15,044 nodes.
+
+Workload (one step = one solve of a new request, inputs resident in HBM):
+  dielectric sphere (icosphere level 4, 5,120 tria6) between two 10x10 plates
+  (24x24 quadratic grids) at z = +-3, +-1 V; medium eps_r 78 / 1e-3 S/m,
+  particle eps_r 2.5 / 1e-4 S/m, f = 1 MHz; N = 15,044 nodes, E = 7,424.
+  step = FP64 collocation assembly -> factor (refined mixed precision, or
+  FP64 LU) -> FP64 solve/refinement to 1e-12 -> Maxwell-stress force.
+
+Prints ONE JSON line.  `value` is refined-mixed solves/sec (FP64-LU solves/sec
+when the mixed factor is unavailable); the FP64 path, p50/p99 latencies,
+errors against the FP64 solve, the dominant kernel's roofline (timed with HIP
+events on the library's stream over the timed region) and a CPU baseline (the
+oracle's restatement of the reference on a bounded sample, extrapolated to
+one full solve) are in the same line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAKS = {  # MI355X published dense peaks
+    "fp64": (78.6, "TFLOP/s", "mfma"), "fp32": (157.3, "TFLOP/s", "mfma"),
+    "fp16": (2500.0, "TFLOP/s", "mfma"), "bf16": (2500.0, "TFLOP/s", "mfma"),
+    "hbm": (8000.0, "GB/s", "hbm"),
+}
+KERNEL_PEAK = {"lu_gemm_fp64": "fp64", "lu_gemm_mixed": "fp16", "assemble_regular": "fp64",
+               "points_field": "fp64", "points_potential": "fp64", "zmatvec": "hbm"}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def kernel_stats(dep, ctx, names):
+    import ctypes
+    out = {}
+    for n in names:
+        la, ms, fl, by = ctypes.c_long(), ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        dep.lib().dep_ctx_kernel_stats(ctx._h, n.encode(), ctypes.byref(la), ctypes.byref(ms),
+                                       ctypes.byref(fl), ctypes.byref(by))
+        if la.value:
+            out[n] = {"launches": la.value, "ms": ms.value, "flops": fl.value, "bytes": by.value}
+    return out
+
+
+def roofline(stats):
+    if not stats:
+        return None
+    name = max(stats, key=lambda k: stats[k]["ms"])
+    st = stats[name]
+    kind = KERNEL_PEAK[name]
+    peak, unit, bound = PEAKS[kind]
+    per_launch_s = st["ms"] / st["launches"] / 1e3
+    if bound == "hbm":
+        achieved = st["bytes"] / st["launches"] / per_launch_s / 1e9
+    else:
+        achieved = st["flops"] / st["launches"] / per_launch_s / 1e12
+    return {"kernel": name, "bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
+            "frac": achieved / peak, "traffic": None, "launches": st["launches"],
+            "avg_launch_ms": st["ms"] / st["launches"],
+            "algorithmic_per_launch": (st["bytes"] if bound == "hbm" else st["flops"]) / st["launches"]}
+
+
+def cpu_baseline(p, s, rows=48, pivots=6, points=24):
+    """Time the oracle (single core) on a bounded sample of the same solve and
+    extrapolate to one full solve: assembly rows, the first LU pivots of the
+    reference's 2N x 2N elimination, and MST field points."""
+    import oracle
+    orc = oracle.oracle()
+    N = p.n_nodes
+    n2 = 2 * N
+    # assembly: `rows` rows of every kind, spread over the mesh
+    idx = np.linspace(0, N - rows, 4).astype(int)
+    t_asm = 0.0
+    for r0 in idx:
+        sub = rows // 4
+        t = time.perf_counter()
+        orc.assemble(p, False, rows=(int(r0), int(r0) + sub))
+        t_asm += time.perf_counter() - t
+    t_asm_full = t_asm / rows * N
+    # LU: the reference's elimination on the real 2N x 2N matrix (values from the device)
+    A = s.matrix(0)
+    b = p.rhs()
+    t = time.perf_counter()
+    orc.gauss_bksb(A, b, max_pivots=pivots, overwrite=True)
+    t_piv = time.perf_counter() - t
+    del A
+    i = np.arange(n2, dtype=np.float64)
+    work = (n2 - i) ** 2
+    t_lu_full = t_piv * work.sum() / work[:pivots].sum()
+    # MST: field at `points` force-surface nodes, then scaled to all of them
+    sigma = s.solution()
+    nodes = np.where(p.bctype[:N] == 6)[0]
+    pick = nodes[np.linspace(0, len(nodes) - 1, points).astype(int)]
+    t = time.perf_counter()
+    for k in pick:
+        orc.field(p, sigma, p.xyz[k])
+    t_fld = (time.perf_counter() - t) / points * len(nodes)
+    total = t_asm_full + t_lu_full + t_fld
+    return {"value": 1.0 / total, "unit": "solves/sec", "cores": 1, "kind": "port",
+            "sample": (f"oracle (gcc -O3, 1 thread) on this workload: {rows} assembly rows "
+                       f"(x{N / rows:.0f}), {pivots} of {n2} elimination pivots of the 2N x 2N real "
+                       f"system (scaled by sum (2N-i)^2), {points} of {len(nodes)} MST field points; "
+                       f"extrapolated s/solve: assembly {t_asm_full:.1f}, LU {t_lu_full:.1f}, "
+                       f"MST field {t_fld:.1f}"),
+            "seconds_per_solve": total}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--fp64-steps", type=int, default=2)
+    ap.add_argument("--precision", default="fp16", choices=["fp16", "bf16", "fp32", "fp64"])
+    ap.add_argument("--config", type=int, default=2, choices=[1, 2])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(batched solves would shard whole systems across GPUs)", file=sys.stderr)
+        sys.exit(2)
+
+    import depsolver_amd as dep
+    from depsolver_amd import mesh
+
+    p = mesh.config2() if args.config == 2 else mesh.config1()
+    N = p.n_nodes
+    ctx = dep.Context(0)
+    g = dep.Geometry(ctx, p)
+    s = dep.System(g)
+    log(f"bench: {p.name} N={N} E={p.n_elems}")
+
+    def step(prec):
+        t0 = time.perf_counter()
+        dep.lib().dep_geom_reassemble(g._h)
+        s.set_frequency(p.freq)
+        s.factor(prec)
+        sigma, info = s.solve(tol=1e-12, max_steps=30)
+        Fce, Xce, Fcm, Xcm = s.mst()
+        return time.perf_counter() - t0, sigma, info, Fce, Fcm
+
+    # FP64 reference solve (also the accuracy yardstick for the mixed path)
+    lat64 = []
+    for k in range(max(1, args.fp64_steps)):
+        dt, sig64, info64, Fce64, Fcm64 = step("fp64")
+        lat64.append(dt)
+        log(f"bench: fp64 step {k} {dt * 1e3:.1f} ms residual {info64['residual']:.2e}")
+    prec = args.precision
+    try:
+        step(prec)
+    except dep.DepError as e:
+        log(f"bench: precision {prec} unavailable ({e}); timing fp64")
+        prec = "fp64"
+    for _ in range(args.warmup):
+        step(prec)
+    ctx.synchronize()
+    dep.lib().dep_ctx_set_profiling(ctx._h, 1)
+    dep.lib().dep_ctx_reset_stats(ctx._h)
+    lat = []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        dt, sig, info, Fce, Fcm = step(prec)
+        lat.append(dt)
+        log(f"bench: {prec} step {k} {dt * 1e3:.1f} ms residual {info['residual']:.2e} "
+            f"iters {info['iterations']} fallback {info['fallback']}")
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    stats = kernel_stats(dep, ctx, list(KERNEL_PEAK))
+    dep.lib().dep_ctx_set_profiling(ctx._h, 0)
+    ms = elapsed / args.steps * 1e3
+    fscale = np.abs(Fce64).max()
+    errors = {
+        "sigma_rel": float(np.abs(sig - sig64).max() / np.abs(sig64).max()),
+        "force_element_rel": float(np.abs(Fce - Fce64).max() / fscale),
+        "force_total_rel": float(np.abs(Fcm - Fcm64).max() / np.abs(Fce64).sum()),
+    }
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("bench: cpu baseline (bounded sample)")
+        cpu = cpu_baseline(p, s)
+    lat_ms = np.array(lat) * 1e3
+    out = {
+        "metric": "solves/sec at N~12k BEM nodes, refined-FP32 vs FP64; p50/p99 solve latency",
+        "value": 1e3 / ms, "unit": "solves/sec", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": {"fp16": "fp16/fp32+f64", "bf16": "bf16/fp32+f64",
+                                       "fp32": "fp32+f64", "fp64": "f64"}[prec],
+        "data": "synthetic (seeded mesh generator, config geometry)",
+        "config": {"workload": f"{p.name}: sphere L4 + 24x24 plates, N={N}, E={p.n_elems}, "
+                               "assemble+factor+refine(1e-12)+MST per step",
+                   "precision": prec, "nodes": N, "elements": p.n_elems},
+        "p50_ms": float(np.percentile(lat_ms, 50)), "p99_ms": float(np.percentile(lat_ms, 99)),
+        "fp64_solves_per_sec": 1.0 / float(np.median(lat64)),
+        "fp64_p50_ms": float(np.median(lat64) * 1e3),
+        "speedup_vs_fp64": float(np.median(lat64) / np.median(lat)),
+        "errors_vs_fp64": errors,
+        "kernels": {k: {"launches": v["launches"], "avg_ms": v["ms"] / v["launches"]} for k, v in stats.items()},
+        "roofline": roofline(stats),
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["speedup_vs_cpu"] = out["value"] / cpu["value"]
+    print(json.dumps(out), flush=True)
+    s.close()
+    g.close()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

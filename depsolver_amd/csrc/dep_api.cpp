@@ -99,6 +99,8 @@ extern "C" int dep_ctx_destroy(dep_ctx* c)
     if (!c) return DEP_OK;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
+    prof_collect(c);
+    for (hipEvent_t e : c->pool) (void)hipEventDestroy(e);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -116,6 +118,85 @@ extern "C" int dep_ctx_synchronize(dep_ctx* c)
 }
 
 extern "C" size_t dep_ctx_device_bytes(const dep_ctx* c) { return c ? c->bytes : 0; }
+
+static hipEvent_t pool_get(dep_ctx* c)
+{
+    if (!c->pool.empty()) {
+        hipEvent_t e = c->pool.back();
+        c->pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+void prof_begin(dep_ctx* c, hipEvent_t* a)
+{
+    *a = nullptr;
+    if (!c->prof) return;
+    *a = pool_get(c);
+    if (*a) (void)hipEventRecord(*a, c->stream);
+}
+
+void prof_end(dep_ctx* c, hipEvent_t a, const char* name, double flops, double bytes)
+{
+    if (!c->prof || !a) return;
+    hipEvent_t b = pool_get(c);
+    if (!b) return;
+    (void)hipEventRecord(b, c->stream);
+    c->pending.push_back({name, a, b, flops, bytes});
+}
+
+void prof_collect(dep_ctx* c)
+{
+    if (c->pending.empty()) return;
+    (void)hipStreamSynchronize(c->stream);
+    for (auto& p : c->pending) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            dep_kstat& s = c->stats[p.name];
+            s.launches++;
+            s.ms += ms;
+            s.flops += p.flops;
+            s.bytes += p.bytes;
+        }
+        c->pool.push_back(p.a);
+        c->pool.push_back(p.b);
+    }
+    c->pending.clear();
+}
+
+extern "C" int dep_ctx_set_profiling(dep_ctx* c, int enable)
+{
+    if (!c) return DEP_ERR_ARG;
+    prof_collect(c);
+    c->prof = enable != 0;
+    return DEP_OK;
+}
+
+extern "C" int dep_ctx_reset_stats(dep_ctx* c)
+{
+    if (!c) return DEP_ERR_ARG;
+    prof_collect(c);
+    c->stats.clear();
+    return DEP_OK;
+}
+
+extern "C" int dep_ctx_kernel_stats(dep_ctx* c, const char* name, long* launches, double* total_ms,
+                                    double* flops, double* bytes)
+{
+    if (!c || !name) return DEP_ERR_ARG;
+    prof_collect(c);
+    auto it = c->stats.find(name);
+    const dep_kstat z;
+    const dep_kstat& s = it == c->stats.end() ? z : it->second;
+    if (launches) *launches = s.launches;
+    if (total_ms) *total_ms = s.ms;
+    if (flops) *flops = s.flops;
+    if (bytes) *bytes = s.bytes;
+    return DEP_OK;
+}
 
 // ---------------------------------------------------------------------------
 // geometry
@@ -292,6 +373,13 @@ extern "C" int dep_geom_destroy(dep_geom* g)
     free_all(g);
     delete g;
     return DEP_OK;
+}
+
+extern "C" int dep_geom_reassemble(dep_geom* g)
+{
+    if (!g) return DEP_ERR_ARG;
+    g->assembled = false;
+    return dep_geom_assemble(g);
 }
 
 extern "C" int dep_geom_assemble(dep_geom* g)

@@ -217,9 +217,18 @@ hipError_t assemble_impl(dep_geom* g)
     hipStream_t st = g->ctx->stream;
     constexpr int TR = 128;
     dim3 grid(g->n_patch, dep_cdiv(g->N, TR));
+    hipEvent_t ev;
+    prof_begin(g->ctx, &ev);
     hipLaunchKernelGGL((k_assemble_regular<NPE, TR>), grid, dim3(TR), 0, st, g->N, g->xyz, g->conn,
                        g->kind, g->normal, (const double4*)g->q, g->patch_ptr, g->patch_elem,
                        g->patch_slot, g->P, g->ldP);
+    // algorithmic flops per (row, element) pair of the 7-point rule, sqrt and
+    // division counted as one flop each: 7 (10 + 2 npe) for G rows, 7 (18 + 2 npe)
+    // for dG/dn rows; bytes: the N x N FP64 blocks written once
+    long nI = 0;
+    for (int8_t k : g->hkind) nI += (k != 0);
+    const double flops = (double)g->E * (7.0 * (10 + 2 * NPE) * (g->N - nI) + 7.0 * (18 + 2 * NPE) * nI);
+    prof_end(g->ctx, ev, "assemble_regular", flops, 8.0 * g->N * (double)g->N);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const int np = NPE * g->E;

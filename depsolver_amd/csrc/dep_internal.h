@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <map>
 #include <string>
 #include <utility>
 #include <vector>
@@ -23,13 +24,33 @@
 
 #define DEP_TC 64   // columns per assembly patch
 
+struct dep_kstat {
+    long launches = 0;
+    double ms = 0.0, flops = 0.0, bytes = 0.0;
+};
+
 struct dep_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     std::string err;
     size_t bytes = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // per-kernel timing with events on `stream` (off unless enabled)
+    bool prof = false;
+    std::map<std::string, dep_kstat> stats;
+    struct Pending {
+        std::string name;
+        hipEvent_t a, b;
+        double flops, bytes;
+    };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> pool;
 };
+
+// time one launch: prof_begin before, prof_end after (no-ops unless profiling)
+void prof_begin(dep_ctx* c, hipEvent_t* a);
+void prof_end(dep_ctx* c, hipEvent_t a, const char* name, double flops, double bytes);
+void prof_collect(dep_ctx* c);   // after a synchronize: fold finished events into stats
 
 struct dep_geom {
     dep_ctx* ctx = nullptr;

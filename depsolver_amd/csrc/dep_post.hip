@@ -307,6 +307,8 @@ hipError_t launch_points(dep_sys* s, int mode, const dep_points* P, double* out)
     const int NO = mode ? 6 : 2;
     const int nchunk = P->nchunk, chunk = dep_cdiv(g->E, nchunk);
     dim3 grid(dep_cdiv(M, 256), nchunk);
+    hipEvent_t ev;
+    prof_begin(g->ctx, &ev);
     if (mode == 0)
         hipLaunchKernelGGL(k_points_regular<0>, grid, dim3(256), 0, st, M, P->pts, g->E, chunk,
                            (const double4*)g->q, (const double2*)s->qs, P->excl_ptr, P->excl_e,
@@ -315,6 +317,9 @@ hipError_t launch_points(dep_sys* s, int mode, const dep_points* P, double* out)
         hipLaunchKernelGGL(k_points_regular<1>, grid, dim3(256), 0, st, M, P->pts, g->E, chunk,
                            (const double4*)g->q, (const double2*)s->qs, P->excl_ptr, P->excl_e,
                            P->partial);
+    // per (point, element): 7 points x (12 + 14) flops for the field, 7 x (10 + 4) for the potential
+    prof_end(g->ctx, ev, mode ? "points_field" : "points_potential",
+             (double)M * g->E * 7.0 * (mode ? 26.0 : 14.0), 48.0 * g->E * 7.0 + 56.0 * M);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (P->npairs > 0) {

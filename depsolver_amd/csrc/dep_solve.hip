@@ -385,8 +385,12 @@ hipError_t launch_lu_fp64(dep_sys* s, double* min_pivot)
             hipLaunchKernelGGL(k_apply_linv, dim3(dep_cdiv(N - r0, 16)), dim3(256), 0, st, N, nb, r0,
                                s->Zr, s->Zi, s->ldZ, k0, invb);
             dim3 grid(dep_cdiv(N - r0, 64), dep_cdiv(N - r0, 64));
+            hipEvent_t ev;
+            prof_begin(g->ctx, &ev);
             hipLaunchKernelGGL(k_gemm_update, grid, dim3(256), 0, st, N, nb, k0, r0, s->Zr, s->Zi,
                                s->ldZ);
+            const double m = N - r0;
+            prof_end(g->ctx, ev, "lu_gemm_fp64", 8.0 * m * m * nb, 32.0 * m * m + 32.0 * m * nb);
         }
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
@@ -439,25 +443,23 @@ hipError_t launch_trsv_fp64(dep_sys* s, const double* rhs, double* out)
 // r = b - Z x (b may be null: r = Z x); x, r split layout (re block, im block)
 hipError_t launch_residual(dep_sys* s, const double* x, double* r)
 {
-    dep_geom* g = s->g;
-    hipLaunchKernelGGL(k_zmatvec, dim3(dep_cdiv(g->N, 4)), dim3(256), 0, g->ctx->stream, g->N, g->P,
-                       g->ldP, s->wr, s->wi, s->shift, x, s->b, r);
-    return hipGetLastError();
+    return launch_zmatvec_w(s, s->wr, s->wi, s->shift, x, s->b, r);
 }
 
 hipError_t launch_zmatvec_w(dep_sys* s, const double* wr, const double* wi, const double* shift,
                             const double* x, const double* b, double* y)
 {
     dep_geom* g = s->g;
+    hipEvent_t ev;
+    prof_begin(g->ctx, &ev);
     hipLaunchKernelGGL(k_zmatvec, dim3(dep_cdiv(g->N, 4)), dim3(256), 0, g->ctx->stream, g->N, g->P,
                        g->ldP, wr, wi, shift, x, b, y);
+    const double n = g->N;
+    prof_end(g->ctx, ev, "zmatvec", 4.0 * n * n, 8.0 * n * n + 48.0 * n);
     return hipGetLastError();
 }
 
 hipError_t launch_matvec(dep_sys* s, const double* x, double* y)
 {
-    dep_geom* g = s->g;
-    hipLaunchKernelGGL(k_zmatvec, dim3(dep_cdiv(g->N, 4)), dim3(256), 0, g->ctx->stream, g->N, g->P,
-                       g->ldP, s->wr, s->wi, s->shift, x, (const double*)nullptr, y);
-    return hipGetLastError();
+    return launch_zmatvec_w(s, s->wr, s->wi, s->shift, x, nullptr, y);
 }

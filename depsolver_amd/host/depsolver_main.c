@@ -1,0 +1,449 @@
+/*
+ * depsolver_main.c -- command-line driver with depSolver's flow and files.
+ *
+ * Follows src/depSolver.c:52-427 (and the REPOSITION section of
+ * src/depSolver-shift.c:129-133,212-217): reads ./input.bem and the files it
+ * names, assembles, solves, post-processes and writes solution.dat,
+ * potential.dat, field.dat, force-mst.dat, force-mp.dat, gmres.log and
+ * bem.log in the reference's formats.  All numerical work runs on the GPU
+ * through libdepsolver_amd.so.
+ *
+ * Differences from the reference, by design:
+ *   - `//` comments are stripped in memory (no <file>.tmp files, no
+ *     strcat into a fixed buffer: src/preProcess.c:55);
+ *   - the section keywords are read by position, as the reference does;
+ *   - DEPSOLVER_PRECISION=fp64|fp16|bf16|fp32 selects the LU factor
+ *     precision for the gaussBksb solver (default fp64, the reference's
+ *     arithmetic); mixed factors are refined in FP64 to DEPSOLVER_TOL
+ *     (default 1e-12) relative residual;
+ *   - DEPSOLVER_GMRES_TOL overrides the reference's GMRES tolerance (1e-6).
+ *
+ * Usage: depSolver-amd [directory]   (default: the current directory)
+ */
+#include <ctype.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#include <unistd.h>
+
+#include "../../include/depsolver_amd.h"
+
+#define EPS0 8.854187817E-12
+
+static FILE *flog;
+
+static void die(const char *msg, const char *detail)
+{
+    fprintf(stderr, "\nERROR: %s%s%s\n*** PROGRAM TERMINATED ***\n\n", msg, detail ? ": " : "",
+            detail ? detail : "");
+    if (flog) {
+        fprintf(flog, "\nERROR: %s%s%s\n*** PROGRAM TERMINATED ***\n\n", msg, detail ? ": " : "",
+                detail ? detail : "");
+        fclose(flog);
+    }
+    exit(1);
+}
+
+static void check(dep_ctx *ctx, int rc, const char *what)
+{
+    if (rc != DEP_OK) die(what, dep_last_error(ctx));
+}
+
+/* whole file with `//` comments removed (src/preProcess.c:48-81 semantics) */
+static char *read_filtered(const char *path)
+{
+    FILE *f = fopen(path, "r");
+    if (!f) die("Can't open input file", path);
+    size_t cap = 1 << 16, len = 0;
+    char *buf = malloc(cap), line[4096];
+    while (fgets(line, sizeof line, f)) {
+        char *c = strstr(line, "//");
+        if (c) {
+            if (c == line) *c = 0;
+            else { c[0] = '\n'; c[1] = 0; }
+        }
+        size_t n = strlen(line);
+        if (len + n + 1 > cap) {
+            cap = 2 * (cap + n);
+            buf = realloc(buf, cap);
+        }
+        memcpy(buf + len, line, n);
+        len += n;
+    }
+    buf[len] = 0;
+    fclose(f);
+    return buf;
+}
+
+/* whitespace tokenizer over a filtered buffer */
+typedef struct { char *p; } toks;
+static char *tok(toks *t)
+{
+    while (*t->p && isspace((unsigned char)*t->p)) t->p++;
+    if (!*t->p) return NULL;
+    char *s = t->p;
+    while (*t->p && !isspace((unsigned char)*t->p)) t->p++;
+    if (*t->p) *t->p++ = 0;
+    return s;
+}
+static char *need(toks *t, const char *what)
+{
+    char *s = tok(t);
+    if (!s) die("input.bem ends before", what);
+    return s;
+}
+
+static double now(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+static void log_time(const char *label, double t)
+{
+    if (t > 60) fprintf(flog, "\n%s%u m %2.1f s", label, (unsigned)(t / 60.0), t - (int)(t / 60.0) * 60.0);
+    else fprintf(flog, "\n%s%2.1f seconds", label, t);
+}
+
+int main(int argc, char **argv)
+{
+    if (argc > 1 && chdir(argv[1]) != 0) die("cannot change to directory", argv[1]);
+    flog = fopen("bem.log", "a");
+    time_t tnow = time(NULL);
+    fprintf(flog, "********************************************************************************\n");
+    fprintf(flog, "depSolver (MI355X hot path) -- input/output compatible with depSolver 2.0.\n\n");
+    fprintf(flog, "********************************************************************************\n\n");
+    fprintf(flog, "depSolver(): started @ %s", asctime(localtime(&tnow)));
+    fflush(flog);
+    const double t0 = now();
+
+    /* ---- input.bem (src/depSolver.c:98-177) ---- */
+    char *inbuf = read_filtered("input.bem");
+    toks t = {inbuf};
+    need(&t, "NODES");
+    const int N = atoi(need(&t, "node count"));
+    char nodefile[256], elemfile[256], bcfile[256], ptsfile[256] = "", fpfile[256] = "";
+    snprintf(nodefile, sizeof nodefile, "%s", need(&t, "node file"));
+    need(&t, "ELEMS");
+    const int E = atoi(need(&t, "element count"));
+    char etype[16];
+    snprintf(etype, sizeof etype, "%s", need(&t, "element type"));
+    for (char *c = etype; *c; c++) *c = (char)tolower((unsigned char)*c);
+    snprintf(elemfile, sizeof elemfile, "%s", need(&t, "element file"));
+    int npe = 0;
+    if (strcmp(etype, "tria3") == 0) npe = 3;
+    else if (strcmp(etype, "tria6") == 0) npe = 6;
+    else die("Error: Unknown element type option", etype);
+    need(&t, "MATERIALS");
+    const int nM = atoi(need(&t, "material count"));
+    double *mats = calloc(2 * (size_t)(nM > 0 ? nM : 1), sizeof(double));
+    for (int i = 0; i < nM; i++) {
+        need(&t, "material id");
+        mats[2 * i] = atof(need(&t, "sigma"));
+        mats[2 * i + 1] = atof(need(&t, "eps_r"));
+    }
+    need(&t, "INTERFACES");
+    const int nI = atoi(need(&t, "interface count"));
+    uint32_t *ifaces = calloc(2 * (size_t)(nI > 0 ? nI : 1), sizeof(uint32_t));
+    for (int i = 0; i < nI; i++) {
+        need(&t, "interface id");
+        ifaces[2 * i] = (uint32_t)atoi(need(&t, "material 1"));
+        ifaces[2 * i + 1] = (uint32_t)atoi(need(&t, "material 2"));
+    }
+    need(&t, "PROBLEMS");
+    const double freq = atof(need(&t, "frequency"));
+    snprintf(bcfile, sizeof bcfile, "%s", need(&t, "bc file"));
+    char *sec = need(&t, "ANALYSIS or REPOSITION");
+    int pNodes = -1;
+    double dr[3] = {0, 0, 0};
+    if (toupper((unsigned char)sec[0]) == 'R') {   /* depSolver-shift.c:129-133 */
+        pNodes = atoi(need(&t, "REPOSITION node"));
+        for (int k = 0; k < 3; k++) dr[k] = atof(need(&t, "displacement"));
+        need(&t, "ANALYSIS");
+    }
+    char solver[16];
+    snprintf(solver, sizeof solver, "%s", need(&t, "solver"));
+    for (char *c = solver; *c; c++) *c = (char)toupper((unsigned char)*c);
+    const int use_gmres = strcmp(solver, "GMRES") == 0;
+    int preCond = 0, nInit = 0;
+    if (use_gmres) {
+        preCond = atoi(need(&t, "preconditioner"));
+        nInit = atoi(need(&t, "initial guess"));
+    }
+    const int ANALYSIS = atoi(need(&t, "analysis type"));
+    int nF = 0;
+    double axis[3] = {0, 0, 0};
+    if (ANALYSIS > 4 && ANALYSIS < 10) {
+        nF = atoi(need(&t, "force points"));
+        axis[0] = atof(need(&t, "radius"));
+        snprintf(fpfile, sizeof fpfile, "%s", need(&t, "force point file"));
+    } else if (ANALYSIS == 10 || ANALYSIS == 11) {
+        die("ANALYSIS 10/11 (dipolar ellipsoid) is not available in this build", NULL);
+    }
+    need(&t, "INTERNALPOINTS");
+    int nP = atoi(need(&t, "internal point count"));
+    char otype[8] = "NAN";
+    if (nP > 0) {
+        snprintf(otype, sizeof otype, "%.3s", need(&t, "output type"));
+        for (char *c = otype; *c; c++) *c = (char)toupper((unsigned char)*c);
+        if (strcmp(otype, "STD") != 0 && strcmp(otype, "VTK") != 0)
+            die("Error: Unknown output type option.", otype);
+        if (strcmp(otype, "VTK") == 0) die("VTK output is not available in this build", NULL);
+        snprintf(ptsfile, sizeof ptsfile, "%s", need(&t, "internal point file"));
+    }
+    int nCols = 0, colType = 0;
+    double *cols = NULL;
+    char *cs = tok(&t);
+    if (cs) {
+        char *nc = tok(&t);
+        nCols = nc ? atoi(nc) : 0;
+        if (nCols > 0) {   /* read as nCols then colType (depSolver.c:171-173) */
+            colType = atoi(need(&t, "column type"));
+            cols = calloc(3 * (size_t)nCols, sizeof(double));
+            for (int i = 0; i < 3 * nCols; i++) cols[i] = atof(need(&t, "column"));
+        }
+    }
+
+    /* ---- mesh, bcs, points (src/depSolver.c:200-252) ---- */
+    double *xyz = malloc(sizeof(double) * 3 * (size_t)N);
+    int32_t *conn = malloc(sizeof(int32_t) * (size_t)E * npe);
+    uint32_t *bct = malloc(sizeof(uint32_t) * 2 * (size_t)N);
+    double *bc = calloc(6 * (size_t)N, sizeof(double));
+    {
+        char *b = read_filtered(nodefile);
+        toks s = {b};
+        for (int i = 0; i < N; i++) {
+            need(&s, "node id");
+            for (int k = 0; k < 3; k++) xyz[3 * i + k] = atof(need(&s, "coordinate"));
+        }
+        free(b);
+        b = read_filtered(elemfile);
+        s.p = b;
+        for (int e = 0; e < E; e++) {
+            need(&s, "element id");
+            for (int k = 0; k < npe; k++) conn[(size_t)e * npe + k] = atoi(need(&s, "node")) - 1;
+        }
+        free(b);
+        b = read_filtered(bcfile);
+        s.p = b;
+        for (int j = 0; j < 2 * N; j++) {
+            need(&s, "bc id");
+            char *ty = need(&s, "bc type");
+            const int v = (ty[0] == 'C' || ty[0] == 'c') ? atoi(ty + 1) + 5 : atoi(ty);
+            bct[j] = (uint32_t)v;
+            const int nv = (v > 2 && v < 6) ? 3 : (v == 0 || v == 6) ? 2 : 1;
+            for (int k = 0; k < nv; k++) bc[3 * j + k] = atof(need(&s, "bc value"));
+        }
+        free(b);
+    }
+    if (pNodes >= 0)   /* depSolver-shift.c:212-217 */
+        for (int i = pNodes; i < N; i++)
+            for (int k = 0; k < 3; k++) xyz[3 * i + k] += dr[k];
+    double *pts = NULL;
+    if (nP > 0) {
+        char *b = read_filtered(ptsfile);
+        toks s = {b};
+        double *tmp = malloc(sizeof(double) * 3 * (size_t)nP);
+        for (int i = 0; i < nP; i++) {
+            need(&s, "point id");
+            for (int k = 0; k < 3; k++) tmp[3 * i + k] = atof(need(&s, "coordinate"));
+        }
+        free(b);
+        /* column filter (src/postProcess.c:123-183) */
+        int m = 0;
+        pts = malloc(sizeof(double) * 3 * (size_t)nP);
+        for (int i = 0; i < nP; i++) {
+            int outside = 0;
+            for (int c = 0; c < nCols; c++) {
+                const double dx = tmp[3 * i] - cols[3 * c], dy = tmp[3 * i + 1] - cols[3 * c + 1];
+                const double rr = cols[3 * c + 2];
+                if (colType == 1) outside += (dx * dx + dy * dy) > rr * rr;
+                else outside += (dx > rr || -dx > rr) && (dy > rr || -dy > rr);
+            }
+            if (outside == nCols) {
+                memcpy(pts + 3 * m, tmp + 3 * i, sizeof(double) * 3);
+                m++;
+            }
+        }
+        free(tmp);
+        nP = m;
+    }
+    double *fpts = NULL;
+    if (nF > 0) {
+        char *b = read_filtered(fpfile);
+        toks s = {b};
+        fpts = malloc(sizeof(double) * 3 * (size_t)nF);
+        for (int i = 0; i < nF; i++) {
+            need(&s, "force point id");
+            for (int k = 0; k < 3; k++) fpts[3 * i + k] = atof(need(&s, "coordinate"));
+        }
+        free(b);
+    }
+    fprintf(flog, "\nUsing %u nodes and %u %s elements to solve AC electrostatic problem", N, E, etype);
+    fprintf(flog, "\nNumber of interfaces: %u\n", nI);
+    fprintf(flog, "Number of different materials: %u\n", nM);
+    fflush(flog);
+    const double t1 = now() - t0;
+
+    /* ---- assembly ---- */
+    dep_ctx *ctx = NULL;
+    if (dep_ctx_create(0, &ctx) != DEP_OK) die("no GPU available", NULL);
+    int32_t *iface = malloc(sizeof(int32_t) * (size_t)N);
+    double *rhs = malloc(sizeof(double) * 2 * (size_t)N);
+    for (int i = 0; i < N; i++) {
+        iface[i] = (bct[i] == 0 || bct[i] == 6) ? (int32_t)bc[3 * i + 1] - 1 : -1;
+        rhs[i] = bc[3 * i];
+        rhs[N + i] = bc[3 * (N + i)];
+    }
+    fprintf(flog, "\ndepSolver(): performing matrix assembly ...");
+    fflush(flog);
+    dep_geom *g = NULL;
+    dep_sys *s = NULL;
+    check(ctx, dep_geom_create(ctx, N, xyz, E, npe, conn, bct, iface, &g), "geometry");
+    check(ctx, dep_geom_assemble(g), "assembly");
+    check(ctx, dep_sys_create(g, nM, mats, nI, ifaces, freq, rhs, &s), "system");
+    check(ctx, dep_ctx_synchronize(ctx), "assembly");
+    const double t2 = now() - t0 - t1;
+
+    /* ---- solve ---- */
+    double *sigma = malloc(sizeof(double) * 2 * (size_t)N);
+    dep_solve_info info;
+    if (use_gmres) {
+        fprintf(flog, "\ndepSolver(): solving linear system with GMRES ...");
+        fflush(flog);
+        double *x0 = NULL;
+        if (nInit) {   /* flat list of the first nInit numbers (gmres_el.c:80-84) */
+            x0 = calloc(2 * (size_t)N, sizeof(double));
+            FILE *fi = fopen("solution.init", "r");
+            if (!fi) die("cannot open solution.init", NULL);
+            for (int i = 0; i < nInit && i < 2 * N; i++)
+                if (fscanf(fi, "%le", &x0[i]) != 1) break;
+            fclose(fi);
+        }
+        const char *gt = getenv("DEPSOLVER_GMRES_TOL");
+        const int rc = dep_gmres(s, preCond, gt ? atof(gt) : 1.0e-6, 600, x0, sigma, &info, "gmres.log");
+        if (rc == DEP_ERR_NOCONV) die("Linear system may be ill-conditioned", dep_last_error(ctx));
+        check(ctx, rc, "GMRES");
+        free(x0);
+    } else {
+        fprintf(flog, "\ndepSolver(): solving linear system with gaussBksb ...");
+        fflush(flog);
+        const char *pr = getenv("DEPSOLVER_PRECISION");
+        int prec = DEP_PREC_FP64;
+        if (pr && strcmp(pr, "fp16") == 0) prec = DEP_PREC_FP16;
+        else if (pr && strcmp(pr, "bf16") == 0) prec = DEP_PREC_BF16;
+        else if (pr && strcmp(pr, "fp32") == 0) prec = DEP_PREC_FP32;
+        const int rc = dep_factor(s, prec);
+        if (rc == DEP_ERR_SINGULAR) die("Singularity in system matrix", dep_last_error(ctx));
+        check(ctx, rc, "factorization");
+        const char *tt = getenv("DEPSOLVER_TOL");
+        check(ctx, dep_solve(s, tt ? atof(tt) : 1e-12, 30, sigma, &info), "solve");
+    }
+    const double t3 = now() - t0 - t1 - t2;
+
+    /* ---- post-processing (src/postProcess.c:372-463) ---- */
+    fprintf(flog, "\ndepSolver(): performing postProcessing tasks ...");
+    fflush(flog);
+    FILE *fp0 = NULL, *fp1 = NULL, *fp2 = NULL;
+    if (ANALYSIS == 0) {
+        fp0 = fopen("potential.dat", "w");
+        fprintf(fp0, "#x\t\ty\t\tz\t\tRe[Pot(x,y,z)]\tIm[Pot(x,y,z)]\n");
+    } else if (ANALYSIS == 1) {
+        fp1 = fopen("field.dat", "w");
+        fprintf(fp1, "#x\t\ty\t\tz\t\tRe[Ex(x,y,z)]\tIm[Ex(x,y,z)]\tRe[Ey(x,y,z)]");
+        fprintf(fp1, "\tIm[Ey(x,y,z)]\tRe[Ez(x,y,z)]\tIm[Ez(x,y,z)]\n");
+    } else {
+        fp0 = fopen("potential.dat", "w");
+        fp1 = fopen("field.dat", "w");
+        fprintf(fp0, "#x\t\ty\t\tz\t\tRe[Pot(x,y,z)]\tIm[Pot(x,y,z)]\n");
+        fprintf(fp1, "#x\t\ty\t\tz\t\tRe[Ex(x,y,z)] \tIm[Ex(x,y,z)]\tRe[Ey(x,y,z)]");
+        fprintf(fp1, "\tIm[Ey(x,y,z)] \tRe[Ez(x,y,z)]\tIm[Ez(x,y,z)]\n");
+    }
+    if (ANALYSIS == 3 || ANALYSIS == 4) {
+        fp2 = fopen("force-mst.dat", "w");
+        fprintf(fp2, "#Xcm\t\tYcm\t\tZcm\t\tFXcm\t\tFYcm\t\tFZcm\n");
+    } else if (ANALYSIS > 4) {
+        fp2 = fopen("force-mp.dat", "w");
+        fprintf(fp2, "#Xcm\t\tYcm\t\tZcm\t\tFXcm\t\tFYcm\t\tFZcm\n");
+    }
+    /* tria3 skips potential/field for ANALYSIS 4, tria6 does not (postProcess.c:298 vs :414) */
+    const int want_pf = nP > 0 && !(npe == 3 && ANALYSIS == 4);
+    if (want_pf) {
+        double *pot = NULL, *ef = NULL;
+        if (ANALYSIS != 1) {
+            pot = malloc(sizeof(double) * 2 * (size_t)nP);
+            check(ctx, dep_post_potential(s, nP, pts, pot), "potential");
+        }
+        if (ANALYSIS != 0) {
+            ef = malloc(sizeof(double) * 6 * (size_t)nP);
+            check(ctx, dep_post_field(s, nP, pts, ef), "field");
+        }
+        for (int i = 0; i < nP; i++) {
+            const double *x = pts + 3 * i;
+            if (pot) {
+                fprintf(fp0, "%e\t%e\t%e\t", x[0], x[1], x[2]);
+                fprintf(fp0, "%e\t%e\n", pot[2 * i], pot[2 * i + 1]);
+            }
+            if (ef) {
+                fprintf(fp1, "%e\t%e\t%e\t", x[0], x[1], x[2]);
+                fprintf(fp1, "%e\t%e\t%e\t", ef[6 * i], ef[6 * i + 1], ef[6 * i + 2]);
+                fprintf(fp1, "%e\t%e\t%e\n", ef[6 * i + 3], ef[6 * i + 4], ef[6 * i + 5]);
+            }
+        }
+        free(pot);
+        free(ef);
+    }
+    if (ANALYSIS == 3 || ANALYSIS == 4) {
+        double *Fce = malloc(sizeof(double) * 3 * (size_t)E), *Xce = malloc(sizeof(double) * 3 * (size_t)E);
+        double Fcm[3], Xcm[3];
+        check(ctx, dep_post_mst(s, mats[1] * EPS0, Fce, Xce, Fcm, Xcm), "MST force");
+        fprintf(fp2, "%e\t%e\t%e\t%e\t%e\t%e\n", Xcm[0], Xcm[1], Xcm[2], Fcm[0], Fcm[1], Fcm[2]);
+        fprintf(fp2, "#Xce\t\tYce\t\tZce\t\tFXce\t\tFYce\t\tFZce\n");
+        for (int e = 0; e < E; e++)
+            if (bct[conn[(size_t)e * npe]] == 6) {
+                fprintf(fp2, "%e\t%e\t%e\t", Xce[3 * e], Xce[3 * e + 1], Xce[3 * e + 2]);
+                fprintf(fp2, "%e\t%e\t%e\n", Fce[3 * e], Fce[3 * e + 1], Fce[3 * e + 2]);
+            }
+        free(Fce);
+        free(Xce);
+    } else if (ANALYSIS > 4) {
+        const int order = ANALYSIS - 4;
+        double *F = malloc(sizeof(double) * 3 * (size_t)nF);
+        check(ctx, dep_post_multipole(s, order, axis[0], nF, fpts, F), "multipolar force");
+        for (int i = 0; i < nF; i++) {
+            fprintf(fp2, "%e\t%e\t%e\t", fpts[3 * i], fpts[3 * i + 1], fpts[3 * i + 2]);
+            fprintf(fp2, "%e\t%e\t%e\n", F[3 * i], F[3 * i + 1], F[3 * i + 2]);
+        }
+        free(F);
+    }
+    if (fp0) fclose(fp0);
+    if (fp1) fclose(fp1);
+    if (fp2) fclose(fp2);
+
+    FILE *fs = fopen("solution.dat", "w");
+    for (int i = 0; i < N; i++) {
+        fprintf(fs, "%le\t%le\t%le\t", xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+        fprintf(fs, "%le\t%le\n", sigma[i], sigma[i + N]);
+    }
+    fclose(fs);
+    const double t4 = now() - t0 - t1 - t2 - t3;
+
+    fprintf(flog, "\n\n*** depSolver(): performance analysis ***");
+    log_time("TIME READING INPUT: \t\t", t1);
+    log_time("TIME IN ASSEMBLY: \t\t", t2);
+    log_time("TIME IN SOLVER: \t\t", t3);
+    log_time("TIME IN POST-PROCESSING: \t", t4);
+    log_time("TOTAL EXECUTION TIME: \t\t", t1 + t2 + t3 + t4);
+    tnow = time(NULL);
+    fprintf(flog, "\n\ndepSolver(): finished @ %s", asctime(localtime(&tnow)));
+    fprintf(flog, "\n\n********************************************************************************\n\n");
+    fclose(flog);
+    dep_sys_destroy(s);
+    dep_geom_destroy(g);
+    dep_ctx_destroy(ctx);
+    free(inbuf); free(mats); free(ifaces); free(xyz); free(conn); free(bct); free(bc);
+    free(iface); free(rhs); free(sigma); free(pts); free(fpts); free(cols);
+    return 0;
+}

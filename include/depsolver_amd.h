@@ -71,6 +71,14 @@ const char *dep_last_error(const dep_ctx *ctx);
 int dep_ctx_synchronize(dep_ctx *ctx);
 /* device memory in use by this context, bytes */
 size_t dep_ctx_device_bytes(const dep_ctx *ctx);
+/* Per-kernel timing with HIP events on the context's stream (for roofline
+ * reporting; off by default).  Names: "assemble_regular", "lu_gemm_fp64",
+ * "lu_gemm_mixed", "points_regular", "zmatvec".  flops/bytes are the
+ * algorithmic counts of the timed launches. */
+int dep_ctx_set_profiling(dep_ctx *ctx, int enable);
+int dep_ctx_reset_stats(dep_ctx *ctx);
+int dep_ctx_kernel_stats(dep_ctx *ctx, const char *kernel, long *launches, double *total_ms,
+                         double *flops, double *bytes);
 
 /* ---- geometry: replaces the geometric half of electricForm[Full]A_tria{3,6}
  *      (src/assembly_el.c:470-589, :329-448, :197-307, :63-174) ------------ */
@@ -84,6 +92,9 @@ int dep_geom_destroy(dep_geom *g);
  * kernelIntegrals.c (intG/intH regular, intSingularG/H singular) without the
  * material/frequency weights.  Idempotent. */
 int dep_geom_assemble(dep_geom *g);
+/* Recompute the blocks (after the caller changed nothing: used to time the
+ * assembly of a new request). */
+int dep_geom_reassemble(dep_geom *g);
 /* Copy the geometric matrix (N x N, row-major, reference node order). */
 int dep_geom_get_blocks(dep_geom *g, double *P);
 

@@ -131,8 +131,9 @@ class Oracle(_Lib):
             raise RuntimeError("orc_assemble failed")
         return mA, vB
 
-    def gauss_bksb(self, A, B, max_pivots=-1):
-        A = _c64(A).copy(); B = _c64(B).copy()
+    def gauss_bksb(self, A, B, max_pivots=-1, overwrite=False):
+        A = _c64(A) if overwrite else _c64(A).copy()
+        B = _c64(B).copy()
         rc = self.lib.orc_gauss_bksb(A.shape[0], A, B, max_pivots)
         if rc != 0:
             raise RuntimeError("singular system")
