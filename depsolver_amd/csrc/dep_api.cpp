@@ -488,6 +488,7 @@ extern "C" int dep_sys_create(dep_geom* g, int nM, const double* mats, int nI, c
     if (r == DEP_OK) r = dep_obj_alloc(s, &s->work, 8 * 2 * (size_t)N);
     if (r == DEP_OK) r = dep_obj_alloc(s, &s->dwork, 65536);
     if (r == DEP_OK) r = dep_obj_alloc(s, &s->qs, (size_t)g->E * 7 * 2);
+    if (r == DEP_OK) r = dep_obj_alloc(s, &s->tbuf, 2 * (size_t)N);
     if (r == DEP_OK) r = set_weights(s, freq);
     if (r == DEP_OK) {
         std::vector<double> hb(2 * (size_t)N);
@@ -514,6 +515,7 @@ extern "C" int dep_sys_destroy(dep_sys* s)
 {
     if (!s) return DEP_OK;
     (void)hipStreamSynchronize(s->g->ctx->stream);
+    if (s->trsv_exec) (void)hipGraphExecDestroy(s->trsv_exec);
     free_all(s);
     delete s;
     return DEP_OK;
@@ -568,10 +570,28 @@ static int ensure_fp64_factor_storage(dep_sys* s)
     s->ldZ = ((long)g->N + 63) / 64 * 64;
     DEP_CHECK(dep_obj_alloc(s, &s->Zr, (size_t)g->N * s->ldZ));
     DEP_CHECK(dep_obj_alloc(s, &s->Zi, (size_t)g->N * s->ldZ));
-    const size_t nblk = dep_cdiv(g->N, 64);
     double* inv = nullptr;
-    DEP_CHECK(dep_obj_alloc(s, &inv, nblk * 4 * 64 * 64));
+    DEP_CHECK(dep_obj_alloc(s, &inv, (size_t)dep_cdiv(g->N, 64) * 4 * 64 * 64));
+    s->opA64 = inv;
+    return DEP_OK;
+}
+
+static int ensure_mixed_storage(dep_sys* s)
+{
+    dep_geom* g = s->g;
+    if (s->Zr32) return DEP_OK;
+    const int N = g->N;
+    s->ld32 = ((long)N + 127) / 128 * 128;
+    DEP_CHECK(dep_obj_alloc(s, &s->Zr32, (size_t)N * s->ld32));
+    DEP_CHECK(dep_obj_alloc(s, &s->Zi32, (size_t)N * s->ld32));
+    float* inv = nullptr;
+    DEP_CHECK(dep_obj_alloc(s, &inv, (size_t)dep_cdiv(N, 64) * 4 * 64 * 64));
     s->opA = inv;
+    uint16_t* ops = nullptr;   // FP16/BF16 operand panels: N x 512 then 2*ld32 x 512
+    DEP_CHECK(dep_obj_alloc(s, &ops, (size_t)N * 512 + 2 * (size_t)s->ld32 * 512));
+    s->opB = ops;
+    DEP_CHECK(dep_obj_alloc(s, &s->rs, (size_t)N));
+    DEP_CHECK(dep_obj_alloc(s, &s->cs, (size_t)N));
     return DEP_OK;
 }
 
@@ -579,13 +599,20 @@ extern "C" int dep_factor(dep_sys* s, int precision)
 {
     if (!s) return DEP_ERR_ARG;
     dep_ctx* c = s->g->ctx;
-    if (precision != DEP_PREC_FP64)
+    if (s->dense) return dep_fail(c, DEP_ERR_STATE, "dep_factor: not for dense systems");
+    if (precision != DEP_PREC_FP64 && precision != DEP_PREC_FP16 && precision != DEP_PREC_BF16)
         return dep_fail(c, DEP_ERR_ARG, "precision %d is not available in this build", precision);
-    DEP_CHECK(ensure_fp64_factor_storage(s));
-    DEP_HIP(c, hipEventRecord(c->ev0, c->stream));
-    DEP_HIP(c, launch_weight_fp64(s));
+    s->factored = false;
     double mp = 0.0;
-    DEP_HIP(c, launch_lu_fp64(s, &mp));
+    DEP_HIP(c, hipEventRecord(c->ev0, c->stream));
+    if (precision == DEP_PREC_FP64) {
+        DEP_CHECK(ensure_fp64_factor_storage(s));
+        DEP_HIP(c, launch_weight_fp64(s));
+        DEP_HIP(c, launch_lu_fp64(s, &mp));
+    } else {
+        DEP_CHECK(ensure_mixed_storage(s));
+        DEP_HIP(c, launch_lu_mixed(s, precision == DEP_PREC_BF16, &mp));
+    }
     DEP_HIP(c, hipEventRecord(c->ev1, c->stream));
     DEP_HIP(c, hipEventSynchronize(c->ev1));
     float ms = 0.0f;
@@ -599,11 +626,47 @@ extern "C" int dep_factor(dep_sys* s, int precision)
     return DEP_OK;
 }
 
-static double host_norm(const std::vector<double>& v)
+// ||v|| of a 2N vector on the device
+static int dev_norm(dep_sys* s, const double* v, double* out)
 {
-    double a = 0.0;
-    for (double x : v) a += x * x;
-    return sqrt(a);
+    dep_ctx* c = s->g->ctx;
+    double* h = s->dwork + 32768;
+    DEP_HIP(c, launch_dots(c, 1, 2L * s->g->N, v, v, h));
+    double hv = 0.0;
+    DEP_HIP(c, hipMemcpyAsync(&hv, h, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    DEP_HIP(c, hipStreamSynchronize(c->stream));
+    *out = sqrt(hv);
+    return DEP_OK;
+}
+
+// out = M^-1 rhs with the current factor.  The launch sequence of the
+// triangular solves (hundreds of small kernels) is captured once per factor
+// precision into a HIP graph working on s->tbuf and replayed.
+static hipError_t apply_factor(dep_sys* s, const double* rhs, double* out)
+{
+    dep_ctx* c = s->g->ctx;
+    const size_t bytes = sizeof(double) * 2 * (size_t)s->g->N;
+    hipError_t e;
+    if (!s->trsv_exec || s->trsv_prec != s->prec) {
+        if (s->trsv_exec) (void)hipGraphExecDestroy(s->trsv_exec);
+        s->trsv_exec = nullptr;
+        hipGraph_t graph = nullptr;
+        if ((e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal)) != hipSuccess) return e;
+        hipError_t le = s->prec == DEP_PREC_FP64 ? launch_trsv_fp64(s, s->tbuf, s->tbuf)
+                                                 : launch_trsv_mixed(s, s->tbuf, s->tbuf);
+        e = hipStreamEndCapture(c->stream, &graph);
+        if (le != hipSuccess) return le;
+        if (e != hipSuccess) return e;
+        e = hipGraphInstantiate(&s->trsv_exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) return e;
+        s->trsv_prec = s->prec;
+    }
+    if (rhs != s->tbuf && (e = hipMemcpyAsync(s->tbuf, rhs, bytes, hipMemcpyDeviceToDevice, c->stream)) != hipSuccess)
+        return e;
+    if ((e = hipGraphLaunch(s->trsv_exec, c->stream)) != hipSuccess) return e;
+    if (out != s->tbuf) e = hipMemcpyAsync(out, s->tbuf, bytes, hipMemcpyDeviceToDevice, c->stream);
+    return e;
 }
 
 extern "C" int dep_solve(dep_sys* s, double tol, int max_steps, double* sigma, dep_solve_info* info)
@@ -613,28 +676,38 @@ extern "C" int dep_solve(dep_sys* s, double tol, int max_steps, double* sigma, d
     dep_ctx* c = g->ctx;
     if (!s->factored) return dep_fail(c, DEP_ERR_STATE, "dep_solve: system is not factored");
     const int N = g->N;
+    const long n2 = 2L * N;
     double* r = s->work;
-    double* d = s->work + 2 * (size_t)N;
+    double* d = s->work + n2;
     DEP_HIP(c, hipEventRecord(c->ev0, c->stream));
-    DEP_HIP(c, launch_trsv_fp64(s, s->b, s->x));
-    std::vector<double> hb(2 * (size_t)N), hr(2 * (size_t)N);
-    DEP_CHECK(download(c, hb.data(), s->b, hb.size()));
-    const double bn = host_norm(hb);
-    int steps = 0;
-    double res = 0.0;
+    double bn = 0.0;
+    DEP_CHECK(dev_norm(s, s->b, &bn));
+    if (bn == 0.0) bn = 1.0;
+    // x = M^-1 b, then refine: r = b - Z x (FP64, from P), x += M^-1 r
+    DEP_HIP(c, apply_factor(s, s->b, s->x));
+    int steps = 0, fallback = 0;
+    double res = 0.0, prev = 1e300;
     for (;;) {
         DEP_HIP(c, launch_residual(s, s->x, r));
-        DEP_CHECK(download(c, hr.data(), r, hr.size()));
-        res = bn > 0 ? host_norm(hr) / bn : host_norm(hr);
+        DEP_CHECK(dev_norm(s, r, &res));
+        res /= bn;
         if (!(res > tol) || steps >= max_steps) break;
-        DEP_HIP(c, launch_trsv_fp64(s, r, d));
-        // x += d
-        std::vector<double> hx(2 * (size_t)N), hd(2 * (size_t)N);
-        DEP_CHECK(download(c, hx.data(), s->x, hx.size()));
-        DEP_CHECK(download(c, hd.data(), d, hd.size()));
-        for (size_t k = 0; k < hx.size(); k++) hx[k] += hd[k];
-        DEP_CHECK(upload(c, s->x, hx.data(), hx.size()));
+        if (s->prec != DEP_PREC_FP64 && steps >= 3 && !(res < 0.5 * prev)) break;   // stalled
+        prev = res;
+        DEP_HIP(c, apply_factor(s, r, d));
+        DEP_HIP(c, launch_axpy(c, n2, d, s->x));
         steps++;
+    }
+    if (res > tol && s->prec != DEP_PREC_FP64) {
+        // refinement did not reach the tolerance: the FP64 LU gives the answer
+        fallback = 2;
+        const int inner = steps;
+        DEP_CHECK(dep_factor(s, DEP_PREC_FP64));
+        DEP_HIP(c, apply_factor(s, s->b, s->x));
+        DEP_HIP(c, launch_residual(s, s->x, r));
+        DEP_CHECK(dev_norm(s, r, &res));
+        res /= bn;
+        steps = inner;
     }
     DEP_HIP(c, hipEventRecord(c->ev1, c->stream));
     DEP_HIP(c, hipEventSynchronize(c->ev1));
@@ -643,15 +716,14 @@ extern "C" int dep_solve(dep_sys* s, double tol, int max_steps, double* sigma, d
     if (info) {
         memset(info, 0, sizeof(*info));
         info->iterations = steps;
+        info->fallback = fallback;
         info->residual = res;
         info->converged = !(res > tol);
         info->min_pivot = s->min_pivot;
         info->factor_ms = s->factor_ms;
         info->solve_ms = ms;
     }
-    if (sigma) {
-        DEP_CHECK(dep_get_solution(s, sigma));
-    }
+    if (sigma) DEP_CHECK(dep_get_solution(s, sigma));
     if (!std::isfinite(res)) return dep_fail(c, DEP_ERR_SINGULAR, "non-finite residual after the solve");
     return DEP_OK;
 }

@@ -162,7 +162,7 @@ struct Dense {
         check(dep_obj_alloc(&s, &s.Zr, (size_t)n * s.ldZ), "alloc");
         check(dep_obj_alloc(&s, &s.Zi, (size_t)n * s.ldZ), "alloc");
         check(dep_obj_alloc(&s, &inv, nblk * 4 * 64 * 64), "alloc");
-        s.opA = inv;
+        s.opA64 = inv;
         check(dep_obj_alloc(&s, &s.b, 2 * (size_t)n), "alloc");
         check(dep_obj_alloc(&s, &s.x, 2 * (size_t)n), "alloc");
         check(dep_obj_alloc(&s, &s.dwork, 65536), "alloc");

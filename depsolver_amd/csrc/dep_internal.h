@@ -100,14 +100,19 @@ struct dep_sys {
     long ldZ = 0;
     double *Zr = nullptr, *Zi = nullptr;    // FP64 factors
     float *Zr32 = nullptr, *Zi32 = nullptr; // FP32 factors
-    float *rs = nullptr, *cs = nullptr;     // equilibration (row, column)
-    void *opA = nullptr, *opB = nullptr;    // low-precision GEMM operand panels
+    double *rs = nullptr, *cs = nullptr;    // equilibration (row, column)
+    long ld32 = 0;                          // leading dimension of the FP32 planes
+    void *opA = nullptr, *opB = nullptr;    // FP32 inverse blocks, low-precision GEMM operands
+    void* opA64 = nullptr;                  // FP64 inverse blocks
     double min_pivot = 0.0;
     double factor_ms = 0.0;
     // solve workspace
     double* work = nullptr;           // 8 vectors of 2N
     double* dwork = nullptr;          // reduction scratch / pivots
     double* qs = nullptr;             // E x 7 complex sigma at quadrature points
+    double* tbuf = nullptr;           // 2N vector the captured triangular solve works on
+    hipGraphExec_t trsv_exec = nullptr;  // captured triangular-solve sequence
+    int trsv_prec = -1;
     size_t bytes = 0;                 // device bytes owned by the system
     std::vector<std::pair<void*, size_t>> allocs;
     dep_ctx* ctx_of() { return g->ctx; }
@@ -158,5 +163,9 @@ hipError_t launch_dots(dep_ctx* c, int nvec, long n2, const double* V, const dou
 hipError_t launch_update(dep_ctx* c, int nvec, long n2, const double* V, const double* h, double* w);
 hipError_t launch_scale(dep_ctx* c, long n2, const double* w, double alpha, double* out);
 hipError_t launch_combine(dep_ctx* c, int nvec, long n2, const double* V, const double* y, double* x);
+size_t mixed_bytes(int N);
+hipError_t launch_lu_mixed(dep_sys* s, int bf16, double* min_pivot);
+hipError_t launch_trsv_mixed(dep_sys* s, const double* rhs, double* out);
+hipError_t launch_axpy(dep_ctx* c, long n, const double* d, double* x);
 hipError_t launch_dense_zmatvec(dep_ctx* c, int N, const double* Zr, const double* Zi, long ldZ,
                                 const double* x, const double* b, double* y);

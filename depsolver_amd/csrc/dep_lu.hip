@@ -1,0 +1,805 @@
+// dep_lu.hip -- refined mixed-precision factorization of the complex system.
+//
+// The mixed form of the dense solve (replacing gaussBksb, src/gaussBksb.c:39):
+//
+//   1. equilibrate: W = Dr Z Dc with Dr = 1/max_j|Z_ij| and Dc = 1/max_i|(Dr Z)_ij|,
+//      built in FP32 (split planes) straight from the FP64 geometric blocks P
+//      and the per-row complex weights, so Z is never stored in FP64;
+//   2. blocked right-looking LU without pivoting (the reference's policy),
+//      outer panels of 256 columns factored in FP32 with 64-wide inner
+//      blocks (LU of the diagonal block in LDS, its triangular inverses kept
+//      for the solves), U12 by block forward substitution;
+//   3. the trailing update W22 -= L21 U12 on MFMA: L21 and U12 are rounded to
+//      FP16 (or BF16) and packed as a real GEMM of depth 2*256
+//      ([Lr | Li] x [[Ur, Ui], [-Ui, Ur]]) on v_mfma_f32_32x32x16_{f16,bf16}
+//      with FP32 accumulation into the FP32 working matrix;
+//   4. the solve applies U^-1 L^-1 with the stored inverse diagonal blocks
+//      (FP32 factors, FP64 vectors); accuracy is recovered by FP64 iterative
+//      refinement on r = b - Z x evaluated from P (dep_solve.hip).
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+
+#include "dep_common.h"
+#include "dep_internal.h"
+
+#define NBI 64     // inner block (diagonal LU in LDS, inverse blocks)
+#define NBO 256    // outer panel width = trailing-update depth
+
+namespace {
+
+template <class T>
+struct cx {
+    T r, i;
+};
+template <class T>
+__device__ __forceinline__ cx<T> cmul(cx<T> a, cx<T> b)
+{
+    return {a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r};
+}
+template <class T>
+__device__ __forceinline__ cx<T> cdiv(cx<T> a, cx<T> b)
+{
+    const T d = b.r * b.r + b.i * b.i;
+    return {(a.r * b.r + a.i * b.i) / d, (a.i * b.r - a.r * b.i) / d};
+}
+
+// ---- equilibration and the FP32 working matrix ------------------------------
+// one wave per row: rs[i] = 1 / max_j |w_i P_ij + shift_i delta_ij|
+__global__ __launch_bounds__(256) void k_rowmax(int N, const double* __restrict__ P, long ldP,
+                                                const double* __restrict__ wr, const double* __restrict__ wi,
+                                                const double* __restrict__ shift, double* __restrict__ rs)
+{
+    const int lane = threadIdx.x & 63, i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= N) return;
+    const double a = wr[i], b = wi[i];
+    double m = 0.0;
+    for (int c = lane; c < N; c += 64) {
+        const double p = P[(long)i * ldP + c];
+        const double zr = a * p + (c == i ? shift[i] : 0.0), zi = b * p;
+        m = fmax(m, zr * zr + zi * zi);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+    if (lane == 0) rs[i] = m > 0.0 ? 1.0 / sqrt(m) : 1.0;
+}
+
+// column maxima of |rs_i Z_ij| over a chunk of rows; squared magnitudes are
+// non-negative, so their IEEE bit patterns order like unsigned integers
+__global__ void k_colmax(int N, int rows_per, const double* __restrict__ P, long ldP,
+                         const double* __restrict__ wr, const double* __restrict__ wi,
+                         const double* __restrict__ shift, const double* __restrict__ rs,
+                         unsigned long long* __restrict__ cm)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N) return;
+    const int i0 = blockIdx.y * rows_per, i1 = min(N, i0 + rows_per);
+    double m = 0.0;
+    for (int i = i0; i < i1; i++) {
+        const double p = P[(long)i * ldP + c];
+        const double zr = rs[i] * (wr[i] * p + (c == i ? shift[i] : 0.0)), zi = rs[i] * wi[i] * p;
+        m = fmax(m, zr * zr + zi * zi);
+    }
+    atomicMax(cm + c, (unsigned long long)__double_as_longlong(m));
+}
+
+__global__ void k_colscale(int N, const unsigned long long* __restrict__ cm, double* __restrict__ cs)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N) return;
+    const double m = __longlong_as_double((long long)cm[c]);
+    cs[c] = m > 0.0 ? 1.0 / sqrt(m) : 1.0;
+}
+
+__global__ void k_build_w(int N, const double* __restrict__ P, long ldP, const double* __restrict__ wr,
+                          const double* __restrict__ wi, const double* __restrict__ shift,
+                          const double* __restrict__ rs, const double* __restrict__ cs,
+                          float* __restrict__ Wr, float* __restrict__ Wi, long ldW)
+{
+    const int i = blockIdx.y;
+    const double a = wr[i] * rs[i], b = wi[i] * rs[i], sh = shift[i] * rs[i];
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < N; c += gridDim.x * blockDim.x) {
+        const double p = P[(long)i * ldP + c];
+        Wr[(long)i * ldW + c] = (float)((a * p + (c == i ? sh : 0.0)) * cs[c]);
+        Wi[(long)i * ldW + c] = (float)(b * p * cs[c]);
+    }
+}
+
+// ---- FP32 panel kernels -----------------------------------------------------
+// LU of the nb x nb diagonal block at (k0,k0) in LDS, then its inverses
+// Linv (unit lower) and Uinv (upper) into inv (2 x NBI x NBI complex)
+__global__ __launch_bounds__(256) void k_diag32(int nb, float* __restrict__ Zr, float* __restrict__ Zi,
+                                                long ldZ, int k0, float* __restrict__ inv,
+                                                float* __restrict__ piv)
+{
+    __shared__ float Dr[NBI][NBI + 1], Di[NBI][NBI + 1];
+    __shared__ float Ir[NBI][NBI + 1], Ii[NBI][NBI + 1];
+    const int t = threadIdx.x;
+    for (int idx = t; idx < nb * nb; idx += 256) {
+        const int r = idx / nb, c = idx % nb;
+        Dr[r][c] = Zr[(long)(k0 + r) * ldZ + k0 + c];
+        Di[r][c] = Zi[(long)(k0 + r) * ldZ + k0 + c];
+    }
+    __syncthreads();
+    for (int p = 0; p < nb; p++) {
+        const cx<float> pv = {Dr[p][p], Di[p][p]};
+        for (int r = p + 1 + t; r < nb; r += 256) {
+            const cx<float> l = cdiv(cx<float>{Dr[r][p], Di[r][p]}, pv);
+            Dr[r][p] = l.r;
+            Di[r][p] = l.i;
+        }
+        __syncthreads();
+        const int m = nb - p - 1;
+        for (int idx = t; idx < m * m; idx += 256) {
+            const int r = p + 1 + idx / m, c = p + 1 + idx % m;
+            const cx<float> lu = cmul(cx<float>{Dr[r][p], Di[r][p]}, cx<float>{Dr[p][c], Di[p][c]});
+            Dr[r][c] -= lu.r;
+            Di[r][c] -= lu.i;
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        float mn = 3.0e38f;
+        for (int p = 0; p < nb; p++) mn = fminf(mn, sqrtf(Dr[p][p] * Dr[p][p] + Di[p][p] * Di[p][p]));
+        piv[k0 / NBI] = mn;
+    }
+    for (int idx = t; idx < nb * nb; idx += 256) {
+        const int r = idx / nb, c = idx % nb;
+        Zr[(long)(k0 + r) * ldZ + k0 + c] = Dr[r][c];
+        Zi[(long)(k0 + r) * ldZ + k0 + c] = Di[r][c];
+    }
+    // Linv by Gauss-Jordan on [L | I] (unit lower): X = I; X[r][:] -= L[r][p] X[p][:] for r > p
+    for (int idx = t; idx < NBI * NBI; idx += 256) {
+        const int r = idx / NBI, c = idx % NBI;
+        Ir[r][c] = (r == c) ? 1.0f : 0.0f;
+        Ii[r][c] = 0.0f;
+    }
+    __syncthreads();
+    for (int p = 0; p < nb - 1; p++) {
+        const int w = p + 1, m = nb - 1 - p;
+        for (int idx = t; idx < m * w; idx += 256) {
+            const int r = p + 1 + idx / w, j = idx % w;
+            const cx<float> lx = cmul(cx<float>{Dr[r][p], Di[r][p]}, cx<float>{Ir[p][j], Ii[p][j]});
+            Ir[r][j] -= lx.r;
+            Ii[r][j] -= lx.i;
+        }
+        __syncthreads();
+    }
+    for (int idx = t; idx < nb * nb; idx += 256) {
+        const int r = idx / nb, c = idx % nb;
+        inv[2 * (r * NBI + c)] = Ir[r][c];
+        inv[2 * (r * NBI + c) + 1] = Ii[r][c];
+    }
+    __syncthreads();
+    // Uinv by backward Gauss-Jordan on [U | I]
+    for (int idx = t; idx < NBI * NBI; idx += 256) {
+        const int r = idx / NBI, c = idx % NBI;
+        Ir[r][c] = (r == c) ? 1.0f : 0.0f;
+        Ii[r][c] = 0.0f;
+    }
+    __syncthreads();
+    for (int p = nb - 1; p >= 0; p--) {
+        const cx<float> d = {Dr[p][p], Di[p][p]};
+        for (int j = p + t; j < nb; j += 256) {
+            const cx<float> v = cdiv(cx<float>{Ir[p][j], Ii[p][j]}, d);
+            Ir[p][j] = v.r;
+            Ii[p][j] = v.i;
+        }
+        __syncthreads();
+        const int w = nb - p;
+        for (int idx = t; idx < p * w; idx += 256) {
+            const int r = idx / w, j = p + idx % w;
+            const cx<float> ux = cmul(cx<float>{Dr[r][p], Di[r][p]}, cx<float>{Ir[p][j], Ii[p][j]});
+            Ir[r][j] -= ux.r;
+            Ii[r][j] -= ux.i;
+        }
+        __syncthreads();
+    }
+    float* Ui = inv + 2 * NBI * NBI;
+    for (int idx = t; idx < nb * nb; idx += 256) {
+        const int r = idx / nb, c = idx % nb;
+        Ui[2 * (r * NBI + c)] = Ir[r][c];
+        Ui[2 * (r * NBI + c) + 1] = Ii[r][c];
+    }
+}
+
+// rows [r0, N): W[i, k0:k0+nb] <- W[i, k0:k0+nb] Uinv   (16 rows per block)
+__global__ __launch_bounds__(256) void k_uinv32(int N, int nb, int r0, float* __restrict__ Zr,
+                                                float* __restrict__ Zi, long ldZ, int k0,
+                                                const float* __restrict__ Uinv)
+{
+    __shared__ float Ar[16][NBI], Ai[16][NBI];
+    __shared__ float Ur[NBI][NBI + 1], Ui[NBI][NBI + 1];
+    const int t = threadIdx.x;
+    const int rb = r0 + blockIdx.x * 16;
+    for (int idx = t; idx < nb * nb; idx += 256) {
+        const int r = idx / nb, c = idx % nb;
+        Ur[r][c] = Uinv[2 * (r * NBI + c)];
+        Ui[r][c] = Uinv[2 * (r * NBI + c) + 1];
+    }
+    for (int idx = t; idx < 16 * nb; idx += 256) {
+        const int r = idx / nb, c = idx % nb, gi = rb + r;
+        Ar[r][c] = gi < N ? Zr[(long)gi * ldZ + k0 + c] : 0.0f;
+        Ai[r][c] = gi < N ? Zi[(long)gi * ldZ + k0 + c] : 0.0f;
+    }
+    __syncthreads();
+    for (int idx = t; idx < 16 * nb; idx += 256) {
+        const int r = idx / nb, c = idx % nb, gi = rb + r;
+        float sr = 0.0f, si = 0.0f;
+        for (int q = 0; q <= c; q++) {
+            sr += Ar[r][q] * Ur[q][c] - Ai[r][q] * Ui[q][c];
+            si += Ar[r][q] * Ui[q][c] + Ai[r][q] * Ur[q][c];
+        }
+        if (gi < N) {
+            Zr[(long)gi * ldZ + k0 + c] = sr;
+            Zi[(long)gi * ldZ + k0 + c] = si;
+        }
+    }
+}
+
+// columns [c0, c1): W[k0:k0+nb, j] <- Linv W[k0:k0+nb, j]   (16 columns per block)
+__global__ __launch_bounds__(256) void k_linv32(int c1, int nb, int c0, float* __restrict__ Zr,
+                                                float* __restrict__ Zi, long ldZ, int k0,
+                                                const float* __restrict__ Linv)
+{
+    __shared__ float Br[NBI][16], Bi[NBI][16];
+    __shared__ float Lr[NBI][NBI + 1], Li[NBI][NBI + 1];
+    const int t = threadIdx.x;
+    const int cb = c0 + blockIdx.x * 16;
+    for (int idx = t; idx < nb * nb; idx += 256) {
+        const int r = idx / nb, c = idx % nb;
+        Lr[r][c] = Linv[2 * (r * NBI + c)];
+        Li[r][c] = Linv[2 * (r * NBI + c) + 1];
+    }
+    for (int idx = t; idx < 16 * nb; idx += 256) {
+        const int r = idx / 16, c = idx % 16, gc = cb + c;
+        Br[r][c] = gc < c1 ? Zr[(long)(k0 + r) * ldZ + gc] : 0.0f;
+        Bi[r][c] = gc < c1 ? Zi[(long)(k0 + r) * ldZ + gc] : 0.0f;
+    }
+    __syncthreads();
+    for (int idx = t; idx < 16 * nb; idx += 256) {
+        const int r = idx / 16, c = idx % 16, gc = cb + c;
+        float sr = 0.0f, si = 0.0f;
+        for (int q = 0; q <= r; q++) {
+            sr += Lr[r][q] * Br[q][c] - Li[r][q] * Bi[q][c];
+            si += Lr[r][q] * Bi[q][c] + Li[r][q] * Br[q][c];
+        }
+        if (gc < c1) {
+            Zr[(long)(k0 + r) * ldZ + gc] = sr;
+            Zi[(long)(k0 + r) * ldZ + gc] = si;
+        }
+    }
+}
+
+// W[ra:rb, ca:cb] -= W[ra:rb, kc:kc+nbk] W[kc:kc+nbk, ca:cb]  (FP32, 64x64 tiles)
+__global__ __launch_bounds__(256) void k_gemm32(int ra, int rb, int ca, int cb, int kc, int nbk,
+                                                float* __restrict__ Zr, float* __restrict__ Zi, long ldZ)
+{
+    constexpr int BM = 64, BN = 64, BK = 16;
+    __shared__ float Asr[BK][BM + 1], Asi[BK][BM + 1];
+    __shared__ float Bsr[BK][BN + 1], Bsi[BK][BN + 1];
+    const int t = threadIdx.x;
+    const int tm = t / 16, tn = t % 16;
+    const int m0 = ra + blockIdx.y * BM, n0 = ca + blockIdx.x * BN;
+    float cr[4][4], ci[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) cr[a][b] = ci[a][b] = 0.0f;
+    for (int kk = 0; kk < nbk; kk += BK) {
+        for (int idx = t; idx < BM * BK; idx += 256) {
+            const int r = idx / BK, k = idx % BK, gi = m0 + r;
+            const bool ok = gi < rb && (kk + k) < nbk;
+            Asr[k][r] = ok ? Zr[(long)gi * ldZ + kc + kk + k] : 0.0f;
+            Asi[k][r] = ok ? Zi[(long)gi * ldZ + kc + kk + k] : 0.0f;
+        }
+        for (int idx = t; idx < BK * BN; idx += 256) {
+            const int k = idx / BN, c = idx % BN, gc = n0 + c;
+            const bool ok = gc < cb && (kk + k) < nbk;
+            Bsr[k][c] = ok ? Zr[(long)(kc + kk + k) * ldZ + gc] : 0.0f;
+            Bsi[k][c] = ok ? Zi[(long)(kc + kk + k) * ldZ + gc] : 0.0f;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < BK; k++) {
+            float ar[4], ai[4], br[4], bi[4];
+#pragma unroll
+            for (int a = 0; a < 4; a++) {
+                ar[a] = Asr[k][tm + 16 * a];
+                ai[a] = Asi[k][tm + 16 * a];
+                br[a] = Bsr[k][tn + 16 * a];
+                bi[a] = Bsi[k][tn + 16 * a];
+            }
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int b = 0; b < 4; b++) {
+                    cr[a][b] += ar[a] * br[b] - ai[a] * bi[b];
+                    ci[a][b] += ar[a] * bi[b] + ai[a] * br[b];
+                }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            const int gi = m0 + tm + 16 * a, gc = n0 + tn + 16 * b;
+            if (gi < rb && gc < cb) {
+                Zr[(long)gi * ldZ + gc] -= cr[a][b];
+                Zi[(long)gi * ldZ + gc] -= ci[a][b];
+            }
+        }
+}
+
+// ---- low-precision operands and the MFMA trailing update --------------------
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+template <class H>
+__device__ __forceinline__ H to_h(float v);
+template <>
+__device__ __forceinline__ _Float16 to_h<_Float16>(float v) { return (_Float16)v; }
+template <>
+__device__ __forceinline__ __bf16 to_h<__bf16>(float v) { return (__bf16)v; }
+
+// A'[i][k] = [Lr | Li] of L21 = W[r0+i, k0+k], K2 = 2*nb padded with zeros to K2p
+template <class H>
+__global__ void k_pack_a(int M, int nb, int K2p, int r0, int k0, const float* __restrict__ Zr,
+                         const float* __restrict__ Zi, long ldZ, H* __restrict__ A)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y;
+    if (k >= K2p || i >= M) return;
+    float v = 0.0f;
+    if (k < nb) v = Zr[(long)(r0 + i) * ldZ + k0 + k];
+    else if (k < 2 * nb) v = Zi[(long)(r0 + i) * ldZ + k0 + k - nb];
+    A[(long)i * K2p + k] = to_h<H>(v);
+}
+
+// B'^T rows: n < NCp -> [Ur ; -Ui] of column n (real part of the product),
+// n >= NCp -> [Ui ; Ur] (imaginary part); one block per 64-column group,
+// staged through LDS so both the reads of U12 and the writes are coalesced.
+template <class H>
+__global__ __launch_bounds__(256) void k_pack_b(int NC, int NCp, int nb, int K2p, int k0, int c0,
+                                                const float* __restrict__ Zr,
+                                                const float* __restrict__ Zi, long ldZ,
+                                                H* __restrict__ BT)
+{
+    __shared__ float sr[NBO][65], si[NBO][65];
+    const int n0 = blockIdx.x * 64;
+    const int t = threadIdx.x;
+    for (int idx = t; idx < nb * 64; idx += 256) {
+        const int k = idx / 64, c = idx % 64, gc = n0 + c;
+        sr[k][c] = gc < NC ? Zr[(long)(k0 + k) * ldZ + c0 + gc] : 0.0f;
+        si[k][c] = gc < NC ? Zi[(long)(k0 + k) * ldZ + c0 + gc] : 0.0f;
+    }
+    __syncthreads();
+    for (int idx = t; idx < 64 * K2p; idx += 256) {
+        const int c = idx / K2p, k = idx % K2p, n = n0 + c;
+        if (n >= NCp) continue;
+        float vr = 0.0f, vi = 0.0f;
+        if (k < nb) {
+            vr = sr[k][c];
+            vi = si[k][c];
+        } else if (k < 2 * nb) {
+            vr = -si[k - nb][c];
+            vi = sr[k - nb][c];
+        }
+        BT[(long)n * K2p + k] = to_h<H>(vr);
+        BT[(long)(NCp + n) * K2p + k] = to_h<H>(vi);
+    }
+}
+
+// C[r0+m][r0+n] -= sum_k A'[m][k] B'^T[n][k] on v_mfma_f32_32x32x16_{f16,bf16}.
+// 128x128 output tile per workgroup (4 waves of 64x64 = 2x2 MFMA blocks),
+// operands staged in LDS 32 deep.  Lane l of an MFMA block holds A rows l%32
+// with k = 8*(l/32)..+7, and accumulator v sits at row 8*(v/4) + 4*(l/32) + v%4,
+// column l%32.  Column tiles below NCp update the real plane, the rest the
+// imaginary plane.
+template <class H, class V8>
+__global__ __launch_bounds__(256) void k_gemm_mfma(int M, int NC, int NCp, int K2p, int r0,
+                                                   const H* __restrict__ A, const H* __restrict__ BT,
+                                                   float* __restrict__ Zr, float* __restrict__ Zi,
+                                                   long ldZ)
+{
+    constexpr int TK = 32, LDA = TK + 8;
+    __shared__ H As[128][LDA], Bs[128][LDA];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+    const int m0 = blockIdx.y * 128;
+    const int n0 = blockIdx.x * 128;        // in [0, 2*NCp)
+    f32x16 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int v = 0; v < 16; v++) acc[a][b][v] = 0.0f;
+    for (int k0 = 0; k0 < K2p; k0 += TK) {
+        // 128 rows x 32 halves per operand = 512 chunks of 8; two per thread
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int ch = t + 256 * h, r = ch >> 2, kc = (ch & 3) * 8;
+            const int gm = m0 + r, gn = n0 + r;
+            V8 va, vb;
+            if (gm < M) va = *(const V8*)(A + (long)gm * K2p + k0 + kc);
+            else
+#pragma unroll
+                for (int j = 0; j < 8; j++) va[j] = to_h<H>(0.0f);
+            vb = *(const V8*)(BT + (long)gn * K2p + k0 + kc);
+            *(V8*)(&As[r][kc]) = va;
+            *(V8*)(&Bs[r][kc]) = vb;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < TK; ks += 16) {
+            V8 fa[2], fb[2];
+#pragma unroll
+            for (int a = 0; a < 2; a++) {
+                fa[a] = *(const V8*)(&As[wm + 32 * a + (lane & 31)][ks + 8 * (lane >> 5)]);
+                fb[a] = *(const V8*)(&Bs[wn + 32 * a + (lane & 31)][ks + 8 * (lane >> 5)]);
+            }
+#pragma unroll
+            for (int a = 0; a < 2; a++)
+#pragma unroll
+                for (int b = 0; b < 2; b++) {
+                    if constexpr (sizeof(H) == 2 && __is_same(H, _Float16))
+                        acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(fa[a], fb[b], acc[a][b], 0, 0, 0);
+                    else
+                        acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[a], fb[b], acc[a][b], 0, 0, 0);
+                }
+        }
+        __syncthreads();
+    }
+    const bool imag = n0 >= NCp;
+    float* C = imag ? Zi : Zr;
+    const int nb0 = imag ? n0 - NCp : n0;
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int v = 0; v < 16; v++) {
+                const int m = m0 + wm + 32 * a + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+                const int n = nb0 + wn + 32 * b + (lane & 31);
+                if (m < M && n < NC) C[(long)(r0 + m) * ldZ + r0 + n] -= acc[a][b][v];
+            }
+}
+
+// Larger-tile variant: 256 (M) x 128 (N) per workgroup, 4 waves of 128 x 64
+// (4 x 2 MFMA blocks), the next 32-deep K chunk prefetched into registers while
+// the current one is consumed from LDS.
+template <class H, class V8>
+__global__ __launch_bounds__(256) void k_gemm_mfma2(int M, int NC, int NCp, int K2p, int r0,
+                                                    const H* __restrict__ A, const H* __restrict__ BT,
+                                                    float* __restrict__ Zr, float* __restrict__ Zi,
+                                                    long ldZ)
+{
+    constexpr int TK = 32, LDA = TK + 8;
+    __shared__ H As[256][LDA], Bs[128][LDA];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int wm = (wave >> 1) * 128, wn = (wave & 1) * 64;
+    const int m0 = blockIdx.y * 256;
+    const int n0 = blockIdx.x * 128;        // in [0, 2*NCp)
+    f32x16 acc[4][2];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int v = 0; v < 16; v++) acc[a][b][v] = 0.0f;
+    // per thread: A chunks ch = t + 256h (h < 4): row ch>>2, k (ch&3)*8; B chunks h < 2
+    V8 ra[4], rb[2];
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int h = 0; h < 4; h++) {
+            const int ch = t + 256 * h, r = ch >> 2, kc = (ch & 3) * 8, gm = m0 + r;
+            if (gm < M) ra[h] = *(const V8*)(A + (long)gm * K2p + k0 + kc);
+            else
+#pragma unroll
+                for (int j = 0; j < 8; j++) ra[h][j] = to_h<H>(0.0f);
+        }
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int ch = t + 256 * h, r = ch >> 2, kc = (ch & 3) * 8;
+            rb[h] = *(const V8*)(BT + (long)(n0 + r) * K2p + k0 + kc);
+        }
+    };
+    auto lstore = [&]() {
+#pragma unroll
+        for (int h = 0; h < 4; h++) {
+            const int ch = t + 256 * h;
+            *(V8*)(&As[ch >> 2][(ch & 3) * 8]) = ra[h];
+        }
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int ch = t + 256 * h;
+            *(V8*)(&Bs[ch >> 2][(ch & 3) * 8]) = rb[h];
+        }
+    };
+    gload(0);
+    lstore();
+    __syncthreads();
+    for (int k0 = 0; k0 < K2p; k0 += TK) {
+        const bool more = k0 + TK < K2p;
+        if (more) gload(k0 + TK);
+#pragma unroll
+        for (int ks = 0; ks < TK; ks += 16) {
+            V8 fa[4], fb[2];
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+                fa[a] = *(const V8*)(&As[wm + 32 * a + (lane & 31)][ks + 8 * (lane >> 5)]);
+#pragma unroll
+            for (int b = 0; b < 2; b++)
+                fb[b] = *(const V8*)(&Bs[wn + 32 * b + (lane & 31)][ks + 8 * (lane >> 5)]);
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int b = 0; b < 2; b++) {
+                    if constexpr (__is_same(H, _Float16))
+                        acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(fa[a], fb[b], acc[a][b], 0, 0, 0);
+                    else
+                        acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[a], fb[b], acc[a][b], 0, 0, 0);
+                }
+        }
+        __syncthreads();
+        if (more) {
+            lstore();
+            __syncthreads();
+        }
+    }
+    const bool imag = n0 >= NCp;
+    float* C = imag ? Zi : Zr;
+    const int nb0 = imag ? n0 - NCp : n0;
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int v = 0; v < 16; v++) {
+                const int m = m0 + wm + 32 * a + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+                const int n = nb0 + wn + 32 * b + (lane & 31);
+                if (m < M && n < NC) C[(long)(r0 + m) * ldZ + r0 + n] -= acc[a][b][v];
+            }
+}
+
+// ---- triangular solves with FP32 factors, FP64 vectors ---------------------
+__global__ __launch_bounds__(256) void k_block_apply32(int nb, const float* __restrict__ Binv,
+                                                       double* __restrict__ vr, double* __restrict__ vi,
+                                                       int lower)
+{
+    __shared__ float Br[NBI][NBI + 1], Bi[NBI][NBI + 1];
+    __shared__ double xr[NBI], xi[NBI];
+    const int t = threadIdx.x;
+    for (int idx = t; idx < NBI * NBI; idx += 256) {
+        const int r = idx / NBI, c = idx % NBI;
+        Br[r][c] = Binv[2 * idx];
+        Bi[r][c] = Binv[2 * idx + 1];
+    }
+    if (t < nb) {
+        xr[t] = vr[t];
+        xi[t] = vi[t];
+    }
+    __syncthreads();
+    const int r = t >> 2, part = t & 3;
+    double sr = 0.0, si = 0.0;
+    if (r < nb) {
+        const int q0 = lower ? 0 : r, q1 = lower ? r : nb - 1;
+        for (int q = q0 + part; q <= q1; q += 4) {
+            const double br = Br[r][q], bi = Bi[r][q];
+            sr += br * xr[q] - bi * xi[q];
+            si += br * xi[q] + bi * xr[q];
+        }
+    }
+    sr += __shfl_xor(sr, 1);
+    si += __shfl_xor(si, 1);
+    sr += __shfl_xor(sr, 2);
+    si += __shfl_xor(si, 2);
+    if (r < nb && part == 0) {
+        vr[r] = sr;
+        vi[r] = si;
+    }
+}
+
+// x[rows] -= W[rows, k0:k0+nb] x[k0:k0+nb]  for rows in [ra, rb): one wave per 4 rows
+__global__ __launch_bounds__(256) void k_panel_gemv32(int ra, int rb, int nb, const float* __restrict__ Zr,
+                                                      const float* __restrict__ Zi, long ldZ, int k0,
+                                                      double* __restrict__ xr, double* __restrict__ xi)
+{
+    const int lane = threadIdx.x & 63;
+    const int i = ra + blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= rb) return;
+    double sr = 0.0, si = 0.0;
+    for (int q = lane; q < nb; q += 64) {
+        const double ar = Zr[(long)i * ldZ + k0 + q], ai = Zi[(long)i * ldZ + k0 + q];
+        sr += ar * xr[k0 + q] - ai * xi[k0 + q];
+        si += ar * xi[k0 + q] + ai * xr[k0 + q];
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        sr += __shfl_xor(sr, o);
+        si += __shfl_xor(si, o);
+    }
+    if (lane == 0) {
+        xr[i] -= sr;
+        xi[i] -= si;
+    }
+}
+
+__global__ void k_vscale(int N, const double* __restrict__ s, const double* __restrict__ in,
+                         double* __restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    out[i] = s[i] * in[i];
+    out[N + i] = s[i] * in[N + i];
+}
+
+__global__ void k_axpy(long n, const double* __restrict__ d, double* __restrict__ x)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] += d[i];
+}
+
+}  // namespace
+
+// Mixed storage in dep_sys: Zr32/Zi32 (N x ldZ FP32), opA = FP32 inverse blocks
+// (nblk x 2 x 64 x 64 complex), opB = packed operands (A' then B'^T), rs/cs.
+size_t mixed_bytes(int N)
+{
+    const size_t ld = ((size_t)N + 127) / 128 * 128;
+    const size_t nblk = ((size_t)N + NBI - 1) / NBI;
+    const size_t NCp = ld;
+    return 2 * (size_t)N * ld * 4 + nblk * 4 * NBI * NBI * 4 + (size_t)N * 2 * NBO * 2 +
+           2 * NCp * 2 * NBO * 2;
+}
+
+hipError_t launch_lu_mixed(dep_sys* s, int bf16, double* min_pivot)
+{
+    dep_geom* g = s->g;
+    dep_ctx* c = g->ctx;
+    hipStream_t st = c->stream;
+    const int N = g->N;
+    const long ld = s->ld32;
+    float* Wr = s->Zr32;
+    float* Wi = s->Zi32;
+    float* inv = (float*)s->opA;
+    float* piv = (float*)s->dwork;
+    const int NCmax = (int)ld;
+    _Float16* Ah = (_Float16*)s->opB;
+    // DEPSOLVER_MIXED_GEMM=v1 selects the 128x128-tile trailing update (for comparison)
+    const char* gsel = getenv("DEPSOLVER_MIXED_GEMM");
+    const bool gemm_v1 = gsel && strcmp(gsel, "v1") == 0;
+    _Float16* Bh = Ah + (size_t)N * 2 * NBO;
+    hipError_t e;
+    // 1. equilibration and W
+    hipLaunchKernelGGL(k_rowmax, dim3(dep_cdiv(N, 4)), dim3(256), 0, st, N, g->P, g->ldP, s->wr, s->wi,
+                       s->shift, (double*)s->rs);
+    e = hipMemsetAsync(s->cs, 0, sizeof(double) * N, st);
+    if (e != hipSuccess) return e;
+    const int rows_per = 512;
+    hipLaunchKernelGGL(k_colmax, dim3(dep_cdiv(N, 256), dep_cdiv(N, rows_per)), dim3(256), 0, st, N,
+                       rows_per, g->P, g->ldP, s->wr, s->wi, s->shift, (const double*)s->rs,
+                       (unsigned long long*)s->cs);
+    hipLaunchKernelGGL(k_colscale, dim3(dep_cdiv(N, 256)), dim3(256), 0, st, N,
+                       (const unsigned long long*)s->cs, (double*)s->cs);
+    hipLaunchKernelGGL(k_build_w, dim3(dep_cdiv(N, 1024), N), dim3(256), 0, st, N, g->P, g->ldP, s->wr,
+                       s->wi, s->shift, (const double*)s->rs, (const double*)s->cs, Wr, Wi, ld);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    // 2-3. blocked LU
+    for (int k0 = 0; k0 < N; k0 += NBO) {
+        const int nbo = std::min(NBO, N - k0), pe = k0 + nbo;
+        for (int kj = k0; kj < pe; kj += NBI) {
+            const int nbi = std::min(NBI, pe - kj);
+            float* invb = inv + (size_t)(kj / NBI) * 4 * NBI * NBI;
+            hipLaunchKernelGGL(k_diag32, dim3(1), dim3(256), 0, st, nbi, Wr, Wi, ld, kj, invb, piv);
+            if (kj + nbi < N)
+                hipLaunchKernelGGL(k_uinv32, dim3(dep_cdiv(N - kj - nbi, 16)), dim3(256), 0, st, N, nbi,
+                                   kj + nbi, Wr, Wi, ld, kj, invb + 2 * NBI * NBI);
+            if (kj + nbi < pe) {
+                hipLaunchKernelGGL(k_linv32, dim3(dep_cdiv(pe - kj - nbi, 16)), dim3(256), 0, st, pe, nbi,
+                                   kj + nbi, Wr, Wi, ld, kj, invb);
+                hipLaunchKernelGGL(k_gemm32, dim3(dep_cdiv(pe - kj - nbi, 64), dep_cdiv(N - kj - nbi, 64)),
+                                   dim3(256), 0, st, kj + nbi, N, kj + nbi, pe, kj, nbi, Wr, Wi, ld);
+            }
+        }
+        if (pe >= N) break;
+        // U12 = L11^-1 A12 by block forward substitution
+        for (int kj = k0; kj < pe; kj += NBI) {
+            const int nbi = std::min(NBI, pe - kj);
+            const float* invb = inv + (size_t)(kj / NBI) * 4 * NBI * NBI;
+            hipLaunchKernelGGL(k_linv32, dim3(dep_cdiv(N - pe, 16)), dim3(256), 0, st, N, nbi, pe, Wr, Wi,
+                               ld, kj, invb);
+            if (kj + nbi < pe)
+                hipLaunchKernelGGL(k_gemm32, dim3(dep_cdiv(N - pe, 64), dep_cdiv(pe - kj - nbi, 64)),
+                                   dim3(256), 0, st, kj + nbi, pe, pe, N, kj, nbi, Wr, Wi, ld);
+        }
+        // low-precision operands and the trailing update
+        const int M = N - pe, NC = N - pe, NCp = (NC + 127) / 128 * 128;
+        const int K2p = (2 * nbo + 31) / 32 * 32;
+        (void)NCmax;
+        if (bf16) {
+            hipLaunchKernelGGL(k_pack_a<__bf16>, dim3(dep_cdiv(K2p, 256), M), dim3(256), 0, st, M, nbo, K2p,
+                               pe, k0, Wr, Wi, ld, (__bf16*)Ah);
+            hipLaunchKernelGGL(k_pack_b<__bf16>, dim3(dep_cdiv(NCp, 64)), dim3(256), 0, st, NC, NCp, nbo,
+                               K2p, k0, pe, Wr, Wi, ld, (__bf16*)Bh);
+        } else {
+            hipLaunchKernelGGL(k_pack_a<_Float16>, dim3(dep_cdiv(K2p, 256), M), dim3(256), 0, st, M, nbo,
+                               K2p, pe, k0, Wr, Wi, ld, Ah);
+            hipLaunchKernelGGL(k_pack_b<_Float16>, dim3(dep_cdiv(NCp, 64)), dim3(256), 0, st, NC, NCp, nbo,
+                               K2p, k0, pe, Wr, Wi, ld, Bh);
+        }
+        hipEvent_t ev;
+        prof_begin(c, &ev);
+        if (gemm_v1) {
+            dim3 grid(2 * NCp / 128, dep_cdiv(M, 128));
+            if (bf16)
+                hipLaunchKernelGGL((k_gemm_mfma<__bf16, bf16x8>), grid, dim3(256), 0, st, M, NC, NCp, K2p, pe,
+                                   (const __bf16*)Ah, (const __bf16*)Bh, Wr, Wi, ld);
+            else
+                hipLaunchKernelGGL((k_gemm_mfma<_Float16, f16x8>), grid, dim3(256), 0, st, M, NC, NCp, K2p,
+                                   pe, (const _Float16*)Ah, (const _Float16*)Bh, Wr, Wi, ld);
+        } else {
+            dim3 grid(2 * NCp / 128, dep_cdiv(M, 256));
+            if (bf16)
+                hipLaunchKernelGGL((k_gemm_mfma2<__bf16, bf16x8>), grid, dim3(256), 0, st, M, NC, NCp, K2p,
+                                   pe, (const __bf16*)Ah, (const __bf16*)Bh, Wr, Wi, ld);
+            else
+                hipLaunchKernelGGL((k_gemm_mfma2<_Float16, f16x8>), grid, dim3(256), 0, st, M, NC, NCp, K2p,
+                                   pe, (const _Float16*)Ah, (const _Float16*)Bh, Wr, Wi, ld);
+        }
+        // complex flops 8 M NC nbo; bytes: C read+write FP32 complex + FP16 operands
+        prof_end(c, ev, "lu_gemm_mixed", 8.0 * M * (double)NC * nbo,
+                 16.0 * M * (double)NC + 2.0 * K2p * (M + 2.0 * NCp));
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    const int nblk = dep_cdiv(N, NBI);
+    std::vector<float> hp(nblk);
+    if ((e = hipMemcpyAsync(hp.data(), piv, sizeof(float) * nblk, hipMemcpyDeviceToHost, st)) != hipSuccess)
+        return e;
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+    double mn = 1e300;
+    for (float v : hp) mn = std::min(mn, (double)v);
+    *min_pivot = mn;
+    return hipSuccess;
+}
+
+// out = Dc U^-1 L^-1 Dr rhs with the FP32 factors (split FP64 vectors; out may alias rhs)
+hipError_t launch_trsv_mixed(dep_sys* s, const double* rhs, double* out)
+{
+    dep_geom* g = s->g;
+    const int N = g->N;
+    hipStream_t st = g->ctx->stream;
+    const int nblk = dep_cdiv(N, NBI);
+    const float* inv = (const float*)s->opA;
+    double* xr = out;
+    double* xi = out + N;
+    hipLaunchKernelGGL(k_vscale, dim3(dep_cdiv(N, 256)), dim3(256), 0, st, N, (const double*)s->rs, rhs, out);
+    for (int kb = 0; kb < nblk; kb++) {
+        const int k0 = kb * NBI, nb = std::min(NBI, N - k0);
+        hipLaunchKernelGGL(k_block_apply32, dim3(1), dim3(256), 0, st, nb, inv + (size_t)kb * 4 * NBI * NBI,
+                           xr + k0, xi + k0, 1);
+        if (k0 + nb < N)
+            hipLaunchKernelGGL(k_panel_gemv32, dim3(dep_cdiv(N - k0 - nb, 4)), dim3(256), 0, st, k0 + nb, N,
+                               nb, s->Zr32, s->Zi32, s->ld32, k0, xr, xi);
+    }
+    for (int kb = nblk - 1; kb >= 0; kb--) {
+        const int k0 = kb * NBI, nb = std::min(NBI, N - k0);
+        hipLaunchKernelGGL(k_block_apply32, dim3(1), dim3(256), 0, st, nb,
+                           inv + (size_t)kb * 4 * NBI * NBI + 2 * NBI * NBI, xr + k0, xi + k0, 0);
+        if (k0 > 0)
+            hipLaunchKernelGGL(k_panel_gemv32, dim3(dep_cdiv(k0, 4)), dim3(256), 0, st, 0, k0, nb, s->Zr32,
+                               s->Zi32, s->ld32, k0, xr, xi);
+    }
+    hipLaunchKernelGGL(k_vscale, dim3(dep_cdiv(N, 256)), dim3(256), 0, st, N, (const double*)s->cs, out, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_axpy(dep_ctx* c, long n, const double* d, double* x)
+{
+    hipLaunchKernelGGL(k_axpy, dim3(dep_cdiv(n, 256)), dim3(256), 0, c->stream, n, d, x);
+    return hipGetLastError();
+}

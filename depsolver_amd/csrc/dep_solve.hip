@@ -14,6 +14,9 @@
 // Z itself is never needed in FP64 for matrix-vector products: Z x is
 // evaluated as w .* (P x) + shift .* x from the real geometric blocks P, which
 // halves the bytes streamed per product (dep_residual below).
+#include <stdlib.h>
+#include <string.h>
+
 #include "dep_common.h"
 #include "dep_internal.h"
 
@@ -90,45 +93,53 @@ __global__ __launch_bounds__(256) void k_lu_diag(int nb, double* __restrict__ Zr
         Zr[(long)(k0 + r) * ldZ + k0 + c] = Dr[r][c];
         Zi[(long)(k0 + r) * ldZ + k0 + c] = Di[r][c];
     }
-    // Linv: column j solved by thread j (forward substitution, unit diagonal)
-    if (t < nb) {
-        const int j = t;
-        for (int r = 0; r < nb; r++) {
-            cplx s = {r == j ? 1.0 : 0.0, 0.0};
-            for (int q = j; q < r; q++) {
-                const cplx lq = cmul({Dr[r][q], Di[r][q]}, {Ir[q][j], Ii[q][j]});
-                s.r -= lq.r;
-                s.i -= lq.i;
-            }
-            Ir[r][j] = (r < j) ? 0.0 : s.r;
-            Ii[r][j] = (r < j) ? 0.0 : s.i;
-        }
+    // Linv by Gauss-Jordan on [L | I] (unit lower): X = I; X[r][:] -= L[r][p] X[p][:] for r > p
+    for (int idx = t; idx < LU_NB * LU_NB; idx += 256) {
+        const int r = idx / LU_NB, c = idx % LU_NB;
+        Ir[r][c] = (r == c) ? 1.0 : 0.0;
+        Ii[r][c] = 0.0;
     }
     __syncthreads();
-    double* Li = inv;
+    for (int p = 0; p < nb - 1; p++) {
+        const int w = p + 1, m = nb - 1 - p;
+        for (int idx = t; idx < m * w; idx += 256) {
+            const int r = p + 1 + idx / w, j = idx % w;
+            const cplx lx = cmul(cplx{Dr[r][p], Di[r][p]}, cplx{Ir[p][j], Ii[p][j]});
+            Ir[r][j] -= lx.r;
+            Ii[r][j] -= lx.i;
+        }
+        __syncthreads();
+    }
     for (int idx = t; idx < nb * nb; idx += 256) {
         const int r = idx / nb, c = idx % nb;
-        Li[2 * (r * LU_NB + c)] = Ir[r][c];
-        Li[2 * (r * LU_NB + c) + 1] = Ii[r][c];
+        inv[2 * (r * LU_NB + c)] = Ir[r][c];
+        inv[2 * (r * LU_NB + c) + 1] = Ii[r][c];
     }
     __syncthreads();
-    // Uinv: column j by thread j, back substitution
-    if (t < nb) {
-        const int j = t;
-        for (int r = nb - 1; r >= 0; r--) {
-            if (r > j) { Ir[r][j] = 0.0; Ii[r][j] = 0.0; continue; }
-            cplx s = {r == j ? 1.0 : 0.0, 0.0};
-            for (int q = r + 1; q <= j; q++) {
-                const cplx uq = cmul({Dr[r][q], Di[r][q]}, {Ir[q][j], Ii[q][j]});
-                s.r -= uq.r;
-                s.i -= uq.i;
-            }
-            const cplx v = cdiv(s, {Dr[r][r], Di[r][r]});
-            Ir[r][j] = v.r;
-            Ii[r][j] = v.i;
+    // Uinv by backward Gauss-Jordan on [U | I]
+    for (int idx = t; idx < LU_NB * LU_NB; idx += 256) {
+        const int r = idx / LU_NB, c = idx % LU_NB;
+        Ir[r][c] = (r == c) ? 1.0 : 0.0;
+        Ii[r][c] = 0.0;
+    }
+    __syncthreads();
+    for (int p = nb - 1; p >= 0; p--) {
+        const cplx d = {Dr[p][p], Di[p][p]};
+        for (int j = p + t; j < nb; j += 256) {
+            const cplx v = cdiv(cplx{Ir[p][j], Ii[p][j]}, d);
+            Ir[p][j] = v.r;
+            Ii[p][j] = v.i;
         }
+        __syncthreads();
+        const int w = nb - p;
+        for (int idx = t; idx < p * w; idx += 256) {
+            const int r = idx / w, j = p + idx % w;
+            const cplx ux = cmul(cplx{Dr[r][p], Di[r][p]}, cplx{Ir[p][j], Ii[p][j]});
+            Ir[r][j] -= ux.r;
+            Ii[r][j] -= ux.i;
+        }
+        __syncthreads();
     }
-    __syncthreads();
     double* Ui = inv + 2 * LU_NB * LU_NB;
     for (int idx = t; idx < nb * nb; idx += 256) {
         const int r = idx / nb, c = idx % nb;
@@ -273,28 +284,118 @@ __global__ __launch_bounds__(256) void k_gemm_update(int N, int nb, int k0, int 
         }
 }
 
-// ---- triangular solves with the stored inverse diagonal blocks ------------
-// In place: v_b <- Binv * v_b for one block of nb rows (split re/im vectors)
-__global__ void k_block_apply(int nb, const double* __restrict__ Binv, double* __restrict__ vr,
-                              double* __restrict__ vi, int lower)
+// trailing update on v_mfma_f64_16x16x4f64: C[r0:, r0:] -= A[r0:, k0:k0+nb] B[k0:k0+nb, r0:].
+// 64x64 complex tile per workgroup, 32x32 per wave = 2x2 blocks of 16x16 per plane;
+// Cr += Ar Br - Ai Bi and Ci += Ar Bi + Ai Br as four real MFMAs per block.  Lane
+// l supplies A[l%16][k = l/16] and B[k = l/16][l%16]; accumulator v of lane l is
+// C[4v + l/16][l%16].
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void k_gemm_f64mfma(int N, int nb, int k0, int r0,
+                                                      double* __restrict__ Zr,
+                                                      double* __restrict__ Zi, long ldZ)
 {
+    constexpr int BM = 64, BN = 64, BK = 16;
+    __shared__ double Asr[BM][BK + 1], Asi[BM][BK + 1];
+    __shared__ double Bsr[BK][BN + 1], Bsi[BK][BN + 1];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    const int m0 = r0 + blockIdx.y * BM, n0 = r0 + blockIdx.x * BN;
+    dbl4 cr[2][2], ci[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) cr[a][b][v] = ci[a][b][v] = 0.0;
+    const int li = lane & 15, lk = lane >> 4;
+    for (int kk = 0; kk < nb; kk += BK) {
+        for (int idx = t; idx < BM * BK; idx += 256) {
+            const int r = idx / BK, k = idx % BK, gi = m0 + r;
+            const bool ok = gi < N && (kk + k) < nb;
+            Asr[r][k] = ok ? Zr[(long)gi * ldZ + k0 + kk + k] : 0.0;
+            Asi[r][k] = ok ? Zi[(long)gi * ldZ + k0 + kk + k] : 0.0;
+        }
+        for (int idx = t; idx < BK * BN; idx += 256) {
+            const int k = idx / BN, c = idx % BN, gc = n0 + c;
+            const bool ok = gc < N && (kk + k) < nb;
+            Bsr[k][c] = ok ? Zr[(long)(k0 + kk + k) * ldZ + gc] : 0.0;
+            Bsi[k][c] = ok ? Zi[(long)(k0 + kk + k) * ldZ + gc] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < BK; ks += 4) {
+            double ar[2], ai[2], nai[2], br[2], bi[2];
+#pragma unroll
+            for (int a = 0; a < 2; a++) {
+                ar[a] = Asr[wm + 16 * a + li][ks + lk];
+                ai[a] = Asi[wm + 16 * a + li][ks + lk];
+                nai[a] = -ai[a];
+                br[a] = Bsr[ks + lk][wn + 16 * a + li];
+                bi[a] = Bsi[ks + lk][wn + 16 * a + li];
+            }
+#pragma unroll
+            for (int a = 0; a < 2; a++)
+#pragma unroll
+                for (int b = 0; b < 2; b++) {
+                    cr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], br[b], cr[a][b], 0, 0, 0);
+                    cr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai[a], bi[b], cr[a][b], 0, 0, 0);
+                    ci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], bi[b], ci[a][b], 0, 0, 0);
+                    ci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[a], br[b], ci[a][b], 0, 0, 0);
+                }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int gi = m0 + wm + 16 * a + 4 * v + lk, gc = n0 + wn + 16 * b + li;
+                if (gi < N && gc < N) {
+                    Zr[(long)gi * ldZ + gc] -= cr[a][b][v];
+                    Zi[(long)gi * ldZ + gc] -= ci[a][b][v];
+                }
+            }
+}
+
+// ---- triangular solves with the stored inverse diagonal blocks ------------
+// In place: v_b <- Binv * v_b for one block of nb rows (split re/im vectors);
+// Binv staged in LDS, 4 threads per row
+__global__ __launch_bounds__(256) void k_block_apply(int nb, const double* __restrict__ Binv,
+                                                     double* __restrict__ vr, double* __restrict__ vi,
+                                                     int lower)
+{
+    __shared__ double Br[LU_NB][LU_NB + 1], Bi[LU_NB][LU_NB + 1];
     __shared__ double xr[LU_NB], xi[LU_NB];
-    const int r = threadIdx.x;
-    if (r < nb) {
-        xr[r] = vr[r];
-        xi[r] = vi[r];
+    const int t = threadIdx.x;
+    for (int idx = t; idx < LU_NB * LU_NB; idx += 256) {
+        const int r = idx / LU_NB, c = idx % LU_NB;
+        Br[r][c] = Binv[2 * idx];
+        Bi[r][c] = Binv[2 * idx + 1];
+    }
+    if (t < nb) {
+        xr[t] = vr[t];
+        xi[t] = vi[t];
     }
     __syncthreads();
-    if (r >= nb) return;
+    const int r = t >> 2, part = t & 3;
     double sr = 0.0, si = 0.0;
-    const int q0 = lower ? 0 : r, q1 = lower ? r : nb - 1;
-    for (int q = q0; q <= q1; q++) {
-        const double br = Binv[2 * (r * LU_NB + q)], bi = Binv[2 * (r * LU_NB + q) + 1];
-        sr += br * xr[q] - bi * xi[q];
-        si += br * xi[q] + bi * xr[q];
+    if (r < nb) {
+        const int q0 = lower ? 0 : r, q1 = lower ? r : nb - 1;
+        for (int q = q0 + part; q <= q1; q += 4) {
+            sr += Br[r][q] * xr[q] - Bi[r][q] * xi[q];
+            si += Br[r][q] * xi[q] + Bi[r][q] * xr[q];
+        }
     }
-    vr[r] = sr;
-    vi[r] = si;
+    sr += __shfl_xor(sr, 1);
+    si += __shfl_xor(si, 1);
+    sr += __shfl_xor(sr, 2);
+    si += __shfl_xor(si, 2);
+    if (r < nb && part == 0) {
+        vr[r] = sr;
+        vi[r] = si;
+    }
 }
 
 // x[rows] -= Z[rows, k0:k0+nb] * x[k0:k0+nb]  for rows in [ra, rb) (split vectors)
@@ -371,8 +472,11 @@ hipError_t launch_lu_fp64(dep_sys* s, double* min_pivot)
     const int N = g->N;
     hipStream_t st = g->ctx->stream;
     const int nblk = dep_cdiv(N, LU_NB);
-    double* inv = (double*)s->opA;
+    double* inv = (double*)s->opA64;
     double* piv = s->dwork;
+    // DEPSOLVER_FP64_GEMM=valu selects the vector-FMA trailing update (for comparison)
+    const char* sel = getenv("DEPSOLVER_FP64_GEMM");
+    const bool use_mfma = !(sel && strcmp(sel, "valu") == 0);
     for (int kb = 0; kb < nblk; kb++) {
         const int k0 = kb * LU_NB, nb = (N - k0 < LU_NB) ? N - k0 : LU_NB;
         double* invb = inv + (size_t)kb * 4 * LU_NB * LU_NB;
@@ -387,8 +491,12 @@ hipError_t launch_lu_fp64(dep_sys* s, double* min_pivot)
             dim3 grid(dep_cdiv(N - r0, 64), dep_cdiv(N - r0, 64));
             hipEvent_t ev;
             prof_begin(g->ctx, &ev);
-            hipLaunchKernelGGL(k_gemm_update, grid, dim3(256), 0, st, N, nb, k0, r0, s->Zr, s->Zi,
-                               s->ldZ);
+            if (use_mfma)
+                hipLaunchKernelGGL(k_gemm_f64mfma, grid, dim3(256), 0, st, N, nb, k0, r0, s->Zr, s->Zi,
+                                   s->ldZ);
+            else
+                hipLaunchKernelGGL(k_gemm_update, grid, dim3(256), 0, st, N, nb, k0, r0, s->Zr, s->Zi,
+                                   s->ldZ);
             const double m = N - r0;
             prof_end(g->ctx, ev, "lu_gemm_fp64", 8.0 * m * m * nb, 32.0 * m * m + 32.0 * m * nb);
         }
@@ -413,7 +521,7 @@ hipError_t launch_trsv_fp64(dep_sys* s, const double* rhs, double* out)
     const int N = g->N;
     hipStream_t st = g->ctx->stream;
     const int nblk = dep_cdiv(N, LU_NB);
-    const double* inv = (const double*)s->opA;
+    const double* inv = (const double*)s->opA64;
     double* xr = out;
     double* xi = out + N;
     hipError_t e;
@@ -424,7 +532,7 @@ hipError_t launch_trsv_fp64(dep_sys* s, const double* rhs, double* out)
     for (int kb = 0; kb < nblk; kb++) {   // forward: L y = r (unit lower)
         const int k0 = kb * LU_NB, nb = (N - k0 < LU_NB) ? N - k0 : LU_NB;
         const double* Linv = inv + (size_t)kb * 4 * LU_NB * LU_NB;
-        hipLaunchKernelGGL(k_block_apply, dim3(1), dim3(LU_NB), 0, st, nb, Linv, xr + k0, xi + k0, 1);
+        hipLaunchKernelGGL(k_block_apply, dim3(1), dim3(256), 0, st, nb, Linv, xr + k0, xi + k0, 1);
         if (k0 + nb < N)
             hipLaunchKernelGGL(k_panel_gemv, dim3(dep_cdiv(N - k0 - nb, 128)), dim3(128), 0, st,
                                k0 + nb, N, nb, s->Zr, s->Zi, s->ldZ, k0, xr, xi);
@@ -432,7 +540,7 @@ hipError_t launch_trsv_fp64(dep_sys* s, const double* rhs, double* out)
     for (int kb = nblk - 1; kb >= 0; kb--) {   // backward: U x = y
         const int k0 = kb * LU_NB, nb = (N - k0 < LU_NB) ? N - k0 : LU_NB;
         const double* Uinv = inv + (size_t)kb * 4 * LU_NB * LU_NB + 2 * LU_NB * LU_NB;
-        hipLaunchKernelGGL(k_block_apply, dim3(1), dim3(LU_NB), 0, st, nb, Uinv, xr + k0, xi + k0, 0);
+        hipLaunchKernelGGL(k_block_apply, dim3(1), dim3(256), 0, st, nb, Uinv, xr + k0, xi + k0, 0);
         if (k0 > 0)
             hipLaunchKernelGGL(k_panel_gemv, dim3(dep_cdiv(k0, 128)), dim3(128), 0, st, 0, k0, nb,
                                s->Zr, s->Zi, s->ldZ, k0, xr, xi);
