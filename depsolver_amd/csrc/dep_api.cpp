@@ -944,11 +944,105 @@ extern "C" int dep_post_mst(dep_sys* s, double eps, double* Fce, double* Xce, do
     return r;
 }
 
-extern "C" int dep_post_multipole(dep_sys* s, int order, double radius, int m, const double* pts,
-                                  double* F)
+// Multipolar DEP force on a sphere of radius R at each point (force.c:572-825):
+// Re E at the point (field_*), the derivative integrals of orders 2..4 of the
+// potential from the device, then the reference's Clausius-Mossotti factors
+// and force sums (orders above 3 do not change F in the reference, so the
+// order is clamped to 3).
+extern "C" int dep_post_multipole(dep_sys* s, int order, double R, int m, const double* pts, double* F)
 {
-    (void)order; (void)radius; (void)m; (void)pts; (void)F;
-    return dep_fail(s ? s->g->ctx : nullptr, DEP_ERR_STATE, "dep_post_multipole: not available yet");
+    if (!s || m < 0 || order < 1 || (m > 0 && (!pts || !F))) return DEP_ERR_ARG;
+    if (m == 0) return DEP_OK;
+    dep_geom* g = s->g;
+    dep_ctx* c = g->ctx;
+    if (s->nM < 2) return dep_fail(c, DEP_ERR_ARG, "dep_post_multipole: needs materials 1 (medium) and 2 (particle)");
+    if (order > 3) order = 3;
+    std::vector<double> ef((size_t)m * 6), terms((size_t)m * 31);
+    DEP_CHECK(eval_points(s, 1, m, pts, ef.data()));   // also leaves s->qs current
+    double* dp = nullptr;
+    double* dt = nullptr;
+    struct Tmp {
+        dep_ctx* c;
+        std::vector<std::pair<void*, size_t>> allocs;
+        dep_ctx* ctx_of() { return c; }
+        ~Tmp() { for (auto& a : allocs) dep_free(c, a.first, a.second); }
+    } t{c, {}};
+    DEP_CHECK(dep_obj_alloc(&t, &dp, 3 * (size_t)m));
+    DEP_CHECK(dep_obj_alloc(&t, &dt, 31 * (size_t)m));
+    DEP_CHECK(upload(c, dp, pts, 3 * (size_t)m));
+    DEP_HIP(c, launch_multipole_terms(s, m, order, dp, dt));
+    DEP_CHECK(download(c, terms.data(), dt, terms.size()));
+    const double B = 1.0 / (DEP_PI4 * DEP_EPS0);
+    const double w = s->freq * DEP_PI2 * DEP_EPS0;
+    const double* mt = s->mats.data();   // (sigma, eps_r) of material 1 (medium) and 2 (particle)
+    const double Eps = mt[1] * DEP_EPS0;
+    double Cc[4] = {0.0, 0.0, 0.0, 0.0};
+    const double EpsDif = w * (mt[3] - mt[1]);
+    double EpsPlus = w * (mt[3] + 2.0 * mt[1]);
+    const double SigDif = mt[2] - mt[0];
+    double SigPlus = mt[2] + 2.0 * mt[0];
+    const double CM1 = (EpsDif * EpsPlus + SigDif * SigPlus) / (EpsPlus * EpsPlus + SigPlus * SigPlus);
+    Cc[1] = DEP_PI2 * Eps * R * R * R;
+    if (order > 1) {
+        EpsPlus = w * (2.0 * mt[3] + 3.0 * mt[1]);
+        SigPlus = 2.0 * mt[2] + 3.0 * mt[0];
+        const double CM2 = (EpsDif * EpsPlus + SigDif * SigPlus) / (EpsPlus * EpsPlus + SigPlus * SigPlus);
+        Cc[2] = Cc[1] * R * R / 3.0;
+        if (order > 2) {
+            EpsPlus = w * (3.0 * mt[3] + 4.0 * mt[1]);
+            SigPlus = 3.0 * mt[2] + 4.0 * mt[0];
+            const double CM3 = (EpsDif * EpsPlus + SigDif * SigPlus) / (EpsPlus * EpsPlus + SigPlus * SigPlus);
+            Cc[3] = Cc[2] * R * R / 10.0 * CM3;
+        }
+        Cc[2] = Cc[2] * CM2;
+    }
+    Cc[1] = Cc[1] * CM1;
+    for (int k = 0; k < m; k++) {
+        // the field-derivative cube E[p][q][r] of force.c (E[1][0][0] = Re Ex, ...)
+        const double* d = &terms[31 * (size_t)k];
+        double E3[7][7][7] = {};
+        E3[1][0][0] = ef[6 * k];
+        E3[0][1][0] = ef[6 * k + 2];
+        E3[0][0][1] = ef[6 * k + 4];
+        static const int s1[6][3] = {{2, 0, 0}, {1, 1, 0}, {1, 0, 1}, {0, 2, 0}, {0, 1, 1}, {0, 0, 2}};
+        static const int s2[10][3] = {{3, 0, 0}, {2, 1, 0}, {2, 0, 1}, {1, 2, 0}, {1, 1, 1},
+                                      {1, 0, 2}, {0, 3, 0}, {0, 2, 1}, {0, 1, 2}, {0, 0, 3}};
+        static const int s3[15][3] = {{4, 0, 0}, {3, 1, 0}, {3, 0, 1}, {2, 2, 0}, {2, 1, 1},
+                                      {2, 0, 2}, {1, 3, 0}, {1, 2, 1}, {1, 1, 2}, {1, 0, 3},
+                                      {0, 4, 0}, {0, 3, 1}, {0, 2, 2}, {0, 1, 3}, {0, 0, 4}};
+        for (int i = 0; i < 6; i++) E3[s1[i][0]][s1[i][1]][s1[i][2]] = B * d[i];
+        for (int i = 0; i < 10; i++) E3[s2[i][0]][s2[i][1]][s2[i][2]] = B * d[6 + i];
+        for (int i = 0; i < 15; i++) E3[s3[i][0]][s3[i][1]][s3[i][2]] = B * d[16 + i];
+#define EC(p, q, r) E3[p][q][r]
+        double* Fk = F + 3 * (size_t)k;
+        Fk[0] = Cc[1] * (EC(1,0,0) * EC(2,0,0) + EC(0,1,0) * EC(1,1,0) + EC(0,0,1) * EC(1,0,1));
+        Fk[1] = Cc[1] * (EC(1,0,0) * EC(1,1,0) + EC(0,1,0) * EC(0,2,0) + EC(0,0,1) * EC(0,1,1));
+        Fk[2] = Cc[1] * (EC(1,0,0) * EC(1,0,1) + EC(0,1,0) * EC(0,1,1) + EC(0,0,1) * EC(0,0,2));
+        if (order > 1) {
+            Fk[0] += Cc[2] * (EC(2,0,0) * EC(3,0,0) + EC(0,2,0) * EC(1,2,0) + EC(0,0,2) * EC(1,0,2) +
+                              2.0 * (EC(1,1,0) * EC(2,1,0) + EC(1,0,1) * EC(2,0,1) + EC(0,1,1) * EC(1,1,1)));
+            Fk[1] += Cc[2] * (EC(2,0,0) * EC(2,1,0) + EC(0,2,0) * EC(0,3,0) + EC(0,0,2) * EC(0,1,2) +
+                              2.0 * (EC(1,1,0) * EC(1,2,0) + EC(1,0,1) * EC(1,1,1) + EC(0,1,1) * EC(0,2,1)));
+            Fk[2] += Cc[2] * (EC(2,0,0) * EC(2,0,1) + EC(0,2,0) * EC(0,2,1) + EC(0,0,2) * EC(0,0,3) +
+                              2.0 * (EC(1,1,0) * EC(1,1,1) + EC(1,0,1) * EC(1,0,2) + EC(0,1,1) * EC(0,1,2)));
+            if (order > 2) {
+                Fk[0] += Cc[3] * (EC(3,0,0) * EC(4,0,0) + EC(0,3,0) * EC(1,3,0) + EC(0,0,3) * EC(1,0,3) +
+                                  3.0 * (EC(2,1,0) * EC(3,1,0) + EC(2,0,1) * EC(3,0,1) + EC(1,2,0) * EC(2,2,0) +
+                                         EC(1,0,2) * EC(2,0,2) + EC(0,2,1) * EC(1,2,1) + EC(0,1,2) * EC(1,1,2)) +
+                                  6.0 * EC(1,1,1) * EC(2,1,1));
+                Fk[1] += Cc[3] * (EC(3,0,0) * EC(3,1,0) + EC(0,3,0) * EC(0,4,0) + EC(0,0,3) * EC(0,1,3) +
+                                  3.0 * (EC(2,1,0) * EC(2,2,0) + EC(2,0,1) * EC(2,1,1) + EC(1,2,0) * EC(1,3,0) +
+                                         EC(1,0,2) * EC(1,1,2) + EC(0,2,1) * EC(0,3,1) + EC(0,1,2) * EC(0,2,2)) +
+                                  6.0 * EC(1,1,1) * EC(1,2,1));
+                Fk[2] += Cc[3] * (EC(3,0,0) * EC(3,0,1) + EC(0,3,0) * EC(0,3,1) + EC(0,0,3) * EC(0,0,4) +
+                                  3.0 * (EC(2,1,0) * EC(2,1,1) + EC(2,0,1) * EC(2,0,2) + EC(1,2,0) * EC(1,2,1) +
+                                         EC(1,0,2) * EC(1,0,3) + EC(0,2,1) * EC(0,2,2) + EC(0,1,2) * EC(0,1,3)) +
+                                  6.0 * EC(1,1,1) * EC(1,1,2));
+            }
+        }
+#undef EC
+    }
+    return DEP_OK;
 }
 
 // GMRES (src/gmres_el.c:66-354): unrestarted, Givens rotations on the host,

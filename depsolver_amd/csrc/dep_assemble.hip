@@ -134,13 +134,13 @@ __global__ __launch_bounds__(TR) void k_assemble_regular(
                 const double4 p = q[e * 7 + g];
                 const double dx = xe0 - p.x, dy = xe1 - p.y, dz = xe2 - p.z;
                 const double rsq = dx * dx + dy * dy + dz * dz;
-                const double r = sqrt(rsq);
+                const double ri = rsqrt_nr(rsq);
                 if (!iface) {
-                    const double A = p.w / r;
+                    const double A = p.w * ri;
 #pragma unroll
                     for (int k = 0; k < NPE; k++) sub[k] += A * SF.v[g][k];
                 } else {
-                    double A = p.w / (rsq * r);
+                    double A = p.w * (ri * ri * ri);
                     A = A * (dx * n0 + dy * n1 + dz * n2);
 #pragma unroll
                     for (int k = 0; k < NPE; k++) sub[k] -= A * SF.v[g][k];

@@ -22,9 +22,10 @@
 #include <algorithm>
 
 #include "dep_common.h"
+#include "dep_diag.h"
 #include "dep_internal.h"
 
-#define NBI 64     // inner block (diagonal LU in LDS, inverse blocks)
+#define NBI DIAG_NB  // inner block (diagonal LU in LDS, inverse blocks)
 #define NBO 256    // outer panel width = trailing-update depth
 
 namespace {
@@ -109,12 +110,13 @@ __global__ void k_build_w(int N, const double* __restrict__ P, long ldP, const d
 // ---- FP32 panel kernels -----------------------------------------------------
 // LU of the nb x nb diagonal block at (k0,k0) in LDS, then its inverses
 // Linv (unit lower) and Uinv (upper) into inv (2 x NBI x NBI complex)
-__global__ __launch_bounds__(256) void k_diag32(int nb, float* __restrict__ Zr, float* __restrict__ Zi,
-                                                long ldZ, int k0, float* __restrict__ inv,
-                                                float* __restrict__ piv)
+template <class T>
+__global__ __launch_bounds__(256) void k_diag(int nb, T* __restrict__ Zr, T* __restrict__ Zi,
+                                                long ldZ, int k0, T* __restrict__ inv,
+                                                T* __restrict__ piv)
 {
-    __shared__ float Dr[NBI][NBI + 1], Di[NBI][NBI + 1];
-    __shared__ float Ir[NBI][NBI + 1], Ii[NBI][NBI + 1];
+    __shared__ T Dr[NBI][NBI + 1], Di[NBI][NBI + 1];
+    __shared__ T Xr[NBI][NBI + 1], Xi[NBI][NBI + 1];
     const int t = threadIdx.x;
     for (int idx = t; idx < nb * nb; idx += 256) {
         const int r = idx / nb, c = idx % nb;
@@ -122,95 +124,22 @@ __global__ __launch_bounds__(256) void k_diag32(int nb, float* __restrict__ Zr, 
         Di[r][c] = Zi[(long)(k0 + r) * ldZ + k0 + c];
     }
     __syncthreads();
-    for (int p = 0; p < nb; p++) {
-        const cx<float> pv = {Dr[p][p], Di[p][p]};
-        for (int r = p + 1 + t; r < nb; r += 256) {
-            const cx<float> l = cdiv(cx<float>{Dr[r][p], Di[r][p]}, pv);
-            Dr[r][p] = l.r;
-            Di[r][p] = l.i;
-        }
-        __syncthreads();
-        const int m = nb - p - 1;
-        for (int idx = t; idx < m * m; idx += 256) {
-            const int r = p + 1 + idx / m, c = p + 1 + idx % m;
-            const cx<float> lu = cmul(cx<float>{Dr[r][p], Di[r][p]}, cx<float>{Dr[p][c], Di[p][c]});
-            Dr[r][c] -= lu.r;
-            Di[r][c] -= lu.i;
-        }
-        __syncthreads();
-    }
-    if (t == 0) {
-        float mn = 3.0e38f;
-        for (int p = 0; p < nb; p++) mn = fminf(mn, sqrtf(Dr[p][p] * Dr[p][p] + Di[p][p] * Di[p][p]));
-        piv[k0 / NBI] = mn;
-    }
+    diag_factor_invert<T>(nb, Dr, Di, Xr, Xi, inv, inv + 2 * NBI * NBI, piv + k0 / NBI);
     for (int idx = t; idx < nb * nb; idx += 256) {
         const int r = idx / nb, c = idx % nb;
         Zr[(long)(k0 + r) * ldZ + k0 + c] = Dr[r][c];
         Zi[(long)(k0 + r) * ldZ + k0 + c] = Di[r][c];
     }
-    // Linv by Gauss-Jordan on [L | I] (unit lower): X = I; X[r][:] -= L[r][p] X[p][:] for r > p
-    for (int idx = t; idx < NBI * NBI; idx += 256) {
-        const int r = idx / NBI, c = idx % NBI;
-        Ir[r][c] = (r == c) ? 1.0f : 0.0f;
-        Ii[r][c] = 0.0f;
-    }
-    __syncthreads();
-    for (int p = 0; p < nb - 1; p++) {
-        const int w = p + 1, m = nb - 1 - p;
-        for (int idx = t; idx < m * w; idx += 256) {
-            const int r = p + 1 + idx / w, j = idx % w;
-            const cx<float> lx = cmul(cx<float>{Dr[r][p], Di[r][p]}, cx<float>{Ir[p][j], Ii[p][j]});
-            Ir[r][j] -= lx.r;
-            Ii[r][j] -= lx.i;
-        }
-        __syncthreads();
-    }
-    for (int idx = t; idx < nb * nb; idx += 256) {
-        const int r = idx / nb, c = idx % nb;
-        inv[2 * (r * NBI + c)] = Ir[r][c];
-        inv[2 * (r * NBI + c) + 1] = Ii[r][c];
-    }
-    __syncthreads();
-    // Uinv by backward Gauss-Jordan on [U | I]
-    for (int idx = t; idx < NBI * NBI; idx += 256) {
-        const int r = idx / NBI, c = idx % NBI;
-        Ir[r][c] = (r == c) ? 1.0f : 0.0f;
-        Ii[r][c] = 0.0f;
-    }
-    __syncthreads();
-    for (int p = nb - 1; p >= 0; p--) {
-        const cx<float> d = {Dr[p][p], Di[p][p]};
-        for (int j = p + t; j < nb; j += 256) {
-            const cx<float> v = cdiv(cx<float>{Ir[p][j], Ii[p][j]}, d);
-            Ir[p][j] = v.r;
-            Ii[p][j] = v.i;
-        }
-        __syncthreads();
-        const int w = nb - p;
-        for (int idx = t; idx < p * w; idx += 256) {
-            const int r = idx / w, j = p + idx % w;
-            const cx<float> ux = cmul(cx<float>{Dr[r][p], Di[r][p]}, cx<float>{Ir[p][j], Ii[p][j]});
-            Ir[r][j] -= ux.r;
-            Ii[r][j] -= ux.i;
-        }
-        __syncthreads();
-    }
-    float* Ui = inv + 2 * NBI * NBI;
-    for (int idx = t; idx < nb * nb; idx += 256) {
-        const int r = idx / nb, c = idx % nb;
-        Ui[2 * (r * NBI + c)] = Ir[r][c];
-        Ui[2 * (r * NBI + c) + 1] = Ii[r][c];
-    }
 }
 
 // rows [r0, N): W[i, k0:k0+nb] <- W[i, k0:k0+nb] Uinv   (16 rows per block)
-__global__ __launch_bounds__(256) void k_uinv32(int N, int nb, int r0, float* __restrict__ Zr,
-                                                float* __restrict__ Zi, long ldZ, int k0,
-                                                const float* __restrict__ Uinv)
+template <class T>
+__global__ __launch_bounds__(256) void k_uinv(int N, int nb, int r0, T* __restrict__ Zr,
+                                                T* __restrict__ Zi, long ldZ, int k0,
+                                                const T* __restrict__ Uinv)
 {
-    __shared__ float Ar[16][NBI], Ai[16][NBI];
-    __shared__ float Ur[NBI][NBI + 1], Ui[NBI][NBI + 1];
+    __shared__ T Ar[16][NBI], Ai[16][NBI];
+    __shared__ T Ur[NBI][NBI + 1], Ui[NBI][NBI + 1];
     const int t = threadIdx.x;
     const int rb = r0 + blockIdx.x * 16;
     for (int idx = t; idx < nb * nb; idx += 256) {
@@ -220,13 +149,13 @@ __global__ __launch_bounds__(256) void k_uinv32(int N, int nb, int r0, float* __
     }
     for (int idx = t; idx < 16 * nb; idx += 256) {
         const int r = idx / nb, c = idx % nb, gi = rb + r;
-        Ar[r][c] = gi < N ? Zr[(long)gi * ldZ + k0 + c] : 0.0f;
-        Ai[r][c] = gi < N ? Zi[(long)gi * ldZ + k0 + c] : 0.0f;
+        Ar[r][c] = gi < N ? Zr[(long)gi * ldZ + k0 + c] : T(0);
+        Ai[r][c] = gi < N ? Zi[(long)gi * ldZ + k0 + c] : T(0);
     }
     __syncthreads();
     for (int idx = t; idx < 16 * nb; idx += 256) {
         const int r = idx / nb, c = idx % nb, gi = rb + r;
-        float sr = 0.0f, si = 0.0f;
+        T sr = T(0), si = T(0);
         for (int q = 0; q <= c; q++) {
             sr += Ar[r][q] * Ur[q][c] - Ai[r][q] * Ui[q][c];
             si += Ar[r][q] * Ui[q][c] + Ai[r][q] * Ur[q][c];
@@ -239,12 +168,13 @@ __global__ __launch_bounds__(256) void k_uinv32(int N, int nb, int r0, float* __
 }
 
 // columns [c0, c1): W[k0:k0+nb, j] <- Linv W[k0:k0+nb, j]   (16 columns per block)
-__global__ __launch_bounds__(256) void k_linv32(int c1, int nb, int c0, float* __restrict__ Zr,
-                                                float* __restrict__ Zi, long ldZ, int k0,
-                                                const float* __restrict__ Linv)
+template <class T>
+__global__ __launch_bounds__(256) void k_linv(int c1, int nb, int c0, T* __restrict__ Zr,
+                                                T* __restrict__ Zi, long ldZ, int k0,
+                                                const T* __restrict__ Linv)
 {
-    __shared__ float Br[NBI][16], Bi[NBI][16];
-    __shared__ float Lr[NBI][NBI + 1], Li[NBI][NBI + 1];
+    __shared__ T Br[NBI][16], Bi[NBI][16];
+    __shared__ T Lr[NBI][NBI + 1], Li[NBI][NBI + 1];
     const int t = threadIdx.x;
     const int cb = c0 + blockIdx.x * 16;
     for (int idx = t; idx < nb * nb; idx += 256) {
@@ -254,13 +184,13 @@ __global__ __launch_bounds__(256) void k_linv32(int c1, int nb, int c0, float* _
     }
     for (int idx = t; idx < 16 * nb; idx += 256) {
         const int r = idx / 16, c = idx % 16, gc = cb + c;
-        Br[r][c] = gc < c1 ? Zr[(long)(k0 + r) * ldZ + gc] : 0.0f;
-        Bi[r][c] = gc < c1 ? Zi[(long)(k0 + r) * ldZ + gc] : 0.0f;
+        Br[r][c] = gc < c1 ? Zr[(long)(k0 + r) * ldZ + gc] : T(0);
+        Bi[r][c] = gc < c1 ? Zi[(long)(k0 + r) * ldZ + gc] : T(0);
     }
     __syncthreads();
     for (int idx = t; idx < 16 * nb; idx += 256) {
         const int r = idx / 16, c = idx % 16, gc = cb + c;
-        float sr = 0.0f, si = 0.0f;
+        T sr = T(0), si = T(0);
         for (int q = 0; q <= r; q++) {
             sr += Lr[r][q] * Br[q][c] - Li[r][q] * Bi[q][c];
             si += Lr[r][q] * Bi[q][c] + Li[r][q] * Br[q][c];
@@ -273,37 +203,38 @@ __global__ __launch_bounds__(256) void k_linv32(int c1, int nb, int c0, float* _
 }
 
 // W[ra:rb, ca:cb] -= W[ra:rb, kc:kc+nbk] W[kc:kc+nbk, ca:cb]  (FP32, 64x64 tiles)
-__global__ __launch_bounds__(256) void k_gemm32(int ra, int rb, int ca, int cb, int kc, int nbk,
-                                                float* __restrict__ Zr, float* __restrict__ Zi, long ldZ)
+template <class T>
+__global__ __launch_bounds__(256) void k_gemmT(int ra, int rb, int ca, int cb, int kc, int nbk,
+                                                T* __restrict__ Zr, T* __restrict__ Zi, long ldZ)
 {
     constexpr int BM = 64, BN = 64, BK = 16;
-    __shared__ float Asr[BK][BM + 1], Asi[BK][BM + 1];
-    __shared__ float Bsr[BK][BN + 1], Bsi[BK][BN + 1];
+    __shared__ T Asr[BK][BM + 1], Asi[BK][BM + 1];
+    __shared__ T Bsr[BK][BN + 1], Bsi[BK][BN + 1];
     const int t = threadIdx.x;
     const int tm = t / 16, tn = t % 16;
     const int m0 = ra + blockIdx.y * BM, n0 = ca + blockIdx.x * BN;
-    float cr[4][4], ci[4][4];
+    T cr[4][4], ci[4][4];
 #pragma unroll
     for (int a = 0; a < 4; a++)
 #pragma unroll
-        for (int b = 0; b < 4; b++) cr[a][b] = ci[a][b] = 0.0f;
+        for (int b = 0; b < 4; b++) cr[a][b] = ci[a][b] = T(0);
     for (int kk = 0; kk < nbk; kk += BK) {
         for (int idx = t; idx < BM * BK; idx += 256) {
             const int r = idx / BK, k = idx % BK, gi = m0 + r;
             const bool ok = gi < rb && (kk + k) < nbk;
-            Asr[k][r] = ok ? Zr[(long)gi * ldZ + kc + kk + k] : 0.0f;
-            Asi[k][r] = ok ? Zi[(long)gi * ldZ + kc + kk + k] : 0.0f;
+            Asr[k][r] = ok ? Zr[(long)gi * ldZ + kc + kk + k] : T(0);
+            Asi[k][r] = ok ? Zi[(long)gi * ldZ + kc + kk + k] : T(0);
         }
         for (int idx = t; idx < BK * BN; idx += 256) {
             const int k = idx / BN, c = idx % BN, gc = n0 + c;
             const bool ok = gc < cb && (kk + k) < nbk;
-            Bsr[k][c] = ok ? Zr[(long)(kc + kk + k) * ldZ + gc] : 0.0f;
-            Bsi[k][c] = ok ? Zi[(long)(kc + kk + k) * ldZ + gc] : 0.0f;
+            Bsr[k][c] = ok ? Zr[(long)(kc + kk + k) * ldZ + gc] : T(0);
+            Bsi[k][c] = ok ? Zi[(long)(kc + kk + k) * ldZ + gc] : T(0);
         }
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < BK; k++) {
-            float ar[4], ai[4], br[4], bi[4];
+            T ar[4], ai[4], br[4], bi[4];
 #pragma unroll
             for (int a = 0; a < 4; a++) {
                 ar[a] = Asr[k][tm + 16 * a];
@@ -331,6 +262,81 @@ __global__ __launch_bounds__(256) void k_gemm32(int ra, int rb, int ca, int cb, 
                 Zi[(long)gi * ldZ + gc] -= ci[a][b];
             }
         }
+}
+
+// FP64 trailing update on v_mfma_f64_16x16x4f64:
+// C[ra:rb, ca:cb] -= A[ra:rb, kc:kc+nbk] B[kc:kc+nbk, ca:cb], 64x64 complex tile per
+// workgroup, 32x32 per wave = 2x2 blocks of 16x16 per plane (Cr += Ar Br - Ai Bi,
+// Ci += Ar Bi + Ai Br as four real MFMAs).  Lane l supplies A[l%16][k = l/16] and
+// B[k = l/16][l%16]; accumulator v of lane l is C[4v + l/16][l%16].
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void k_gemm_f64mfma(int ra, int rb, int ca, int cb, int kc, int nbk,
+                                                      double* __restrict__ Zr,
+                                                      double* __restrict__ Zi, long ldZ)
+{
+    constexpr int BM = 64, BN = 64, BK = 16;
+    __shared__ double Asr[BM][BK + 1], Asi[BM][BK + 1];
+    __shared__ double Bsr[BK][BN + 1], Bsi[BK][BN + 1];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    const int m0 = ra + blockIdx.y * BM, n0 = ca + blockIdx.x * BN;
+    dbl4 cr[2][2], ci[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) cr[a][b][v] = ci[a][b][v] = 0.0;
+    const int li = lane & 15, lk = lane >> 4;
+    for (int kk = 0; kk < nbk; kk += BK) {
+        for (int idx = t; idx < BM * BK; idx += 256) {
+            const int r = idx / BK, k = idx % BK, gi = m0 + r;
+            const bool ok = gi < rb && (kk + k) < nbk;
+            Asr[r][k] = ok ? Zr[(long)gi * ldZ + kc + kk + k] : 0.0;
+            Asi[r][k] = ok ? Zi[(long)gi * ldZ + kc + kk + k] : 0.0;
+        }
+        for (int idx = t; idx < BK * BN; idx += 256) {
+            const int k = idx / BN, c = idx % BN, gc = n0 + c;
+            const bool ok = gc < cb && (kk + k) < nbk;
+            Bsr[k][c] = ok ? Zr[(long)(kc + kk + k) * ldZ + gc] : 0.0;
+            Bsi[k][c] = ok ? Zi[(long)(kc + kk + k) * ldZ + gc] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < BK; ks += 4) {
+            double ar[2], ai[2], nai[2], br[2], bi[2];
+#pragma unroll
+            for (int a = 0; a < 2; a++) {
+                ar[a] = Asr[wm + 16 * a + li][ks + lk];
+                ai[a] = Asi[wm + 16 * a + li][ks + lk];
+                nai[a] = -ai[a];
+                br[a] = Bsr[ks + lk][wn + 16 * a + li];
+                bi[a] = Bsi[ks + lk][wn + 16 * a + li];
+            }
+#pragma unroll
+            for (int a = 0; a < 2; a++)
+#pragma unroll
+                for (int b = 0; b < 2; b++) {
+                    cr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], br[b], cr[a][b], 0, 0, 0);
+                    cr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai[a], bi[b], cr[a][b], 0, 0, 0);
+                    ci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], bi[b], ci[a][b], 0, 0, 0);
+                    ci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[a], br[b], ci[a][b], 0, 0, 0);
+                }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int gi = m0 + wm + 16 * a + 4 * v + lk, gc = n0 + wn + 16 * b + li;
+                if (gi < rb && gc < cb) {
+                    Zr[(long)gi * ldZ + gc] -= cr[a][b][v];
+                    Zi[(long)gi * ldZ + gc] -= ci[a][b][v];
+                }
+            }
 }
 
 // ---- low-precision operands and the MFMA trailing update --------------------
@@ -469,74 +475,66 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(int M, int NC, int NCp, int K
             }
 }
 
-// Larger-tile variant: 256 (M) x 128 (N) per workgroup, 4 waves of 128 x 64
-// (4 x 2 MFMA blocks), the next 32-deep K chunk prefetched into registers while
-// the current one is consumed from LDS.
+// Pipelined variant: 128x128 tile per workgroup (4 waves of 64x64), two LDS
+// buffers; the next 32-deep K chunk is loaded into registers while the
+// current buffer feeds the MFMAs, then stored into the other buffer, so each
+// chunk costs one barrier.
 template <class H, class V8>
-__global__ __launch_bounds__(256) void k_gemm_mfma2(int M, int NC, int NCp, int K2p, int r0,
+__global__ __launch_bounds__(256) void k_gemm_mfma3(int M, int NC, int NCp, int K2p, int r0,
                                                     const H* __restrict__ A, const H* __restrict__ BT,
                                                     float* __restrict__ Zr, float* __restrict__ Zi,
                                                     long ldZ)
 {
     constexpr int TK = 32, LDA = TK + 8;
-    __shared__ H As[256][LDA], Bs[128][LDA];
+    __shared__ H As[2][128][LDA], Bs[2][128][LDA];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const int wm = (wave >> 1) * 128, wn = (wave & 1) * 64;
-    const int m0 = blockIdx.y * 256;
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+    const int m0 = blockIdx.y * 128;
     const int n0 = blockIdx.x * 128;        // in [0, 2*NCp)
-    f32x16 acc[4][2];
+    f32x16 acc[2][2];
 #pragma unroll
-    for (int a = 0; a < 4; a++)
+    for (int a = 0; a < 2; a++)
 #pragma unroll
         for (int b = 0; b < 2; b++)
 #pragma unroll
             for (int v = 0; v < 16; v++) acc[a][b][v] = 0.0f;
-    // per thread: A chunks ch = t + 256h (h < 4): row ch>>2, k (ch&3)*8; B chunks h < 2
-    V8 ra[4], rb[2];
+    V8 ra[2], rb[2];
+    const int r_lo = t >> 2, kc = (t & 3) * 8;    // chunks t and t + 256: rows r_lo and r_lo + 64
     auto gload = [&](int k0) {
 #pragma unroll
-        for (int h = 0; h < 4; h++) {
-            const int ch = t + 256 * h, r = ch >> 2, kc = (ch & 3) * 8, gm = m0 + r;
+        for (int h = 0; h < 2; h++) {
+            const int r = r_lo + 64 * h, gm = m0 + r;
             if (gm < M) ra[h] = *(const V8*)(A + (long)gm * K2p + k0 + kc);
             else
 #pragma unroll
                 for (int j = 0; j < 8; j++) ra[h][j] = to_h<H>(0.0f);
-        }
-#pragma unroll
-        for (int h = 0; h < 2; h++) {
-            const int ch = t + 256 * h, r = ch >> 2, kc = (ch & 3) * 8;
             rb[h] = *(const V8*)(BT + (long)(n0 + r) * K2p + k0 + kc);
         }
     };
-    auto lstore = [&]() {
-#pragma unroll
-        for (int h = 0; h < 4; h++) {
-            const int ch = t + 256 * h;
-            *(V8*)(&As[ch >> 2][(ch & 3) * 8]) = ra[h];
-        }
+    auto lstore = [&](int buf) {
 #pragma unroll
         for (int h = 0; h < 2; h++) {
-            const int ch = t + 256 * h;
-            *(V8*)(&Bs[ch >> 2][(ch & 3) * 8]) = rb[h];
+            *(V8*)(&As[buf][r_lo + 64 * h][kc]) = ra[h];
+            *(V8*)(&Bs[buf][r_lo + 64 * h][kc]) = rb[h];
         }
     };
     gload(0);
-    lstore();
+    lstore(0);
     __syncthreads();
+    int buf = 0;
     for (int k0 = 0; k0 < K2p; k0 += TK) {
         const bool more = k0 + TK < K2p;
         if (more) gload(k0 + TK);
 #pragma unroll
         for (int ks = 0; ks < TK; ks += 16) {
-            V8 fa[4], fb[2];
+            V8 fa[2], fb[2];
 #pragma unroll
-            for (int a = 0; a < 4; a++)
-                fa[a] = *(const V8*)(&As[wm + 32 * a + (lane & 31)][ks + 8 * (lane >> 5)]);
+            for (int a = 0; a < 2; a++) {
+                fa[a] = *(const V8*)(&As[buf][wm + 32 * a + (lane & 31)][ks + 8 * (lane >> 5)]);
+                fb[a] = *(const V8*)(&Bs[buf][wn + 32 * a + (lane & 31)][ks + 8 * (lane >> 5)]);
+            }
 #pragma unroll
-            for (int b = 0; b < 2; b++)
-                fb[b] = *(const V8*)(&Bs[wn + 32 * b + (lane & 31)][ks + 8 * (lane >> 5)]);
-#pragma unroll
-            for (int a = 0; a < 4; a++)
+            for (int a = 0; a < 2; a++)
 #pragma unroll
                 for (int b = 0; b < 2; b++) {
                     if constexpr (__is_same(H, _Float16))
@@ -545,17 +543,15 @@ __global__ __launch_bounds__(256) void k_gemm_mfma2(int M, int NC, int NCp, int 
                         acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[a], fb[b], acc[a][b], 0, 0, 0);
                 }
         }
+        if (more) lstore(buf ^ 1);
         __syncthreads();
-        if (more) {
-            lstore();
-            __syncthreads();
-        }
+        buf ^= 1;
     }
     const bool imag = n0 >= NCp;
     float* C = imag ? Zi : Zr;
     const int nb0 = imag ? n0 - NCp : n0;
 #pragma unroll
-    for (int a = 0; a < 4; a++)
+    for (int a = 0; a < 2; a++)
 #pragma unroll
         for (int b = 0; b < 2; b++)
 #pragma unroll
@@ -567,11 +563,12 @@ __global__ __launch_bounds__(256) void k_gemm_mfma2(int M, int NC, int NCp, int 
 }
 
 // ---- triangular solves with FP32 factors, FP64 vectors ---------------------
-__global__ __launch_bounds__(256) void k_block_apply32(int nb, const float* __restrict__ Binv,
+template <class T>
+__global__ __launch_bounds__(256) void k_block_applyT(int nb, const T* __restrict__ Binv,
                                                        double* __restrict__ vr, double* __restrict__ vi,
                                                        int lower)
 {
-    __shared__ float Br[NBI][NBI + 1], Bi[NBI][NBI + 1];
+    __shared__ T Br[NBI][NBI + 1], Bi[NBI][NBI + 1];
     __shared__ double xr[NBI], xi[NBI];
     const int t = threadIdx.x;
     for (int idx = t; idx < NBI * NBI; idx += 256) {
@@ -605,8 +602,9 @@ __global__ __launch_bounds__(256) void k_block_apply32(int nb, const float* __re
 }
 
 // x[rows] -= W[rows, k0:k0+nb] x[k0:k0+nb]  for rows in [ra, rb): one wave per 4 rows
-__global__ __launch_bounds__(256) void k_panel_gemv32(int ra, int rb, int nb, const float* __restrict__ Zr,
-                                                      const float* __restrict__ Zi, long ldZ, int k0,
+template <class T>
+__global__ __launch_bounds__(256) void k_panel_gemvT(int ra, int rb, int nb, const T* __restrict__ Zr,
+                                                      const T* __restrict__ Zi, long ldZ, int k0,
                                                       double* __restrict__ xr, double* __restrict__ xi)
 {
     const int lane = threadIdx.x & 63;
@@ -638,6 +636,12 @@ __global__ void k_vscale(int N, const double* __restrict__ s, const double* __re
     out[N + i] = s[i] * in[N + i];
 }
 
+__global__ void k_copy(long n, const double* __restrict__ in, double* __restrict__ out)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = in[i];
+}
+
 __global__ void k_axpy(long n, const double* __restrict__ d, double* __restrict__ x)
 {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -646,63 +650,36 @@ __global__ void k_axpy(long n, const double* __restrict__ d, double* __restrict_
 
 }  // namespace
 
-// Mixed storage in dep_sys: Zr32/Zi32 (N x ldZ FP32), opA = FP32 inverse blocks
-// (nblk x 2 x 64 x 64 complex), opB = packed operands (A' then B'^T), rs/cs.
-size_t mixed_bytes(int N)
-{
-    const size_t ld = ((size_t)N + 127) / 128 * 128;
-    const size_t nblk = ((size_t)N + NBI - 1) / NBI;
-    const size_t NCp = ld;
-    return 2 * (size_t)N * ld * 4 + nblk * 4 * NBI * NBI * 4 + (size_t)N * 2 * NBO * 2 +
-           2 * NCp * 2 * NBO * 2;
-}
-
-hipError_t launch_lu_mixed(dep_sys* s, int bf16, double* min_pivot)
+// Blocked right-looking LU of the working matrix W (N x ld split planes of T):
+// outer panels of NBO columns, inner NBI blocks; inv receives the inverse
+// diagonal blocks (nblk x 2 x NBI x NBI complex T); piv (T) one entry per block.
+// mode 0: FP64 trailing update in place (T = double); 1/2: FP16/BF16 MFMA on
+// packed operands (T = float).
+template <class T>
+static hipError_t lu_blocked(dep_sys* s, T* Wr, T* Wi, long ld, T* inv, T* piv, int mode)
 {
     dep_geom* g = s->g;
     dep_ctx* c = g->ctx;
     hipStream_t st = c->stream;
     const int N = g->N;
-    const long ld = s->ld32;
-    float* Wr = s->Zr32;
-    float* Wi = s->Zi32;
-    float* inv = (float*)s->opA;
-    float* piv = (float*)s->dwork;
-    const int NCmax = (int)ld;
-    _Float16* Ah = (_Float16*)s->opB;
-    // DEPSOLVER_MIXED_GEMM=v1 selects the 128x128-tile trailing update (for comparison)
+    hipError_t e;
     const char* gsel = getenv("DEPSOLVER_MIXED_GEMM");
     const bool gemm_v1 = gsel && strcmp(gsel, "v1") == 0;
-    _Float16* Bh = Ah + (size_t)N * 2 * NBO;
-    hipError_t e;
-    // 1. equilibration and W
-    hipLaunchKernelGGL(k_rowmax, dim3(dep_cdiv(N, 4)), dim3(256), 0, st, N, g->P, g->ldP, s->wr, s->wi,
-                       s->shift, (double*)s->rs);
-    e = hipMemsetAsync(s->cs, 0, sizeof(double) * N, st);
-    if (e != hipSuccess) return e;
-    const int rows_per = 512;
-    hipLaunchKernelGGL(k_colmax, dim3(dep_cdiv(N, 256), dep_cdiv(N, rows_per)), dim3(256), 0, st, N,
-                       rows_per, g->P, g->ldP, s->wr, s->wi, s->shift, (const double*)s->rs,
-                       (unsigned long long*)s->cs);
-    hipLaunchKernelGGL(k_colscale, dim3(dep_cdiv(N, 256)), dim3(256), 0, st, N,
-                       (const unsigned long long*)s->cs, (double*)s->cs);
-    hipLaunchKernelGGL(k_build_w, dim3(dep_cdiv(N, 1024), N), dim3(256), 0, st, N, g->P, g->ldP, s->wr,
-                       s->wi, s->shift, (const double*)s->rs, (const double*)s->cs, Wr, Wi, ld);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    // 2-3. blocked LU
+    const char* fsel = getenv("DEPSOLVER_FP64_GEMM");
+    const bool f64_valu = fsel && strcmp(fsel, "valu") == 0;
     for (int k0 = 0; k0 < N; k0 += NBO) {
         const int nbo = std::min(NBO, N - k0), pe = k0 + nbo;
         for (int kj = k0; kj < pe; kj += NBI) {
             const int nbi = std::min(NBI, pe - kj);
-            float* invb = inv + (size_t)(kj / NBI) * 4 * NBI * NBI;
-            hipLaunchKernelGGL(k_diag32, dim3(1), dim3(256), 0, st, nbi, Wr, Wi, ld, kj, invb, piv);
+            T* invb = inv + (size_t)(kj / NBI) * 4 * NBI * NBI;
+            hipLaunchKernelGGL(k_diag<T>, dim3(1), dim3(256), 0, st, nbi, Wr, Wi, ld, kj, invb, piv);
             if (kj + nbi < N)
-                hipLaunchKernelGGL(k_uinv32, dim3(dep_cdiv(N - kj - nbi, 16)), dim3(256), 0, st, N, nbi,
+                hipLaunchKernelGGL(k_uinv<T>, dim3(dep_cdiv(N - kj - nbi, 16)), dim3(256), 0, st, N, nbi,
                                    kj + nbi, Wr, Wi, ld, kj, invb + 2 * NBI * NBI);
             if (kj + nbi < pe) {
-                hipLaunchKernelGGL(k_linv32, dim3(dep_cdiv(pe - kj - nbi, 16)), dim3(256), 0, st, pe, nbi,
+                hipLaunchKernelGGL(k_linv<T>, dim3(dep_cdiv(pe - kj - nbi, 16)), dim3(256), 0, st, pe, nbi,
                                    kj + nbi, Wr, Wi, ld, kj, invb);
-                hipLaunchKernelGGL(k_gemm32, dim3(dep_cdiv(pe - kj - nbi, 64), dep_cdiv(N - kj - nbi, 64)),
+                hipLaunchKernelGGL(k_gemmT<T>, dim3(dep_cdiv(pe - kj - nbi, 64), dep_cdiv(N - kj - nbi, 64)),
                                    dim3(256), 0, st, kj + nbi, N, kj + nbi, pe, kj, nbi, Wr, Wi, ld);
             }
         }
@@ -710,92 +687,161 @@ hipError_t launch_lu_mixed(dep_sys* s, int bf16, double* min_pivot)
         // U12 = L11^-1 A12 by block forward substitution
         for (int kj = k0; kj < pe; kj += NBI) {
             const int nbi = std::min(NBI, pe - kj);
-            const float* invb = inv + (size_t)(kj / NBI) * 4 * NBI * NBI;
-            hipLaunchKernelGGL(k_linv32, dim3(dep_cdiv(N - pe, 16)), dim3(256), 0, st, N, nbi, pe, Wr, Wi,
+            const T* invb = inv + (size_t)(kj / NBI) * 4 * NBI * NBI;
+            hipLaunchKernelGGL(k_linv<T>, dim3(dep_cdiv(N - pe, 16)), dim3(256), 0, st, N, nbi, pe, Wr, Wi,
                                ld, kj, invb);
             if (kj + nbi < pe)
-                hipLaunchKernelGGL(k_gemm32, dim3(dep_cdiv(N - pe, 64), dep_cdiv(pe - kj - nbi, 64)),
+                hipLaunchKernelGGL(k_gemmT<T>, dim3(dep_cdiv(N - pe, 64), dep_cdiv(pe - kj - nbi, 64)),
                                    dim3(256), 0, st, kj + nbi, pe, pe, N, kj, nbi, Wr, Wi, ld);
         }
-        // low-precision operands and the trailing update
-        const int M = N - pe, NC = N - pe, NCp = (NC + 127) / 128 * 128;
-        const int K2p = (2 * nbo + 31) / 32 * 32;
-        (void)NCmax;
-        if (bf16) {
-            hipLaunchKernelGGL(k_pack_a<__bf16>, dim3(dep_cdiv(K2p, 256), M), dim3(256), 0, st, M, nbo, K2p,
-                               pe, k0, Wr, Wi, ld, (__bf16*)Ah);
-            hipLaunchKernelGGL(k_pack_b<__bf16>, dim3(dep_cdiv(NCp, 64)), dim3(256), 0, st, NC, NCp, nbo,
-                               K2p, k0, pe, Wr, Wi, ld, (__bf16*)Bh);
-        } else {
-            hipLaunchKernelGGL(k_pack_a<_Float16>, dim3(dep_cdiv(K2p, 256), M), dim3(256), 0, st, M, nbo,
-                               K2p, pe, k0, Wr, Wi, ld, Ah);
-            hipLaunchKernelGGL(k_pack_b<_Float16>, dim3(dep_cdiv(NCp, 64)), dim3(256), 0, st, NC, NCp, nbo,
-                               K2p, k0, pe, Wr, Wi, ld, Bh);
-        }
+        const int M = N - pe, NC = N - pe;
         hipEvent_t ev;
-        prof_begin(c, &ev);
-        if (gemm_v1) {
-            dim3 grid(2 * NCp / 128, dep_cdiv(M, 128));
-            if (bf16)
-                hipLaunchKernelGGL((k_gemm_mfma<__bf16, bf16x8>), grid, dim3(256), 0, st, M, NC, NCp, K2p, pe,
-                                   (const __bf16*)Ah, (const __bf16*)Bh, Wr, Wi, ld);
+        if (mode == 0) {
+            prof_begin(c, &ev);
+            dim3 grid(dep_cdiv(NC, 64), dep_cdiv(M, 64));
+            if (f64_valu)
+                hipLaunchKernelGGL(k_gemmT<T>, grid, dim3(256), 0, st, pe, N, pe, N, k0, nbo, Wr, Wi, ld);
             else
-                hipLaunchKernelGGL((k_gemm_mfma<_Float16, f16x8>), grid, dim3(256), 0, st, M, NC, NCp, K2p,
-                                   pe, (const _Float16*)Ah, (const _Float16*)Bh, Wr, Wi, ld);
+                hipLaunchKernelGGL(k_gemm_f64mfma, grid, dim3(256), 0, st, pe, N, pe, N, k0, nbo,
+                                   (double*)Wr, (double*)Wi, ld);
+            prof_end(c, ev, "lu_gemm_fp64", 8.0 * M * (double)NC * nbo, 32.0 * M * (double)NC + 32.0 * nbo * (M + NC));
         } else {
-            dim3 grid(2 * NCp / 128, dep_cdiv(M, 256));
-            if (bf16)
-                hipLaunchKernelGGL((k_gemm_mfma2<__bf16, bf16x8>), grid, dim3(256), 0, st, M, NC, NCp, K2p,
-                                   pe, (const __bf16*)Ah, (const __bf16*)Bh, Wr, Wi, ld);
-            else
-                hipLaunchKernelGGL((k_gemm_mfma2<_Float16, f16x8>), grid, dim3(256), 0, st, M, NC, NCp, K2p,
-                                   pe, (const _Float16*)Ah, (const _Float16*)Bh, Wr, Wi, ld);
+            // low-precision operands and the MFMA trailing update
+            const int NCp = (NC + 127) / 128 * 128, K2p = (2 * nbo + 31) / 32 * 32;
+            _Float16* Ah = (_Float16*)s->opB;
+            _Float16* Bh = Ah + (size_t)N * 2 * NBO;
+            const float* Fr = (const float*)Wr;
+            const float* Fi = (const float*)Wi;
+            if (mode == 2) {
+                hipLaunchKernelGGL(k_pack_a<__bf16>, dim3(dep_cdiv(K2p, 256), M), dim3(256), 0, st, M, nbo, K2p,
+                                   pe, k0, Fr, Fi, ld, (__bf16*)Ah);
+                hipLaunchKernelGGL(k_pack_b<__bf16>, dim3(dep_cdiv(NCp, 64)), dim3(256), 0, st, NC, NCp, nbo,
+                                   K2p, k0, pe, Fr, Fi, ld, (__bf16*)Bh);
+            } else {
+                hipLaunchKernelGGL(k_pack_a<_Float16>, dim3(dep_cdiv(K2p, 256), M), dim3(256), 0, st, M, nbo,
+                                   K2p, pe, k0, Fr, Fi, ld, Ah);
+                hipLaunchKernelGGL(k_pack_b<_Float16>, dim3(dep_cdiv(NCp, 64)), dim3(256), 0, st, NC, NCp, nbo,
+                                   K2p, k0, pe, Fr, Fi, ld, Bh);
+            }
+            prof_begin(c, &ev);
+            dim3 grid(2 * NCp / 128, dep_cdiv(M, 128));
+            float* Cr = (float*)Wr;
+            float* Ci = (float*)Wi;
+            if (mode == 2) {
+                if (gemm_v1)
+                    hipLaunchKernelGGL((k_gemm_mfma<__bf16, bf16x8>), grid, dim3(256), 0, st, M, NC, NCp, K2p,
+                                       pe, (const __bf16*)Ah, (const __bf16*)Bh, Cr, Ci, ld);
+                else
+                    hipLaunchKernelGGL((k_gemm_mfma3<__bf16, bf16x8>), grid, dim3(256), 0, st, M, NC, NCp, K2p,
+                                       pe, (const __bf16*)Ah, (const __bf16*)Bh, Cr, Ci, ld);
+            } else {
+                if (gemm_v1)
+                    hipLaunchKernelGGL((k_gemm_mfma<_Float16, f16x8>), grid, dim3(256), 0, st, M, NC, NCp, K2p,
+                                       pe, (const _Float16*)Ah, (const _Float16*)Bh, Cr, Ci, ld);
+                else
+                    hipLaunchKernelGGL((k_gemm_mfma3<_Float16, f16x8>), grid, dim3(256), 0, st, M, NC, NCp,
+                                       K2p, pe, (const _Float16*)Ah, (const _Float16*)Bh, Cr, Ci, ld);
+            }
+            // complex flops 8 M NC nbo; bytes: C read+write FP32 complex + FP16 operands
+            prof_end(c, ev, "lu_gemm_mixed", 8.0 * M * (double)NC * nbo,
+                     16.0 * M * (double)NC + 2.0 * K2p * (M + 2.0 * NCp));
         }
-        // complex flops 8 M NC nbo; bytes: C read+write FP32 complex + FP16 operands
-        prof_end(c, ev, "lu_gemm_mixed", 8.0 * M * (double)NC * nbo,
-                 16.0 * M * (double)NC + 2.0 * K2p * (M + 2.0 * NCp));
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
-    const int nblk = dep_cdiv(N, NBI);
-    std::vector<float> hp(nblk);
-    if ((e = hipMemcpyAsync(hp.data(), piv, sizeof(float) * nblk, hipMemcpyDeviceToHost, st)) != hipSuccess)
-        return e;
-    if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+    return hipGetLastError();
+}
+
+template <class T>
+static hipError_t min_pivot_of(dep_ctx* c, const T* piv, int nblk, double* out)
+{
+    std::vector<T> hp(nblk);
+    hipError_t e = hipMemcpyAsync(hp.data(), piv, sizeof(T) * nblk, hipMemcpyDeviceToHost, c->stream);
+    if (e != hipSuccess) return e;
+    if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return e;
     double mn = 1e300;
-    for (float v : hp) mn = std::min(mn, (double)v);
-    *min_pivot = mn;
+    for (T v : hp) mn = std::min(mn, (double)v);
+    *out = mn;
     return hipSuccess;
 }
 
-// out = Dc U^-1 L^-1 Dr rhs with the FP32 factors (split FP64 vectors; out may alias rhs)
-hipError_t launch_trsv_mixed(dep_sys* s, const double* rhs, double* out)
+// out = Dc U^-1 L^-1 Dr rhs (Dr, Dc optional); split FP64 vectors; out may alias rhs
+template <class T>
+static hipError_t trsv_blocked(dep_sys* s, const T* Wr, const T* Wi, long ld, const T* inv, const double* rs,
+                               const double* cs, const double* rhs, double* out)
 {
     dep_geom* g = s->g;
     const int N = g->N;
     hipStream_t st = g->ctx->stream;
     const int nblk = dep_cdiv(N, NBI);
-    const float* inv = (const float*)s->opA;
     double* xr = out;
     double* xi = out + N;
-    hipLaunchKernelGGL(k_vscale, dim3(dep_cdiv(N, 256)), dim3(256), 0, st, N, (const double*)s->rs, rhs, out);
+    if (rs)
+        hipLaunchKernelGGL(k_vscale, dim3(dep_cdiv(N, 256)), dim3(256), 0, st, N, rs, rhs, out);
+    else if (out != rhs)
+        hipLaunchKernelGGL(k_copy, dim3(dep_cdiv(2L * N, 256)), dim3(256), 0, st, 2L * N, rhs, out);
     for (int kb = 0; kb < nblk; kb++) {
         const int k0 = kb * NBI, nb = std::min(NBI, N - k0);
-        hipLaunchKernelGGL(k_block_apply32, dim3(1), dim3(256), 0, st, nb, inv + (size_t)kb * 4 * NBI * NBI,
+        hipLaunchKernelGGL(k_block_applyT<T>, dim3(1), dim3(256), 0, st, nb, inv + (size_t)kb * 4 * NBI * NBI,
                            xr + k0, xi + k0, 1);
         if (k0 + nb < N)
-            hipLaunchKernelGGL(k_panel_gemv32, dim3(dep_cdiv(N - k0 - nb, 4)), dim3(256), 0, st, k0 + nb, N,
-                               nb, s->Zr32, s->Zi32, s->ld32, k0, xr, xi);
+            hipLaunchKernelGGL(k_panel_gemvT<T>, dim3(dep_cdiv(N - k0 - nb, 4)), dim3(256), 0, st, k0 + nb, N,
+                               nb, Wr, Wi, ld, k0, xr, xi);
     }
     for (int kb = nblk - 1; kb >= 0; kb--) {
         const int k0 = kb * NBI, nb = std::min(NBI, N - k0);
-        hipLaunchKernelGGL(k_block_apply32, dim3(1), dim3(256), 0, st, nb,
+        hipLaunchKernelGGL(k_block_applyT<T>, dim3(1), dim3(256), 0, st, nb,
                            inv + (size_t)kb * 4 * NBI * NBI + 2 * NBI * NBI, xr + k0, xi + k0, 0);
         if (k0 > 0)
-            hipLaunchKernelGGL(k_panel_gemv32, dim3(dep_cdiv(k0, 4)), dim3(256), 0, st, 0, k0, nb, s->Zr32,
-                               s->Zi32, s->ld32, k0, xr, xi);
+            hipLaunchKernelGGL(k_panel_gemvT<T>, dim3(dep_cdiv(k0, 4)), dim3(256), 0, st, 0, k0, nb, Wr, Wi, ld,
+                               k0, xr, xi);
     }
-    hipLaunchKernelGGL(k_vscale, dim3(dep_cdiv(N, 256)), dim3(256), 0, st, N, (const double*)s->cs, out, out);
+    if (cs) hipLaunchKernelGGL(k_vscale, dim3(dep_cdiv(N, 256)), dim3(256), 0, st, N, cs, out, out);
     return hipGetLastError();
+}
+
+// ---- entry points -----------------------------------------------------------
+// FP64: Z (built by launch_weight_fp64, or supplied by a dense caller) in s->Zr/Zi
+hipError_t launch_lu_fp64(dep_sys* s, double* min_pivot)
+{
+    hipError_t e = lu_blocked<double>(s, s->Zr, s->Zi, s->ldZ, (double*)s->opA64, s->dwork, 0);
+    if (e != hipSuccess) return e;
+    return min_pivot_of<double>(s->g->ctx, s->dwork, dep_cdiv(s->g->N, NBI), min_pivot);
+}
+
+hipError_t launch_trsv_fp64(dep_sys* s, const double* rhs, double* out)
+{
+    return trsv_blocked<double>(s, s->Zr, s->Zi, s->ldZ, (const double*)s->opA64, nullptr, nullptr, rhs, out);
+}
+
+hipError_t launch_lu_mixed(dep_sys* s, int bf16, double* min_pivot)
+{
+    dep_geom* g = s->g;
+    hipStream_t st = g->ctx->stream;
+    const int N = g->N;
+    const long ld = s->ld32;
+    hipError_t e;
+    // equilibration and W = Dr Z Dc in FP32
+    hipLaunchKernelGGL(k_rowmax, dim3(dep_cdiv(N, 4)), dim3(256), 0, st, N, g->P, g->ldP, s->wr, s->wi,
+                       s->shift, s->rs);
+    if ((e = hipMemsetAsync(s->cs, 0, sizeof(double) * N, st)) != hipSuccess) return e;
+    const int rows_per = 512;
+    hipLaunchKernelGGL(k_colmax, dim3(dep_cdiv(N, 256), dep_cdiv(N, rows_per)), dim3(256), 0, st, N,
+                       rows_per, g->P, g->ldP, s->wr, s->wi, s->shift, (const double*)s->rs,
+                       (unsigned long long*)s->cs);
+    hipLaunchKernelGGL(k_colscale, dim3(dep_cdiv(N, 256)), dim3(256), 0, st, N,
+                       (const unsigned long long*)s->cs, s->cs);
+    hipLaunchKernelGGL(k_build_w, dim3(dep_cdiv(N, 1024), N), dim3(256), 0, st, N, g->P, g->ldP, s->wr,
+                       s->wi, s->shift, (const double*)s->rs, (const double*)s->cs, s->Zr32, s->Zi32, ld);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    float* piv = (float*)s->dwork;
+    e = lu_blocked<float>(s, s->Zr32, s->Zi32, ld, (float*)s->opA, piv, bf16 ? 2 : 1);
+    if (e != hipSuccess) return e;
+    return min_pivot_of<float>(g->ctx, piv, dep_cdiv(N, NBI), min_pivot);
+}
+
+hipError_t launch_trsv_mixed(dep_sys* s, const double* rhs, double* out)
+{
+    return trsv_blocked<float>(s, s->Zr32, s->Zi32, s->ld32, (const float*)s->opA, s->rs, s->cs, rhs, out);
 }
 
 hipError_t launch_axpy(dep_ctx* c, long n, const double* d, double* x)

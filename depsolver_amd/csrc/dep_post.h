@@ -29,6 +29,7 @@ void dep_points_release(dep_points* P);
 
 hipError_t launch_sigma_at_points(dep_sys* s);
 hipError_t launch_points(dep_sys* s, int mode, const dep_points* P, double* out);
+hipError_t launch_multipole_terms(dep_sys* s, int m, int order, const double* pts, double* out);
 hipError_t launch_mst(dep_sys* s, const dep_points* P, const double* ef, const int* nodes, int nf,
                       const int* felems, double half_eps, double* Efn, double* Fce, double* Xce,
                       double* Fcm);

@@ -79,13 +79,13 @@ __global__ __launch_bounds__(256) void k_points_regular(
             const double2 sg = qs[e * 7 + g];
             const double dx = x0 - y.x, dy = x1 - y.y, dz = x2 - y.z;
             const double rsq = dx * dx + dy * dy + dz * dz;
-            const double r = sqrt(rsq);
+            const double ri = rsqrt_nr(rsq);
             if constexpr (MODE == 0) {
-                const double A = y.w / r;
+                const double A = y.w * ri;
                 acc[0] += A * sg.x;
                 acc[1] += A * sg.y;
             } else {
-                const double A = y.w / (rsq * r);
+                const double A = y.w * (ri * ri * ri);
                 const double ar = A * sg.x, ai = A * sg.y;
                 acc[0] += ar * dx;
                 acc[1] += ai * dx;
@@ -282,6 +282,89 @@ __global__ __launch_bounds__(256) void k_sum3(int n, const double* __restrict__ 
     }
 }
 
+// Multipolar force support (forceIntegrals.c:352-619 intDE_*, force.c:709-825):
+// for one evaluation point per workgroup, the 31 derivative integrals of the
+// Green's-function gradient up to third order, contracted with Re sigma
+// interpolated at the quadrature points (force.c:759 uses Re sigma only).
+// Slot order matches DE[0..30] of the reference.  Fixed-order block reduction.
+__global__ __launch_bounds__(256) void k_multipole_terms(int E, int order, const double* __restrict__ pts,
+                                                         const double4* __restrict__ q,
+                                                         const double2* __restrict__ qs,
+                                                         double* __restrict__ out /* m x 31 */)
+{
+    __shared__ double red[31][256];
+    const int p = blockIdx.x, t = threadIdx.x;
+    const double x0 = pts[3 * p], x1 = pts[3 * p + 1], x2 = pts[3 * p + 2];
+    double acc[31];
+#pragma unroll
+    for (int k = 0; k < 31; k++) acc[k] = 0.0;
+    for (int e = t; e < E; e += 256) {
+#pragma unroll
+        for (int g = 0; g < 7; g++) {
+            const double4 y = q[e * 7 + g];
+            const double s = qs[e * 7 + g].x;
+            // y.w = W J; the reference's A = J W / r^3 (pow(rsq, 1.5))
+            const double dx = x0 - y.x, dy = x1 - y.y, dz = x2 - y.z;
+            const double dx2 = dx * dx, dy2 = dy * dy, dz2 = dz * dz;
+            const double rsq = dx2 + dy2 + dz2;
+            const double A = y.w / (rsq * sqrt(rsq));
+            const double B = 3.0 * A / rsq;
+            double DE[31];
+            DE[0] = A - B * dx2;
+            DE[1] = -B * dx * dy;
+            DE[2] = -B * dx * dz;
+            DE[3] = A - B * dy2;
+            DE[4] = -B * dy * dz;
+            DE[5] = A - B * dz2;
+#pragma unroll
+            for (int k = 6; k < 31; k++) DE[k] = 0.0;
+            if (order > 1) {
+                const double C = 5.0 * B / rsq;
+                DE[6] = (C * dx2 - 3.0 * B) * dx;
+                DE[7] = (C * dx2 - B) * dy;
+                DE[8] = (C * dx2 - B) * dz;
+                DE[9] = (C * dy2 - B) * dx;
+                DE[10] = C * dx * dy * dz;
+                DE[11] = (C * dz2 - B) * dx;
+                DE[12] = (C * dy2 - 3.0 * B) * dy;
+                DE[13] = (C * dy2 - B) * dz;
+                DE[14] = (C * dz2 - B) * dy;
+                DE[15] = (C * dz2 - 3.0 * B) * dz;
+                if (order > 2) {
+                    const double D = 7.0 * C / rsq;
+                    DE[16] = (-D * dx2 + 6.0 * C) * dx2 - 3.0 * B;
+                    DE[17] = (-D * dx2 + 3.0 * C) * dx * dy;
+                    DE[18] = (-D * dx2 + 3.0 * C) * dx * dz;
+                    DE[19] = -D * dx2 * dy2 + C * (dx2 + dy2) - B;
+                    DE[20] = (-D * dx2 + C) * dy * dz;
+                    DE[21] = -D * dx2 * dz2 + C * (dx2 + dz2) - B;
+                    DE[22] = (-D * dy2 + 3.0 * C) * dx * dy;
+                    DE[23] = (-D * dy2 + C) * dx * dz;
+                    DE[24] = (-D * dz2 + C) * dx * dy;
+                    DE[25] = (-D * dz2 + 3.0 * C) * dx * dz;
+                    DE[26] = (-D * dy2 + 6.0 * C) * dy2 - 3.0 * B;
+                    DE[27] = (-D * dy2 + 3.0 * C) * dy * dz;
+                    DE[28] = -D * dy2 * dz2 + C * (dy2 + dz2) - B;
+                    DE[29] = (-D * dz2 + 3.0 * C) * dy * dz;
+                    DE[30] = (-D * dz2 + 6.0 * C) * dz2 - 3.0 * B;
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 31; k++) acc[k] += DE[k] * s;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 31; k++) red[k][t] = acc[k];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w)
+#pragma unroll
+            for (int k = 0; k < 31; k++) red[k][t] += red[k][t + w];
+        __syncthreads();
+    }
+    if (t < 31) out[31 * p + t] = red[t][0];
+}
+
 }  // namespace
 
 hipError_t launch_sigma_at_points(dep_sys* s)
@@ -363,5 +446,13 @@ hipError_t launch_mst(dep_sys* s, const dep_points* P, const double* ef, const i
         hipLaunchKernelGGL(k_mst_elements<6>, dim3(dep_cdiv(nf, 64)), dim3(64), 0, st, nf, felems,
                            g->xyz, g->conn, Efn, half_eps, Fce, Xce);
     hipLaunchKernelGGL(k_sum3, dim3(1), dim3(256), 0, st, nf, Fce, Fcm);
+    return hipGetLastError();
+}
+
+hipError_t launch_multipole_terms(dep_sys* s, int m, int order, const double* pts, double* out)
+{
+    dep_geom* g = s->g;
+    hipLaunchKernelGGL(k_multipole_terms, dim3(m), dim3(256), 0, g->ctx->stream, g->E, order, pts,
+                       (const double4*)g->q, (const double2*)s->qs, out);
     return hipGetLastError();
 }

@@ -1,0 +1,163 @@
+"""GPU: GMRES (gmres_el semantics), the multipolar force, the command-line
+driver on depSolver input files, and the reference-signature shims."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from depsolver_amd import mesh
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+GOLD = os.path.join(HERE, "golden")
+
+
+def load_case(name):
+    z = np.load(os.path.join(GOLD, f"{name}.npz"))
+    p = mesh.Problem(z["xyz"], z["conn"].astype(np.int64), z["bctype"], z["bc"], z["mats"],
+                     z["ifaces"], float(z["freq"]), None, name)
+    return p, z
+
+
+def rel(a, b):
+    return float(np.abs(np.asarray(a) - np.asarray(b)).max() / np.abs(b).max())
+
+
+@pytest.mark.parametrize("name", ["tiny_tria6", "tiny_tria3", "config1"])
+def test_gmres_tight_tolerance_reaches_lu_solution(ctx, name):
+    import depsolver_amd as dep
+    print(f"[gpu] gmres {name}", flush=True)
+    p, z = load_case(name)
+    g, s = dep.Geometry(ctx, p), None
+    try:
+        s = dep.System(g)
+        sigma, info = s.gmres(precond=1, tol=1e-12, max_iters=600)
+        assert info["converged"] == 1
+        assert rel(sigma, z["sigma"]) < 1e-9
+    finally:
+        if s:
+            s.close()
+        g.close()
+
+
+@pytest.mark.parametrize("name", ["tiny_tria6", "tiny_tria3"])
+def test_gmres_follows_the_reference_iteration(ctx, orc, name, tmp_path):
+    # same Krylov iteration as gmres_el at its 1e-6 tolerance: same iteration count,
+    # same solution to well below the tolerance, same gmres.log layout
+    import depsolver_amd as dep
+    p, z = load_case(name)
+    Ael, b = orc.assemble(p, False)
+    rc, xo, it = orc.gmres_el(Ael, b, precond=1, tol=1e-6, log_path=str(tmp_path / "ref.log"))
+    g = dep.Geometry(ctx, p)
+    s = dep.System(g)
+    try:
+        sigma, info = s.gmres(precond=1, tol=1e-6, log_path=str(tmp_path / "gpu.log"))
+        assert info["iterations"] == it
+        assert rel(sigma, xo) < 1e-8
+        a = open(tmp_path / "ref.log").read().split("\n")
+        c = open(tmp_path / "gpu.log").read().split("\n")
+        assert len(a) == len(c) and a[:6] == c[:6] and a[-4:] == c[-4:]
+    finally:
+        s.close()
+        g.close()
+
+
+@pytest.mark.parametrize("name", ["tiny_tria6", "tiny_tria3"])
+def test_multipole_force_matches_reference(ctx, name):
+    import depsolver_amd as dep
+    p, z = load_case(name)
+    g = dep.Geometry(ctx, p)
+    s = dep.System(g)
+    try:
+        s.set_solution(z["sigma"])
+        for o in (1, 2, 3):
+            F = s.multipole(z["xf"], 1.0, order=o)
+            ref = z["multipole"][:, o - 1]
+            assert np.abs(F - ref).max() <= 1e-8 * np.abs(ref).max(), (o, F, ref)
+    finally:
+        s.close()
+        g.close()
+
+
+def run_driver(workdir, env=None):
+    exe = os.path.join(ROOT, "depsolver_amd", "bin", "depSolver-amd")
+    e = dict(os.environ)
+    e.update(env or {})
+    r = subprocess.run([exe, str(workdir)], capture_output=True, text=True, timeout=240, env=e)
+    assert r.returncode == 0, r.stderr
+
+
+def test_driver_reproduces_shipped_dep_trap_2(tmp_path):
+    # the reference's own example (tria3, REPOSITION, GMRES + Jacobi at 1e-6, MST force)
+    import shutil
+    for f in ("input.bem", "nodes.bem", "elems.bem", "bcs.bem"):
+        shutil.copy(os.path.join(GOLD, "dep-trap-2", f), tmp_path / f)
+    run_driver(tmp_path)
+    want = np.loadtxt(os.path.join(GOLD, "dep-trap-2", "solution.dat"))
+    got = np.loadtxt(tmp_path / "solution.dat")
+    assert np.abs(got[:, :3] - want[:, :3]).max() < 1e-12
+    assert np.abs(got[:, 3:] - want[:, 3:]).max() <= 1e-5 * np.abs(want[:, 3:]).max()
+    w = [l for l in open(os.path.join(GOLD, "dep-trap-2", "force-mst.dat")) if not l.startswith("#")]
+    g = [l for l in open(tmp_path / "force-mst.dat") if not l.startswith("#")]
+    assert len(w) == len(g)
+    W = np.array([[float(v) for v in l.split()] for l in w])
+    G = np.array([[float(v) for v in l.split()] for l in g])
+    assert np.abs(G[:, :3] - W[:, :3]).max() <= 1e-6 * np.abs(W[:, :3]).max()
+    assert np.abs(G[1:, 3:] - W[1:, 3:]).max() <= 1e-4 * np.abs(W[1:, 3:]).max()
+    log = open(tmp_path / "gmres.log").read()
+    assert "PROBLEM CONVERGES IN" in log
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp16"])
+def test_driver_lu_path_matches_oracle(tmp_path, orc, prec):
+    p = mesh.sphere_between_plates(1, 4)
+    pts = np.array([[0.1, 0.2, 1.6], [0.0, 0.0, 0.0], [1.0, -2.0, 2.0]])
+    mesh.write_bem(str(tmp_path), p, solver="gaussBksb", analysis=3, internal_points=pts)
+    run_driver(tmp_path, {"DEPSOLVER_PRECISION": prec})
+    A, b = orc.assemble(p, True)
+    x = orc.gauss_bksb(A, b)
+    sol = np.loadtxt(tmp_path / "solution.dat")
+    N = p.n_nodes
+    # solution.dat prints 7 significant digits (%le)
+    assert np.abs(sol[:, 3] - x[:N]).max() <= 1e-6 * np.abs(x).max()
+    pot = np.loadtxt(tmp_path / "potential.dat")
+    for k, pt in enumerate(pts):
+        assert np.abs(pot[k, 3:] - orc.potential(p, x, pt)).max() <= 1e-6 * np.abs(pot[:, 3:]).max()
+    Fce, Xce, Fcm, Xcm = orc.force_mst(p, x)
+    rows = np.loadtxt(tmp_path / "force-mst.dat")
+    assert np.abs(rows[0, :3] - Xcm).max() < 1e-6
+    assert np.abs(rows[1:, 3:] - Fce[p.bctype[p.conn[:, 0]] == 6]).max() <= 1e-6 * np.abs(Fce).max()
+
+
+@pytest.mark.parametrize("solver", ["lu", "gmres"])
+def test_reference_signature_shims(tmp_path, orc, solver):
+    exe = tmp_path / "dropin"
+    lib = os.path.join(ROOT, "depsolver_amd")
+    subprocess.run(["gcc", "-O2", "-std=c99", "-o", str(exe), os.path.join(HERE, "dropin_main.c"),
+                    f"-L{lib}", "-ldepsolver_amd", f"-Wl,-rpath,{lib}"], check=True)
+    p = mesh.sphere_between_plates(1, 4)
+    N = p.n_nodes
+    with open(tmp_path / "problem.txt", "w") as f:
+        f.write(f"{N} {p.n_elems} {p.npe} {len(p.mats)} {len(p.ifaces)} {p.freq:.17e}\n")
+        for x in p.xyz:
+            f.write("%.17e %.17e %.17e\n" % tuple(x))
+        for e in p.conn + 1:
+            f.write(" ".join(str(int(v)) for v in e) + "\n")
+        for m in p.mats:
+            f.write("%.17e %.17e\n" % tuple(m))
+        for i in p.ifaces:
+            f.write("%d %d\n" % tuple(i))
+        for j in range(2 * N):
+            f.write("%d %.17e %.17e %.17e\n" % (p.bctype[j], *p.bc[j]))
+    r = subprocess.run([str(exe), str(tmp_path / "problem.txt"), str(tmp_path / "x.txt"), solver],
+                       capture_output=True, text=True, timeout=120, cwd=tmp_path)
+    assert r.returncode == 0, r.stderr
+    x = np.loadtxt(tmp_path / "x.txt")
+    A, b = orc.assemble(p, True)
+    want = orc.gauss_bksb(A, b)
+    tol = 1e-10 if solver == "lu" else 1e-5
+    assert rel(x, want) < tol
+    if solver == "gmres":
+        assert "PROBLEM CONVERGES IN" in open(tmp_path / "gmres.log").read()
