@@ -119,7 +119,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--fp64-steps", type=int, default=2)
-    ap.add_argument("--precision", default="fp16", choices=["fp16", "bf16", "fp32", "fp64"])
+    ap.add_argument("--precision", default="fp16", choices=["fp16", "bf16", "fp64"])
+    ap.add_argument("--gmres-steps", type=int, default=1)
     ap.add_argument("--config", type=int, default=2, choices=[1, 2])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -147,12 +148,28 @@ def main():
         Fce, Xce, Fcm, Xcm = s.mst()
         return time.perf_counter() - t0, sigma, info, Fce, Fcm
 
+    def step_gmres():
+        t0 = time.perf_counter()
+        dep.lib().dep_geom_reassemble(g._h)
+        s.set_frequency(p.freq)
+        sigma, info = s.gmres(precond=1, tol=1e-12, max_iters=600)
+        Fce, Xce, Fcm, Xcm = s.mst()
+        return time.perf_counter() - t0, sigma, info, Fce, Fcm
+
     # FP64 reference solve (also the accuracy yardstick for the mixed path)
     lat64 = []
     for k in range(max(1, args.fp64_steps)):
         dt, sig64, info64, Fce64, Fcm64 = step("fp64")
         lat64.append(dt)
         log(f"bench: fp64 step {k} {dt * 1e3:.1f} ms residual {info64['residual']:.2e}")
+    # FP64 GMRES (Jacobi, unrestarted, tolerance 1e-12): the other plain FP64 path
+    latg, gm = [], None
+    for k in range(args.gmres_steps):
+        dt, sigg, infog, Fceg, Fcmg = step_gmres()
+        latg.append(dt)
+        gm = {"iterations": infog["iterations"], "converged": infog["converged"],
+              "sigma_rel_vs_lu": float(np.abs(sigg - sig64).max() / np.abs(sig64).max())}
+        log(f"bench: fp64 gmres step {k} {dt * 1e3:.1f} ms iterations {infog['iterations']}")
     prec = args.precision
     try:
         step(prec)
@@ -192,7 +209,7 @@ def main():
         "value": 1e3 / ms, "unit": "solves/sec", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": {"fp16": "fp16/fp32+f64", "bf16": "bf16/fp32+f64",
-                                       "fp32": "fp32+f64", "fp64": "f64"}[prec],
+                                       "fp64": "f64"}[prec],
         "data": "synthetic (seeded mesh generator, config geometry)",
         "config": {"workload": f"{p.name}: sphere L4 + 24x24 plates, N={N}, E={p.n_elems}, "
                                "assemble+factor+refine(1e-12)+MST per step",
@@ -201,6 +218,9 @@ def main():
         "fp64_solves_per_sec": 1.0 / float(np.median(lat64)),
         "fp64_p50_ms": float(np.median(lat64) * 1e3),
         "speedup_vs_fp64": float(np.median(lat64) / np.median(lat)),
+        "fp64_gmres": None if not latg else dict(gm, solves_per_sec=1.0 / float(np.median(latg)),
+                                                  p50_ms=float(np.median(latg) * 1e3),
+                                                  speedup=float(np.median(latg) / np.median(lat))),
         "errors_vs_fp64": errors,
         "kernels": {k: {"launches": v["launches"], "avg_ms": v["ms"] / v["launches"]} for k, v in stats.items()},
         "roofline": roofline(stats),

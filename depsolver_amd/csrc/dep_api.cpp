@@ -76,6 +76,8 @@ static int download(dep_ctx* c, T* dst, const T* src, size_t count)
 // context
 // ---------------------------------------------------------------------------
 
+extern "C" int dep_ctx_destroy(dep_ctx* c);
+
 extern "C" int dep_ctx_create(int device, dep_ctx** out)
 {
     if (!out) return DEP_ERR_ARG;
@@ -85,9 +87,13 @@ extern "C" int dep_ctx_create(int device, dep_ctx** out)
     if (device < 0 || device >= n) return DEP_ERR_ARG;
     dep_ctx* c = new dep_ctx();
     c->device = device;
+    // both streams at the default priority
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
-        delete c;
+        hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->join, hipEventDisableTiming) != hipSuccess) {
+        dep_ctx_destroy(c);
         return DEP_ERR_HIP;
     }
     *out = c;
@@ -98,12 +104,16 @@ extern "C" int dep_ctx_destroy(dep_ctx* c)
 {
     if (!c) return DEP_OK;
     (void)hipSetDevice(c->device);
-    (void)hipStreamSynchronize(c->stream);
-    prof_collect(c);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->stream2) (void)hipStreamSynchronize(c->stream2);
+    if (c->stream && c->stream2) prof_collect(c);
     for (hipEvent_t e : c->pool) (void)hipEventDestroy(e);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->fork) (void)hipEventDestroy(c->fork);
+    if (c->join) (void)hipEventDestroy(c->join);
     if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->stream2) (void)hipStreamDestroy(c->stream2);
     delete c;
     return DEP_OK;
 }
@@ -131,20 +141,20 @@ static hipEvent_t pool_get(dep_ctx* c)
     return e;
 }
 
-void prof_begin(dep_ctx* c, hipEvent_t* a)
+void prof_begin(dep_ctx* c, hipEvent_t* a, hipStream_t st)
 {
     *a = nullptr;
     if (!c->prof) return;
     *a = pool_get(c);
-    if (*a) (void)hipEventRecord(*a, c->stream);
+    if (*a) (void)hipEventRecord(*a, st ? st : c->stream);
 }
 
-void prof_end(dep_ctx* c, hipEvent_t a, const char* name, double flops, double bytes)
+void prof_end(dep_ctx* c, hipEvent_t a, const char* name, double flops, double bytes, hipStream_t st)
 {
     if (!c->prof || !a) return;
     hipEvent_t b = pool_get(c);
     if (!b) return;
-    (void)hipEventRecord(b, c->stream);
+    (void)hipEventRecord(b, st ? st : c->stream);
     c->pending.push_back({name, a, b, flops, bytes});
 }
 
@@ -152,6 +162,7 @@ void prof_collect(dep_ctx* c)
 {
     if (c->pending.empty()) return;
     (void)hipStreamSynchronize(c->stream);
+    (void)hipStreamSynchronize(c->stream2);
     for (auto& p : c->pending) {
         float ms = 0.0f;
         if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
@@ -487,6 +498,8 @@ extern "C" int dep_sys_create(dep_geom* g, int nM, const double* mats, int nI, c
     if (r == DEP_OK) r = dep_obj_alloc(s, &s->x, 2 * (size_t)N);
     if (r == DEP_OK) r = dep_obj_alloc(s, &s->work, 8 * 2 * (size_t)N);
     if (r == DEP_OK) r = dep_obj_alloc(s, &s->dwork, 65536);
+    if (r == DEP_OK && hipMemsetAsync(s->dwork, 0, sizeof(double) * 65536, g->ctx->stream) != hipSuccess)
+        r = dep_fail(g->ctx, DEP_ERR_HIP, "hipMemsetAsync failed");
     if (r == DEP_OK) r = dep_obj_alloc(s, &s->qs, (size_t)g->E * 7 * 2);
     if (r == DEP_OK) r = dep_obj_alloc(s, &s->tbuf, 2 * (size_t)N);
     if (r == DEP_OK) r = set_weights(s, freq);
@@ -587,8 +600,9 @@ static int ensure_mixed_storage(dep_sys* s)
     float* inv = nullptr;
     DEP_CHECK(dep_obj_alloc(s, &inv, (size_t)dep_cdiv(N, 64) * 4 * 64 * 64));
     s->opA = inv;
-    uint16_t* ops = nullptr;   // FP16/BF16 operand panels: N x 512 then 2*ld32 x 512
-    DEP_CHECK(dep_obj_alloc(s, &ops, (size_t)N * 512 + 2 * (size_t)s->ld32 * 512));
+    // FP16/BF16 operand panels: A' N x 2 NBO, B' 2 ld32 x 2 NBO, B' of the next panel 2 NBO x 2 NBO
+    uint16_t* ops = nullptr;
+    DEP_CHECK(dep_obj_alloc(s, &ops, (size_t)(N + 2 * s->ld32 + 2 * DEP_LU_NBO) * 2 * DEP_LU_NBO));
     s->opB = ops;
     DEP_CHECK(dep_obj_alloc(s, &s->rs, (size_t)N));
     DEP_CHECK(dep_obj_alloc(s, &s->cs, (size_t)N));
@@ -631,7 +645,7 @@ static int dev_norm(dep_sys* s, const double* v, double* out)
 {
     dep_ctx* c = s->g->ctx;
     double* h = s->dwork + 32768;
-    DEP_HIP(c, launch_dots(c, 1, 2L * s->g->N, v, v, h));
+    DEP_HIP(c, launch_dots(c, 1, 2L * s->g->N, v, v, h, h + 64));
     double hv = 0.0;
     DEP_HIP(c, hipMemcpyAsync(&hv, h, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     DEP_HIP(c, hipStreamSynchronize(c->stream));
@@ -713,6 +727,9 @@ extern "C" int dep_solve(dep_sys* s, double tol, int max_steps, double* sigma, d
     DEP_HIP(c, hipEventSynchronize(c->ev1));
     float ms = 0.0f;
     DEP_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    int timed_out = 0;
+    DEP_HIP(c, trsv_status(s, &timed_out));
+    if (timed_out) return dep_fail(c, DEP_ERR_HIP, "dep_solve: a triangular sweep exceeded its wait limit");
     if (info) {
         memset(info, 0, sizeof(*info));
         info->iterations = steps;
@@ -1047,6 +1064,35 @@ extern "C" int dep_post_multipole(dep_sys* s, int order, double R, int m, const 
 
 // GMRES (src/gmres_el.c:66-354): unrestarted, Givens rotations on the host,
 // Krylov vectors and products on the device.
+// GMRES workspace, kept in the system between calls (a serving loop does not
+// allocate per solve) and grown when a call asks for more iterations.
+static void sys_release(dep_sys* s, void* p)
+{
+    for (auto it = s->allocs.begin(); it != s->allocs.end(); ++it)
+        if (it->first == p) {
+            dep_free(s->g->ctx, p, it->second);
+            s->allocs.erase(it);
+            return;
+        }
+}
+
+static int ensure_gmres_storage(dep_sys* s, int max_iters)
+{
+    if (s->gm_V && s->gm_cap >= max_iters) return DEP_OK;
+    const size_t N = s->g->N, n2 = 2 * N;
+    if (s->gm_V) {
+        DEP_HIP(s->g->ctx, hipStreamSynchronize(s->g->ctx->stream));
+        sys_release(s, s->gm_V);
+        sys_release(s, s->gm_red);
+        s->gm_V = s->gm_red = nullptr;
+    }
+    if (!s->gm_vec) DEP_CHECK(dep_obj_alloc(s, &s->gm_vec, 3 * N + 3 * n2));
+    DEP_CHECK(dep_obj_alloc(s, &s->gm_V, (size_t)(max_iters + 1) * n2));
+    DEP_CHECK(dep_obj_alloc(s, &s->gm_red, 2 + (size_t)(max_iters + 1) * (2 + DEP_DOT_CHUNKS)));
+    s->gm_cap = max_iters;
+    return DEP_OK;
+}
+
 extern "C" int dep_gmres(dep_sys* s, int precond, double tol_rel, int max_iters, const double* x0,
                          double* sigma, dep_solve_info* info, const char* log_path)
 {
@@ -1065,22 +1111,22 @@ extern "C" int dep_gmres(dep_sys* s, int precond, double tol_rel, int max_iters,
         DEP_HIP(c, hipStreamSynchronize(c->stream));
         for (int i = 0; i < N; i++) d[i] = 1.0 / (s->hwr[i] * pd[i] + s->hshift[i]);
     }
-    struct Tmp {
-        dep_ctx* c;
-        std::vector<std::pair<void*, size_t>> allocs;
-        dep_ctx* ctx_of() { return c; }
-        ~Tmp() { for (auto& a : allocs) dep_free(c, a.first, a.second); }
-    } t{c, {}};
-    double *gw = nullptr, *gwi = nullptr, *gsh = nullptr, *b = nullptr, *x = nullptr, *V = nullptr,
-           *w = nullptr, *h = nullptr;
-    DEP_CHECK(dep_obj_alloc(&t, &gw, N));
-    DEP_CHECK(dep_obj_alloc(&t, &gwi, N));
-    DEP_CHECK(dep_obj_alloc(&t, &gsh, N));
-    DEP_CHECK(dep_obj_alloc(&t, &b, n2));
-    DEP_CHECK(dep_obj_alloc(&t, &x, n2));
-    DEP_CHECK(dep_obj_alloc(&t, &w, n2));
-    DEP_CHECK(dep_obj_alloc(&t, &h, 2 * (size_t)(max_iters + 2)));
-    DEP_CHECK(dep_obj_alloc(&t, &V, (size_t)(max_iters + 1) * n2));
+    DEP_CHECK(ensure_gmres_storage(s, max_iters));
+    double* gw = s->gm_vec;
+    double* gwi = gw + N;
+    double* gsh = gwi + N;
+    double* b = gsh + N;
+    double* x = b + n2;
+    double* w = x + n2;
+    double* V = s->gm_V;
+    // reductions: [0] |w|^2 before, [1] after orthogonalisation, then the two CGS passes'
+    // coefficients (cap + 1 each), then the dot-product partials
+    const int cap = s->gm_cap;
+    double* red = s->gm_red;
+    double* h1 = red + 2;
+    double* h2 = h1 + (cap + 1);
+    double* part = h2 + (cap + 1);
+    double* h = h1;
     std::vector<double> hb(n2);
     DEP_CHECK(download(c, hb.data(), s->b, n2));
     if (!s->dense) {
@@ -1101,9 +1147,9 @@ extern "C" int dep_gmres(dep_sys* s, int precond, double tol_rel, int max_iters,
         return launch_zmatvec_w(s, gw, gwi, gsh, in, rhs, out);
     };
     auto norm = [&](const double* v, double* out) -> int {
-        DEP_HIP(c, launch_dots(c, 1, n2, v, v, h));
+        DEP_HIP(c, launch_dots(c, 1, n2, v, v, red, part));
         double hv = 0.0;
-        DEP_CHECK(download(c, &hv, h, 1));
+        DEP_CHECK(download(c, &hv, red, 1));
         *out = sqrt(hv);
         return DEP_OK;
     };
@@ -1137,7 +1183,7 @@ extern "C" int dep_gmres(dep_sys* s, int precond, double tol_rel, int max_iters,
     DEP_HIP(c, launch_scale(c, n2, V, 1.0 / beta, V));
     const int M = max_iters;
     std::vector<double> H((size_t)(M + 1) * (M + 1), 0.0), cs(M + 1, 0.0), sn(M + 1, 0.0),
-        res(M + 2, 0.0), hh(M + 2);
+        res(M + 2, 0.0), hh(2 + 2 * (size_t)(cap + 1));
 #define HH(r, col) H[(size_t)(r) * (M + 1) + (col)]
     res[0] = beta;
     double err0 = beta, err1 = beta;
@@ -1147,19 +1193,17 @@ extern "C" int dep_gmres(dep_sys* s, int precond, double tol_rel, int max_iters,
         iters++;
         double* vi = V + (size_t)i * n2;
         DEP_HIP(c, matvec(vi, nullptr, w));
-        double wn = 0.0;
-        DEP_CHECK(norm(w, &wn));
-        // CGS2
-        std::vector<double> hsum(i + 1, 0.0);
-        for (int pass = 0; pass < 2; pass++) {
-            DEP_HIP(c, launch_dots(c, i + 1, n2, V, w, h));
-            DEP_HIP(c, launch_update(c, i + 1, n2, V, h, w));
-            DEP_CHECK(download(c, hh.data(), h, i + 1));
-            for (int j = 0; j <= i; j++) hsum[j] += hh[j];
-        }
-        for (int j = 0; j <= i; j++) HH(j, i) = hsum[j];
-        double hn = 0.0;
-        DEP_CHECK(norm(w, &hn));
+        // CGS2 on the device; one transfer brings back both passes and both norms
+        DEP_HIP(c, launch_dots(c, 1, n2, w, w, red, part));
+        DEP_HIP(c, launch_dots(c, i + 1, n2, V, w, h1, part));
+        DEP_HIP(c, launch_update(c, i + 1, n2, V, h1, w));
+        DEP_HIP(c, launch_dots(c, i + 1, n2, V, w, h2, part));
+        DEP_HIP(c, launch_update(c, i + 1, n2, V, h2, w));
+        DEP_HIP(c, launch_dots(c, 1, n2, w, w, red + 1, part));
+        DEP_CHECK(download(c, hh.data(), red, 2 + 2 * (size_t)(cap + 1)));
+        const double wn = sqrt(hh[0]);
+        for (int j = 0; j <= i; j++) HH(j, i) = hh[2 + j] + hh[2 + (cap + 1) + j];
+        const double hn = sqrt(hh[1]);
         HH(i + 1, i) = hn;
         if (hn != 0.0) DEP_HIP(c, launch_scale(c, n2, w, 1.0 / hn, V + (size_t)(i + 1) * n2));
         for (int j = 0; j <= i - 1; j++) {

@@ -29,6 +29,7 @@ extern FILE* file_log __attribute__((weak));
 
 hipError_t launch_lu_fp64(dep_sys* s, double* min_pivot);
 hipError_t launch_trsv_fp64(dep_sys* s, const double* rhs, double* out);
+hipError_t trsv_status(dep_sys* s, int* timed_out);
 
 namespace {
 
@@ -166,6 +167,7 @@ struct Dense {
         check(dep_obj_alloc(&s, &s.b, 2 * (size_t)n), "alloc");
         check(dep_obj_alloc(&s, &s.x, 2 * (size_t)n), "alloc");
         check(dep_obj_alloc(&s, &s.dwork, 65536), "alloc");
+        if (hipMemset(s.dwork, 0, sizeof(double) * 65536) != hipSuccess) fatal("hipMemset failed");
     }
     ~Dense()
     {
@@ -243,6 +245,8 @@ extern "C" int gaussBksb(unsigned int N, double** A, double* B)
         exit(1);
     }
     if (launch_trsv_fp64(&d.s, d.s.b, d.s.x) != hipSuccess) fatal("gaussBksb: solve kernels failed");
+    int timed_out = 0;
+    if (trsv_status(&d.s, &timed_out) != hipSuccess || timed_out) fatal("gaussBksb: triangular sweep failed");
     std::vector<double> x(2 * (size_t)n);
     if (hipMemcpy(x.data(), d.s.x, sizeof(double) * 2 * n, hipMemcpyDeviceToHost) != hipSuccess)
         fatal("hipMemcpy failed");

@@ -1,13 +1,22 @@
 // dep_diag.h -- LU of one 64 x 64 complex diagonal block and the inverses of
-// its triangular factors, in LDS, one barrier per elimination step.
+// its triangular factors, register-resident, one barrier per step.
 //
-// Elimination without pivoting (the reference's gaussBksb exchanges rows only
-// for |pivot| <= 1e-9, never reached here): at step p every thread updating
-// (r, c > p) forms its own multiplier D[r][p] / D[p][p], and the thread of
-// column p stores the multiplier in the staging array X, so no value is read
-// and written in the same step.  The inverses use Gauss-Jordan sweeps on
-// [L | I] (unit lower, forward) and [U | I] (backward); each row is final one
-// step before it is needed and is written straight to the output.
+// Thread t of the 1024-thread block owns row r = t & 63 and the 4 columns
+// c = 4w + k (w = t >> 6, the wave index; k = 0..3) of the block: its
+// entries of L\U and of both inverses live in registers.  An elimination
+// step publishes one row and one column through double-buffered LDS vectors
+// and synchronises once; the published row is zero outside the columns the
+// step changes and the published column is zero outside the rows it changes,
+// so a thread applies the step to its entries without per-entry branches,
+// and a wave none of whose columns the step touches skips it (a uniform
+// branch).  Sixteen waves, four per SIMD, overlap each other's latencies.
+//
+// Elimination has no pivoting (the reference's gaussBksb exchanges rows only
+// for |pivot| <= 1e-9, src/gaussBksb.c:56-74, which these matrices never
+// reach).  The inverses are Gauss-Jordan sweeps run side by side in one loop:
+// forward on [L | I] (unit lower) and backward on [U | I]; at the end the
+// registers hold L^-1 and U^-1.  A block smaller than 64 is padded with the
+// identity, which decouples from the real part.
 #pragma once
 
 #include "dep_common.h"
@@ -30,96 +39,193 @@ __device__ __forceinline__ dcx<T> drecip(dcx<T> a)
     return {a.r / d, -a.i / d};
 }
 
-// D holds the block on entry and L\U on exit (L unit lower below the
-// diagonal); X is scratch.  Linv, Uinv: NBxNB complex interleaved outputs
-// (row-major, stride DIAG_NB).  Returns min |U_pp| through *piv (thread 0).
-// Must be called by all 256 threads of the block.
+// Entry k == kp of a register array (kp uniform); avoids dynamic register
+// indexing, which would go through scratch memory.
 template <class T>
-__device__ void diag_factor_invert(int nb, T (*Dr)[DIAG_NB + 1], T (*Di)[DIAG_NB + 1],
-                                   T (*Xr)[DIAG_NB + 1], T (*Xi)[DIAG_NB + 1], T* __restrict__ Linv,
-                                   T* __restrict__ Uinv, T* __restrict__ piv)
+__device__ __forceinline__ dcx<T> dpick(const T (&ar)[4], const T (&ai)[4], int kp)
 {
-    const int t = threadIdx.x;
-    // ---- LU: one barrier per step; multipliers staged in X
+    dcx<T> v{T(0), T(0)};
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        if (k == kp) v = {ar[k], ai[k]};
+    return v;
+}
+
+// x[k] -= m * (vr, vi)[4w + k] for the 4 owned columns (vr, vi 16-byte aligned)
+template <class T>
+__device__ __forceinline__ void drow_update(T (&xr)[4], T (&xi)[4], dcx<T> m, const T* vr, const T* vi, int w)
+{
+    typedef T v4 __attribute__((ext_vector_type(4)));
+    const v4 a = *(const v4*)(vr + 4 * w), b = *(const v4*)(vi + 4 * w);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        xr[k] -= m.r * a[k] - m.i * b[k];
+        xi[k] -= m.r * b[k] + m.i * a[k];
+    }
+}
+
+#define DIAG_THREADS 1024
+
+// Factor the nb x nb block at (k0, k0) of the split planes Zr/Zi (row
+// stride ldZ): L\U is written back in place, Linv and Uinv (DIAG_NB x DIAG_NB
+// complex interleaved, row-major) to inv and inv + 2 DIAG_NB^2, and the
+// smallest |U_pp| to *piv.  Called by all DIAG_THREADS threads of the block.
+template <class T>
+__device__ void diag_factor_invert(int nb, T* __restrict__ Zr, T* __restrict__ Zi, long ldZ, int k0,
+                                   T* __restrict__ inv, T* __restrict__ piv)
+{
+    __shared__ T Sr[DIAG_NB][DIAG_NB + 1], Si[DIAG_NB][DIAG_NB + 1];   // staging for coalesced I/O
+    __shared__ __align__(16) T ar_[2][DIAG_NB], ai_[2][DIAG_NB], br_[2][DIAG_NB], bi_[2][DIAG_NB];
+    __shared__ __align__(16) T cr_[2][DIAG_NB], ci_[2][DIAG_NB], er_[2][DIAG_NB], ei_[2][DIAG_NB];
+    __shared__ T ipr_[2], ipi_[2];
+    __shared__ T udr[DIAG_NB], udi[DIAG_NB];
+    const int t = threadIdx.x, r = t & 63;
+    const int w = __builtin_amdgcn_readfirstlane(t >> 6);   // wave index (uniform): columns 4w .. 4w+3
+
+    for (int idx = t; idx < DIAG_NB * DIAG_NB; idx += DIAG_THREADS) {
+        const int rr = idx >> 6, cc = idx & 63;
+        const bool in = rr < nb && cc < nb;
+        Sr[rr][cc] = in ? Zr[(long)(k0 + rr) * ldZ + k0 + cc] : T(rr == cc ? 1 : 0);
+        Si[rr][cc] = in ? Zi[(long)(k0 + rr) * ldZ + k0 + cc] : T(0);
+    }
+    __syncthreads();
+    T dr[4], di[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        dr[k] = Sr[r][4 * w + k];
+        di[k] = Si[r][4 * w + k];
+    }
+
+    // ---- LU.  Step p publishes row p right of the pivot (a), column p below it
+    // (b) and 1/pivot; rows r > p subtract l_r * a and store l_r in column p.
     for (int p = 0; p < nb - 1; p++) {
-        const dcx<T> ip = drecip(dcx<T>{Dr[p][p], Di[p][p]});
-        const int m = nb - 1 - p, w = nb - p;
-        for (int idx = t; idx < m * w; idx += 256) {
-            const int r = p + 1 + idx / w, c = p + idx % w;
-            const dcx<T> l = dmul(dcx<T>{Dr[r][p], Di[r][p]}, ip);
-            if (c == p) {
-                Xr[r][p] = l.r;
-                Xi[r][p] = l.i;
-            } else {
-                const dcx<T> u = dmul(l, dcx<T>{Dr[p][c], Di[p][c]});
-                Dr[r][c] -= u.r;
-                Di[r][c] -= u.i;
+        const int b = p & 1;
+        if (r == p) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int c = 4 * w + k;
+                ar_[b][c] = c > p ? dr[k] : T(0);
+                ai_[b][c] = c > p ? di[k] : T(0);
+            }
+            if (w == (p >> 2)) {
+                const dcx<T> ip = drecip(dpick(dr, di, p & 3));
+                ipr_[b] = ip.r;
+                ipi_[b] = ip.i;
             }
         }
+        if (w == (p >> 2)) {
+            const dcx<T> v = dpick(dr, di, p & 3);
+            br_[b][r] = r > p ? v.r : T(0);
+            bi_[b][r] = r > p ? v.i : T(0);
+        }
         __syncthreads();
-    }
-    for (int idx = t; idx < nb * nb; idx += 256) {
-        const int r = idx / nb, c = idx % nb;
-        if (r > c) {
-            Dr[r][c] = Xr[r][c];
-            Di[r][c] = Xi[r][c];
+        if (4 * w + 3 >= p) {
+            const dcx<T> l = dmul(dcx<T>{br_[b][r], bi_[b][r]}, dcx<T>{ipr_[b], ipi_[b]});
+            drow_update(dr, di, l, ar_[b], ai_[b], w);
+            if (r > p && w == (p >> 2)) {
+                const int kp = p & 3;
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    if (k == kp) {
+                        dr[k] = l.r;
+                        di[k] = l.i;
+                    }
+            }
         }
     }
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        if (4 * w + k == r) {
+            udr[r] = dr[k];
+            udi[r] = di[k];
+        }
+    __syncthreads();
     if (t == 0) {
-        T mn = Dr[0][0] * Dr[0][0] + Di[0][0] * Di[0][0];
-        for (int p = 1; p < nb; p++) mn = fmin(mn, Dr[p][p] * Dr[p][p] + Di[p][p] * Di[p][p]);
+        T mn = udr[0] * udr[0] + udi[0] * udi[0];
+        for (int p = 1; p < nb; p++) mn = fmin(mn, udr[p] * udr[p] + udi[p] * udi[p]);
         *piv = sqrt(mn);
     }
-    __syncthreads();
-    // ---- Linv: X = I; step p: X[r][j] -= L[r][p] X[p][j] (r > p, j <= p); row p final at step p
-    for (int idx = t; idx < DIAG_NB * DIAG_NB; idx += 256) {
-        const int r = idx / DIAG_NB, c = idx % DIAG_NB;
-        Xr[r][c] = (r == c) ? T(1) : T(0);
-        Xi[r][c] = T(0);
-        Linv[2 * idx] = (r == c) ? T(1) : T(0);
-        Linv[2 * idx + 1] = T(0);
+
+    // ---- inverses: forward sweep p = s on [L | I] (x), backward q = nb-1-s on [U | I] (y).
+    // Row p of x is final at step p and zero right of the diagonal; column p of L
+    // is published zero on rows <= p.  Row q of y is scaled by 1/U_qq (it is then
+    // final and zero left of the diagonal); column q of U is published zero on rows >= q.
+    T xr[4], xi[4], yr[4], yi[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        xr[k] = yr[k] = T(4 * w + k == r ? 1 : 0);
+        xi[k] = yi[k] = T(0);
     }
-    __syncthreads();
-    for (int p = 0; p < nb; p++) {
-        const int w = p + 1, m = nb - 1 - p;
-        for (int idx = t; idx < (m + 1) * w; idx += 256) {
-            const int r = p + idx / w, j = idx % w;
-            if (r == p) {
-                Linv[2 * (p * DIAG_NB + j)] = Xr[p][j];
-                Linv[2 * (p * DIAG_NB + j) + 1] = Xi[p][j];
-            } else {
-                const dcx<T> lx = dmul(dcx<T>{Dr[r][p], Di[r][p]}, dcx<T>{Xr[p][j], Xi[p][j]});
-                Xr[r][j] -= lx.r;
-                Xi[r][j] -= lx.i;
+    for (int s = 0; s < nb; s++) {
+        const int b = s & 1, p = s, q = nb - 1 - s;
+        if (r == p) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                ar_[b][4 * w + k] = xr[k];
+                ai_[b][4 * w + k] = xi[k];
             }
         }
-        __syncthreads();
-    }
-    // ---- Uinv: X = I; step p (descending): x = X[p][j] / U[p][p] is row p of the
-    // inverse; X[r][j] -= U[r][p] x for r < p
-    for (int idx = t; idx < DIAG_NB * DIAG_NB; idx += 256) {
-        const int r = idx / DIAG_NB, c = idx % DIAG_NB;
-        Xr[r][c] = (r == c) ? T(1) : T(0);
-        Xi[r][c] = T(0);
-        Uinv[2 * idx] = T(0);
-        Uinv[2 * idx + 1] = T(0);
-    }
-    __syncthreads();
-    for (int p = nb - 1; p >= 0; p--) {
-        const dcx<T> ip = drecip(dcx<T>{Dr[p][p], Di[p][p]});
-        const int w = nb - p;
-        for (int idx = t; idx < (p + 1) * w; idx += 256) {
-            const int r = idx / w, j = p + idx % w;
-            const dcx<T> x = dmul(dcx<T>{Xr[p][j], Xi[p][j]}, ip);
-            if (r == p) {
-                Uinv[2 * (p * DIAG_NB + j)] = x.r;
-                Uinv[2 * (p * DIAG_NB + j) + 1] = x.i;
-            } else {
-                const dcx<T> ux = dmul(dcx<T>{Dr[r][p], Di[r][p]}, x);
-                Xr[r][j] -= ux.r;
-                Xi[r][j] -= ux.i;
+        if (w == (p >> 2)) {
+            const dcx<T> v = dpick(dr, di, p & 3);
+            br_[b][r] = r > p ? v.r : T(0);
+            bi_[b][r] = r > p ? v.i : T(0);
+        }
+        if (r == q) {
+            const dcx<T> iq = drecip(dcx<T>{udr[q], udi[q]});
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const dcx<T> v = dmul(dcx<T>{yr[k], yi[k]}, iq);
+                yr[k] = v.r;
+                yi[k] = v.i;
+                cr_[b][4 * w + k] = v.r;
+                ci_[b][4 * w + k] = v.i;
             }
         }
+        if (w == (q >> 2)) {
+            const dcx<T> v = dpick(dr, di, q & 3);
+            er_[b][r] = r < q ? v.r : T(0);
+            ei_[b][r] = r < q ? v.i : T(0);
+        }
         __syncthreads();
+        if (4 * w <= p) drow_update(xr, xi, dcx<T>{br_[b][r], bi_[b][r]}, ar_[b], ai_[b], w);
+        if (4 * w + 3 >= q) drow_update(yr, yi, dcx<T>{er_[b][r], ei_[b][r]}, cr_[b], ci_[b], w);
+    }
+
+    // ---- write L^-1, U^-1 and L\U through the staging tile
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        Sr[r][4 * w + k] = xr[k];
+        Si[r][4 * w + k] = xi[k];
+    }
+    __syncthreads();
+    for (int idx = t; idx < DIAG_NB * DIAG_NB; idx += DIAG_THREADS) {
+        inv[2 * idx] = Sr[idx >> 6][idx & 63];
+        inv[2 * idx + 1] = Si[idx >> 6][idx & 63];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        Sr[r][4 * w + k] = yr[k];
+        Si[r][4 * w + k] = yi[k];
+    }
+    __syncthreads();
+    T* __restrict__ uinv = inv + 2 * DIAG_NB * DIAG_NB;
+    for (int idx = t; idx < DIAG_NB * DIAG_NB; idx += DIAG_THREADS) {
+        uinv[2 * idx] = Sr[idx >> 6][idx & 63];
+        uinv[2 * idx + 1] = Si[idx >> 6][idx & 63];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        Sr[r][4 * w + k] = dr[k];
+        Si[r][4 * w + k] = di[k];
+    }
+    __syncthreads();
+    for (int idx = t; idx < DIAG_NB * DIAG_NB; idx += DIAG_THREADS) {
+        const int rr = idx >> 6, cc = idx & 63;
+        if (rr < nb && cc < nb) {
+            Zr[(long)(k0 + rr) * ldZ + k0 + cc] = Sr[rr][cc];
+            Zi[(long)(k0 + rr) * ldZ + k0 + cc] = Si[rr][cc];
+        }
     }
 }

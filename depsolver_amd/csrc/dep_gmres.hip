@@ -13,22 +13,35 @@
 
 namespace {
 
-// h[j] = dot(V[j], w) for j < nvec; one workgroup per j
-__global__ __launch_bounds__(256) void k_dots(int nvec, long n2, const double* __restrict__ V,
-                                              const double* __restrict__ w, double* __restrict__ h)
+// Partial dot products: workgroup (c, j) sums V[j][e] w[e] over chunk c of
+// the vector (DOT_CHUNKS chunks); k_dots_reduce adds the chunks of each j in
+// a fixed order, so the result does not depend on scheduling.
+#define DOT_CHUNKS DEP_DOT_CHUNKS
+__global__ __launch_bounds__(256) void k_dots_partial(long n2, const double* __restrict__ V,
+                                                      const double* __restrict__ w,
+                                                      double* __restrict__ part)
 {
-    __shared__ double sh[256];
-    const int j = blockIdx.x;
+    __shared__ double sh[4];
+    const int c = blockIdx.x, j = blockIdx.y;
+    const long len = (n2 + DOT_CHUNKS - 1) / DOT_CHUNKS;
+    const long e0 = c * len, e1 = min(n2, e0 + len);
     const double* v = V + (long)j * n2;
     double s = 0.0;
-    for (long t = threadIdx.x; t < n2; t += 256) s += v[t] * w[t];
-    sh[threadIdx.x] = s;
+    for (long e = e0 + threadIdx.x; e < e1; e += 256) s += v[e] * w[e];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
     __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) h[j] = sh[0];
+    if (threadIdx.x == 0) part[(long)j * DOT_CHUNKS + c] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+
+__global__ void k_dots_reduce(int nvec, const double* __restrict__ part, double* __restrict__ h)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nvec) return;
+    double s = 0.0;
+    for (int c = 0; c < DOT_CHUNKS; c++) s += part[(long)j * DOT_CHUNKS + c];
+    h[j] = s;
 }
 
 // w -= sum_j h[j] V[j]
@@ -94,9 +107,12 @@ __global__ __launch_bounds__(256) void k_dense_zmatvec(int N, const double* __re
 
 }  // namespace
 
-hipError_t launch_dots(dep_ctx* c, int nvec, long n2, const double* V, const double* w, double* h)
+// h[j] = dot(V[j], w) for j < nvec (V rows n2 apart); part: DOT_CHUNKS * nvec scratch
+hipError_t launch_dots(dep_ctx* c, int nvec, long n2, const double* V, const double* w, double* h,
+                       double* part)
 {
-    hipLaunchKernelGGL(k_dots, dim3(nvec), dim3(256), 0, c->stream, nvec, n2, V, w, h);
+    hipLaunchKernelGGL(k_dots_partial, dim3(DOT_CHUNKS, nvec), dim3(256), 0, c->stream, n2, V, w, part);
+    hipLaunchKernelGGL(k_dots_reduce, dim3(dep_cdiv(nvec, 256)), dim3(256), 0, c->stream, nvec, part, h);
     return hipGetLastError();
 }
 

@@ -29,9 +29,14 @@ struct dep_kstat {
     double ms = 0.0, flops = 0.0, bytes = 0.0;
 };
 
+// panel width of the blocked LU (columns per outer step)
+#define DEP_LU_NBO 256
+
 struct dep_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t stream2 = nullptr;           // second stream of the LU's lookahead
+    hipEvent_t fork = nullptr, join = nullptr;
     std::string err;
     size_t bytes = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -48,8 +53,8 @@ struct dep_ctx {
 };
 
 // time one launch: prof_begin before, prof_end after (no-ops unless profiling)
-void prof_begin(dep_ctx* c, hipEvent_t* a);
-void prof_end(dep_ctx* c, hipEvent_t a, const char* name, double flops, double bytes);
+void prof_begin(dep_ctx* c, hipEvent_t* a, hipStream_t st = nullptr);
+void prof_end(dep_ctx* c, hipEvent_t a, const char* name, double flops, double bytes, hipStream_t st = nullptr);
 void prof_collect(dep_ctx* c);   // after a synchronize: fold finished events into stats
 
 struct dep_geom {
@@ -113,6 +118,10 @@ struct dep_sys {
     double* tbuf = nullptr;           // 2N vector the captured triangular solve works on
     hipGraphExec_t trsv_exec = nullptr;  // captured triangular-solve sequence
     int trsv_prec = -1;
+    // GMRES workspace: coefficient vectors (3N + 3 x 2N), basis (gm_cap + 1 vectors),
+    // reductions and dot-product partials
+    int gm_cap = 0;
+    double *gm_vec = nullptr, *gm_V = nullptr, *gm_red = nullptr;
     size_t bytes = 0;                 // device bytes owned by the system
     std::vector<std::pair<void*, size_t>> allocs;
     dep_ctx* ctx_of() { return g->ctx; }
@@ -159,13 +168,18 @@ hipError_t launch_residual(dep_sys* s, const double* x, double* r);   // r = b -
 hipError_t launch_matvec(dep_sys* s, const double* x, double* y);     // y = Z x
 hipError_t launch_zmatvec_w(dep_sys* s, const double* wr, const double* wi, const double* shift,
                             const double* x, const double* b, double* y);
-hipError_t launch_dots(dep_ctx* c, int nvec, long n2, const double* V, const double* w, double* h);
+// h[j] = dot(V[j], w), j < nvec; part: DEP_DOT_CHUNKS * nvec doubles of scratch
+#define DEP_DOT_CHUNKS 32
+hipError_t launch_dots(dep_ctx* c, int nvec, long n2, const double* V, const double* w, double* h,
+                       double* part);
 hipError_t launch_update(dep_ctx* c, int nvec, long n2, const double* V, const double* h, double* w);
 hipError_t launch_scale(dep_ctx* c, long n2, const double* w, double alpha, double* out);
 hipError_t launch_combine(dep_ctx* c, int nvec, long n2, const double* V, const double* y, double* x);
 size_t mixed_bytes(int N);
 hipError_t launch_lu_mixed(dep_sys* s, int bf16, double* min_pivot);
 hipError_t launch_trsv_mixed(dep_sys* s, const double* rhs, double* out);
+// reads and clears the triangular sweeps' wait-timeout flag (synchronises)
+hipError_t trsv_status(dep_sys* s, int* timed_out);
 hipError_t launch_axpy(dep_ctx* c, long n, const double* d, double* x);
 hipError_t launch_dense_zmatvec(dep_ctx* c, int N, const double* Zr, const double* Zi, long ldZ,
                                 const double* x, const double* b, double* y);

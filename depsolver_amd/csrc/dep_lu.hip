@@ -26,7 +26,7 @@
 #include "dep_internal.h"
 
 #define NBI DIAG_NB  // inner block (diagonal LU in LDS, inverse blocks)
-#define NBO 256    // outer panel width = trailing-update depth
+#define NBO DEP_LU_NBO   // outer panel width = trailing-update depth
 
 namespace {
 
@@ -108,28 +108,14 @@ __global__ void k_build_w(int N, const double* __restrict__ P, long ldP, const d
 }
 
 // ---- FP32 panel kernels -----------------------------------------------------
-// LU of the nb x nb diagonal block at (k0,k0) in LDS, then its inverses
-// Linv (unit lower) and Uinv (upper) into inv (2 x NBI x NBI complex)
+// LU of the nb x nb diagonal block at (k0,k0), then its inverses Linv (unit
+// lower) and Uinv (upper) into inv (2 x NBI x NBI complex)
 template <class T>
-__global__ __launch_bounds__(256) void k_diag(int nb, T* __restrict__ Zr, T* __restrict__ Zi,
+__global__ __launch_bounds__(DIAG_THREADS) void k_diag(int nb, T* __restrict__ Zr, T* __restrict__ Zi,
                                                 long ldZ, int k0, T* __restrict__ inv,
                                                 T* __restrict__ piv)
 {
-    __shared__ T Dr[NBI][NBI + 1], Di[NBI][NBI + 1];
-    __shared__ T Xr[NBI][NBI + 1], Xi[NBI][NBI + 1];
-    const int t = threadIdx.x;
-    for (int idx = t; idx < nb * nb; idx += 256) {
-        const int r = idx / nb, c = idx % nb;
-        Dr[r][c] = Zr[(long)(k0 + r) * ldZ + k0 + c];
-        Di[r][c] = Zi[(long)(k0 + r) * ldZ + k0 + c];
-    }
-    __syncthreads();
-    diag_factor_invert<T>(nb, Dr, Di, Xr, Xi, inv, inv + 2 * NBI * NBI, piv + k0 / NBI);
-    for (int idx = t; idx < nb * nb; idx += 256) {
-        const int r = idx / nb, c = idx % nb;
-        Zr[(long)(k0 + r) * ldZ + k0 + c] = Dr[r][c];
-        Zi[(long)(k0 + r) * ldZ + k0 + c] = Di[r][c];
-    }
+    diag_factor_invert<T>(nb, Zr, Zi, ldZ, k0, inv, piv + k0 / NBI);
 }
 
 // rows [r0, N): W[i, k0:k0+nb] <- W[i, k0:k0+nb] Uinv   (16 rows per block)
@@ -339,6 +325,135 @@ __global__ __launch_bounds__(256) void k_gemm_f64mfma(int ra, int rb, int ca, in
             }
 }
 
+// ---- complex GEMM on 16x16x4 MFMA (FP32 panels, FP64 factor) ----------------
+// Lane l supplies A[l%16][k = l/16] and B[k = l/16][l%16]; accumulator v of
+// lane l is row 4v + l/16 (FP64) or 4(l/16) + v (FP32) of the 16x16 block,
+// column l%16.
+template <class T>
+struct mfma16;
+template <>
+struct mfma16<double> {
+    typedef double acc __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ acc op(double a, double b, acc c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int v, int lane) { return 4 * v + (lane >> 4); }
+};
+template <>
+struct mfma16<float> {
+    typedef float acc __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ acc op(float a, float b, acc c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int v, int lane) { return 4 * (lane >> 4) + v; }
+};
+
+// W[ra:rb, ca:cb] -= W[ra:rb, kc:kc+nbk] W[kc:kc+nbk, ca:cb] (complex, split planes of
+// T): 64x64 complex tile per workgroup, 32x32 per wave = 2x2 blocks of 16x16 per plane,
+// Cr += Ar Br - Ai Bi and Ci += Ar Bi + Ai Br as four real MFMAs.  The next 16-deep
+// K chunk is loaded into registers while the current one feeds the MFMAs (two LDS
+// buffers, one barrier per chunk).
+template <class T>
+__global__ __launch_bounds__(256) void k_gemm_cmfma(int ra, int rb, int ca, int cb, int kc, int nbk,
+                                                    T* __restrict__ Zr, T* __restrict__ Zi, long ldZ)
+{
+    constexpr int BM = 64, BN = 64, BK = 16;
+    typedef typename mfma16<T>::acc acc;
+    __shared__ T Asr[2][BM][BK + 1], Asi[2][BM][BK + 1];
+    __shared__ T Bsr[2][BK][BN + 1], Bsi[2][BK][BN + 1];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    const int m0 = ra + blockIdx.y * BM, n0 = ca + blockIdx.x * BN;
+    const int li = lane & 15, lk = lane >> 4;
+    // staging: A rows t/4, k (t%4)*4 .. +3; B row k = t/16, columns (t%16)*4 .. +3
+    const int a_r = t >> 2, a_k = (t & 3) * 4, b_k = t >> 4, b_c = (t & 15) * 4;
+    T ga_r[4], ga_i[4], gb_r[4], gb_i[4];
+    auto gload = [&](int kk) {
+        const int gi = m0 + a_r;
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const bool ok = gi < rb && kk + a_k + e < nbk;
+            ga_r[e] = ok ? Zr[(long)gi * ldZ + kc + kk + a_k + e] : T(0);
+            ga_i[e] = ok ? Zi[(long)gi * ldZ + kc + kk + a_k + e] : T(0);
+        }
+        const bool kok = kk + b_k < nbk;
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int gc = n0 + b_c + e;
+            const bool ok = kok && gc < cb;
+            gb_r[e] = ok ? Zr[(long)(kc + kk + b_k) * ldZ + gc] : T(0);
+            gb_i[e] = ok ? Zi[(long)(kc + kk + b_k) * ldZ + gc] : T(0);
+        }
+    };
+    auto lstore = [&](int buf) {
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            Asr[buf][a_r][a_k + e] = ga_r[e];
+            Asi[buf][a_r][a_k + e] = ga_i[e];
+            Bsr[buf][b_k][b_c + e] = gb_r[e];
+            Bsi[buf][b_k][b_c + e] = gb_i[e];
+        }
+    };
+    acc cr[2][2], ci[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) cr[a][b][v] = ci[a][b][v] = T(0);
+    gload(0);
+    lstore(0);
+    __syncthreads();
+    int buf = 0;
+    for (int kk = 0; kk < nbk; kk += BK) {
+        const bool more = kk + BK < nbk;
+        if (more) gload(kk + BK);
+#pragma unroll
+        for (int ks = 0; ks < BK; ks += 4) {
+            T ar[2], ai[2], nai[2], br[2], bi[2];
+#pragma unroll
+            for (int a = 0; a < 2; a++) {
+                ar[a] = Asr[buf][wm + 16 * a + li][ks + lk];
+                ai[a] = Asi[buf][wm + 16 * a + li][ks + lk];
+                nai[a] = -ai[a];
+                br[a] = Bsr[buf][ks + lk][wn + 16 * a + li];
+                bi[a] = Bsi[buf][ks + lk][wn + 16 * a + li];
+            }
+#pragma unroll
+            for (int a = 0; a < 2; a++)
+#pragma unroll
+                for (int b = 0; b < 2; b++) {
+                    cr[a][b] = mfma16<T>::op(ar[a], br[b], cr[a][b]);
+                    ci[a][b] = mfma16<T>::op(ar[a], bi[b], ci[a][b]);
+                }
+#pragma unroll
+            for (int a = 0; a < 2; a++)
+#pragma unroll
+                for (int b = 0; b < 2; b++) {
+                    cr[a][b] = mfma16<T>::op(nai[a], bi[b], cr[a][b]);
+                    ci[a][b] = mfma16<T>::op(ai[a], br[b], ci[a][b]);
+                }
+        }
+        if (more) lstore(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+    }
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int gi = m0 + wm + 16 * a + mfma16<T>::row(v, lane), gc = n0 + wn + 16 * b + li;
+                if (gi < rb && gc < cb) {
+                    Zr[(long)gi * ldZ + gc] -= cr[a][b][v];
+                    Zi[(long)gi * ldZ + gc] -= ci[a][b][v];
+                }
+            }
+}
+
 // ---- low-precision operands and the MFMA trailing update --------------------
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
@@ -399,14 +514,14 @@ __global__ __launch_bounds__(256) void k_pack_b(int NC, int NCp, int nb, int K2p
     }
 }
 
-// C[r0+m][r0+n] -= sum_k A'[m][k] B'^T[n][k] on v_mfma_f32_32x32x16_{f16,bf16}.
+// C[r0+m][c0+n] -= sum_k A'[m][k] B'^T[n][k] on v_mfma_f32_32x32x16_{f16,bf16}.
 // 128x128 output tile per workgroup (4 waves of 64x64 = 2x2 MFMA blocks),
 // operands staged in LDS 32 deep.  Lane l of an MFMA block holds A rows l%32
 // with k = 8*(l/32)..+7, and accumulator v sits at row 8*(v/4) + 4*(l/32) + v%4,
 // column l%32.  Column tiles below NCp update the real plane, the rest the
 // imaginary plane.
 template <class H, class V8>
-__global__ __launch_bounds__(256) void k_gemm_mfma(int M, int NC, int NCp, int K2p, int r0,
+__global__ __launch_bounds__(256) void k_gemm_mfma(int M, int NC, int NCp, int K2p, int r0, int c0,
                                                    const H* __restrict__ A, const H* __restrict__ BT,
                                                    float* __restrict__ Zr, float* __restrict__ Zi,
                                                    long ldZ)
@@ -471,7 +586,7 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(int M, int NC, int NCp, int K
             for (int v = 0; v < 16; v++) {
                 const int m = m0 + wm + 32 * a + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
                 const int n = nb0 + wn + 32 * b + (lane & 31);
-                if (m < M && n < NC) C[(long)(r0 + m) * ldZ + r0 + n] -= acc[a][b][v];
+                if (m < M && n < NC) C[(long)(r0 + m) * ldZ + c0 + n] -= acc[a][b][v];
             }
 }
 
@@ -480,7 +595,7 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(int M, int NC, int NCp, int K
 // current buffer feeds the MFMAs, then stored into the other buffer, so each
 // chunk costs one barrier.
 template <class H, class V8>
-__global__ __launch_bounds__(256) void k_gemm_mfma3(int M, int NC, int NCp, int K2p, int r0,
+__global__ __launch_bounds__(256) void k_gemm_mfma3(int M, int NC, int NCp, int K2p, int r0, int c0,
                                                     const H* __restrict__ A, const H* __restrict__ BT,
                                                     float* __restrict__ Zr, float* __restrict__ Zi,
                                                     long ldZ)
@@ -558,73 +673,188 @@ __global__ __launch_bounds__(256) void k_gemm_mfma3(int M, int NC, int NCp, int 
             for (int v = 0; v < 16; v++) {
                 const int m = m0 + wm + 32 * a + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
                 const int n = nb0 + wn + 32 * b + (lane & 31);
-                if (m < M && n < NC) C[(long)(r0 + m) * ldZ + r0 + n] -= acc[a][b][v];
+                if (m < M && n < NC) C[(long)(r0 + m) * ldZ + c0 + n] -= acc[a][b][v];
             }
 }
 
-// ---- triangular solves with FP32 factors, FP64 vectors ---------------------
-template <class T>
-__global__ __launch_bounds__(256) void k_block_applyT(int nb, const T* __restrict__ Binv,
-                                                       double* __restrict__ vr, double* __restrict__ vi,
-                                                       int lower)
+// ---- sync-free blocked triangular sweep ------------------------------------
+// One workgroup per 64-row block.  Workgroups take their block from a ticket
+// counter in dependency order (forward: block 0 first; backward: the last
+// block first), so a workgroup only ever waits for blocks whose workgroups
+// are already running.  Block i accumulates W[i-block, j-block] x_j over the
+// blocks j it depends on as soon as each one's ready flag is set (agent-scope
+// release/acquire), then applies its inverse diagonal block:
+//   forward   x_i = Linv_ii (b_i - sum_{j<i} L_ij x_j)
+//   backward  x_i = Uinv_ii (y_i - sum_{j>i} U_ij x_j)
+// sync: [0] ticket, [1 .. nblk] ready flags (zeroed before the sweep); err
+// is set if a wait exceeds TRSV_SPIN_LIMIT polls (the sweep then finishes
+// with invalid values and the host reports the failure).
+#define TRSV_SPIN_LIMIT (1L << 22)
+
+// Flags are polled with relaxed agent-scope loads; the waiting thread issues
+// one acquire fence once the flags it needs are set (an acquire per poll would
+// invalidate the caches on every iteration of the wait).
+__device__ __forceinline__ int ld_relaxed(const int* p)
 {
-    __shared__ T Br[NBI][NBI + 1], Bi[NBI][NBI + 1];
-    __shared__ double xr[NBI], xi[NBI];
-    const int t = threadIdx.x;
-    for (int idx = t; idx < NBI * NBI; idx += 256) {
-        const int r = idx / NBI, c = idx % NBI;
-        Br[r][c] = Binv[2 * idx];
-        Bi[r][c] = Binv[2 * idx + 1];
-    }
-    if (t < nb) {
-        xr[t] = vr[t];
-        xi[t] = vi[t];
-    }
-    __syncthreads();
-    const int r = t >> 2, part = t & 3;
-    double sr = 0.0, si = 0.0;
-    if (r < nb) {
-        const int q0 = lower ? 0 : r, q1 = lower ? r : nb - 1;
-        for (int q = q0 + part; q <= q1; q += 4) {
-            const double br = Br[r][q], bi = Bi[r][q];
-            sr += br * xr[q] - bi * xi[q];
-            si += br * xi[q] + bi * xr[q];
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <class T>
+struct trsv_tile {   // this thread's 4 rows x 4 columns of one 64 x 64 block
+    T r[4][4], i[4][4];
+};
+
+// rows rs + 16m (m < 4) and columns 4 cg .. 4 cg + 3 of block (row0, col0);
+// entries outside the nbr x nbc block read as zero
+template <class T>
+__device__ __forceinline__ void trsv_load(trsv_tile<T>& o, const T* __restrict__ Wr, const T* __restrict__ Wi,
+                                          long ld, int row0, int col0, int nbr, int nbc, int rs, int cg)
+{
+    typedef T v4 __attribute__((ext_vector_type(4)));
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        const int rr = rs + 16 * m;
+        v4 a = {T(0), T(0), T(0), T(0)}, b = a;
+        if (rr < nbr) {
+            const long off = (long)(row0 + rr) * ld + col0 + 4 * cg;
+            a = *(const v4*)(Wr + off);
+            b = *(const v4*)(Wi + off);
         }
-    }
-    sr += __shfl_xor(sr, 1);
-    si += __shfl_xor(si, 1);
-    sr += __shfl_xor(sr, 2);
-    si += __shfl_xor(si, 2);
-    if (r < nb && part == 0) {
-        vr[r] = sr;
-        vi[r] = si;
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const bool ok = 4 * cg + e < nbc;
+            o.r[m][e] = ok ? a[e] : T(0);
+            o.i[m][e] = ok ? b[e] : T(0);
+        }
     }
 }
 
-// x[rows] -= W[rows, k0:k0+nb] x[k0:k0+nb]  for rows in [ra, rb): one wave per 4 rows
 template <class T>
-__global__ __launch_bounds__(256) void k_panel_gemvT(int ra, int rb, int nb, const T* __restrict__ Zr,
-                                                      const T* __restrict__ Zi, long ldZ, int k0,
-                                                      double* __restrict__ xr, double* __restrict__ xi)
+__global__ __launch_bounds__(256) void k_trsv_sweep(int N, int nblk, const T* __restrict__ Wr,
+                                                     const T* __restrict__ Wi, long ld,
+                                                     const T* __restrict__ inv, double* xr, double* xi,
+                                                     int lower, int* sync, int* err)
 {
-    const int lane = threadIdx.x & 63;
-    const int i = ra + blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (i >= rb) return;
-    double sr = 0.0, si = 0.0;
-    for (int q = lane; q < nb; q += 64) {
-        const double ar = Zr[(long)i * ldZ + k0 + q], ai = Zi[(long)i * ldZ + k0 + q];
-        sr += ar * xr[k0 + q] - ai * xi[k0 + q];
-        si += ar * xi[k0 + q] + ai * xr[k0 + q];
+    __shared__ T Br[NBI][NBI + 1], Bi[NBI][NBI + 1];
+    __shared__ double sr[NBI], si[NBI], outr[NBI], outi[NBI];
+    __shared__ int s_tk, s_done[2];   // s_done alternates per catch-up (one barrier apart)
+    const int t = threadIdx.x, cg = t & 15, rs = t >> 4;
+    if (t == 0) s_tk = atomicAdd(&sync[0], 1);
+    __syncthreads();
+    const int tk = s_tk;
+    const int ib = lower ? tk : nblk - 1 - tk;
+    const int i0 = ib * NBI, nbi = min(NBI, N - i0);
+    int* flags = sync + 1;
+    const T* B = inv + (size_t)ib * 4 * NBI * NBI + (lower ? 0 : 2 * NBI * NBI);
+    for (int idx = t; idx < NBI * NBI; idx += 256) {
+        Br[idx >> 6][idx & 63] = B[2 * idx];
+        Bi[idx >> 6][idx & 63] = B[2 * idx + 1];
     }
+    double ar[4] = {0.0, 0.0, 0.0, 0.0}, ai[4] = {0.0, 0.0, 0.0, 0.0};
+    trsv_tile<T> cur, nxt;
+    if (tk > 0) {
+        const int jb = lower ? 0 : nblk - 1;
+        trsv_load(cur, Wr, Wi, ld, i0, jb * NBI, nbi, min(NBI, N - jb * NBI), rs, cg);
+    }
+    int done = 0, catchup = 0;
+    for (int d = 0; d < tk; d++) {
+        const int jb = lower ? d : nblk - 1 - d;
+        const int j0 = jb * NBI, nbj = min(NBI, N - j0);
+        if (d + 1 < tk) {   // prefetch the next dependency block
+            const int jn = lower ? d + 1 : nblk - 2 - d;
+            trsv_load(nxt, Wr, Wi, ld, i0, jn * NBI, nbi, min(NBI, N - jn * NBI), rs, cg);
+        }
+        if (d >= done) {
+            if (t == 0) {
+                int dd = done;
+                long spins = 0;
+                while (dd < tk) {
+                    if (ld_relaxed(&flags[lower ? dd : nblk - 1 - dd])) {
+                        dd++;
+                        continue;
+                    }
+                    if (dd > d) break;
+                    if (++spins > TRSV_SPIN_LIMIT) {
+                        atomicOr(err, 1);
+                        dd = tk;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(2);
+                }
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                s_done[catchup & 1] = dd;
+            }
+            __syncthreads();   // orders every thread's reads of x after thread 0's acquire
+            done = s_done[catchup & 1];
+            catchup++;
+        }
+        double yr[4], yi[4];
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        sr += __shfl_xor(sr, o);
-        si += __shfl_xor(si, o);
+        for (int e = 0; e < 4; e++) {
+            const int c = j0 + min(4 * cg + e, nbj - 1);
+            yr[e] = xr[c];
+            yi[e] = xi[c];
+        }
+#pragma unroll
+        for (int m = 0; m < 4; m++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const double lr = cur.r[m][e], li = cur.i[m][e];
+                ar[m] += lr * yr[e] - li * yi[e];
+                ai[m] += lr * yi[e] + li * yr[e];
+            }
+        cur = nxt;
     }
-    if (lane == 0) {
-        xr[i] -= sr;
-        xi[i] -= si;
+    // row sums over the 16 lanes of a row group, then s = rhs - sum
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+#pragma unroll
+        for (int o = 1; o < 16; o <<= 1) {
+            ar[m] += __shfl_xor(ar[m], o);
+            ai[m] += __shfl_xor(ai[m], o);
+        }
+        const int rr = rs + 16 * m;
+        if (cg == 0) {
+            sr[rr] = rr < nbi ? xr[i0 + rr] - ar[m] : 0.0;
+            si[rr] = rr < nbi ? xi[i0 + rr] - ai[m] : 0.0;
+        }
     }
+    __syncthreads();
+    // x_i = B s  (B = Linv_ii or Uinv_ii; zero outside its triangle)
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        const int rr = rs + 16 * m;
+        double yr = 0.0, yi = 0.0;
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int q = 4 * cg + e;
+            const double br = Br[rr][q], bi = Bi[rr][q];
+            yr += br * sr[q] - bi * si[q];
+            yi += br * si[q] + bi * sr[q];
+        }
+#pragma unroll
+        for (int o = 1; o < 16; o <<= 1) {
+            yr += __shfl_xor(yr, o);
+            yi += __shfl_xor(yi, o);
+        }
+        if (cg == 0) {
+            outr[rr] = yr;
+            outi[rr] = yi;
+        }
+    }
+    __syncthreads();
+    if (t < nbi) {   // one wave writes the block's result
+        xr[i0 + t] = outr[t];
+        xi[i0 + t] = outi[t];
+    }
+    __syncthreads();
+    if (t == 0) __hip_atomic_store(&flags[ib], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ void k_fill_int(int n, int* __restrict__ p, int v)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
 }
 
 __global__ void k_vscale(int N, const double* __restrict__ s, const double* __restrict__ in,
@@ -655,97 +885,150 @@ __global__ void k_axpy(long n, const double* __restrict__ d, double* __restrict_
 // diagonal blocks (nblk x 2 x NBI x NBI complex T); piv (T) one entry per block.
 // mode 0: FP64 trailing update in place (T = double); 1/2: FP16/BF16 MFMA on
 // packed operands (T = float).
+//
+// Lookahead of one panel: after panel k is factored, the main stream updates
+// only the next panel's columns and factors that panel, while the second
+// stream computes the rest of U12 and the bulk of the trailing update.  The
+// two streams write disjoint parts of W (rows >= pe of the next panel's
+// columns, and rows k0.. of the columns beyond it); the main stream waits for
+// the second before the packed operands are rebuilt for the next panel.
 template <class T>
 static hipError_t lu_blocked(dep_sys* s, T* Wr, T* Wi, long ld, T* inv, T* piv, int mode)
 {
     dep_geom* g = s->g;
     dep_ctx* c = g->ctx;
-    hipStream_t st = c->stream;
+    hipStream_t st = c->stream, st2 = c->stream2;
     const int N = g->N;
     hipError_t e;
     const char* gsel = getenv("DEPSOLVER_MIXED_GEMM");
     const bool gemm_v1 = gsel && strcmp(gsel, "v1") == 0;
     const char* fsel = getenv("DEPSOLVER_FP64_GEMM");
     const bool f64_valu = fsel && strcmp(fsel, "valu") == 0;
-    for (int k0 = 0; k0 < N; k0 += NBO) {
-        const int nbo = std::min(NBO, N - k0), pe = k0 + nbo;
+    const char* psel = getenv("DEPSOLVER_PANEL_GEMM");
+    const bool panel_valu = psel && strcmp(psel, "valu") == 0;
+    // W[ra:rb, ca:cb] -= W[ra:rb, kc:kc+nbk] W[kc:kc+nbk, ca:cb] inside the panel and for U12
+    auto gemm_small = [&](int ra, int rb, int ca, int cb, int kc, int nbk, hipStream_t q) {
+        dim3 grid(dep_cdiv(cb - ca, 64), dep_cdiv(rb - ra, 64));
+        if (panel_valu)
+            hipLaunchKernelGGL(k_gemmT<T>, grid, dim3(256), 0, q, ra, rb, ca, cb, kc, nbk, Wr, Wi, ld);
+        else
+            hipLaunchKernelGGL(k_gemm_cmfma<T>, grid, dim3(256), 0, q, ra, rb, ca, cb, kc, nbk, Wr, Wi, ld);
+    };
+    // low-precision modes: A' = [Lr | Li] of the panel (N x 2 NBO), B' of the columns beyond
+    // the next panel (2 ld32 x 2 NBO) and B' of the next panel's columns (2 NBO x 2 NBO)
+    _Float16* Ah = mode ? (_Float16*)s->opB : nullptr;
+    _Float16* Bh = mode ? Ah + (size_t)N * 2 * NBO : nullptr;
+    _Float16* Bn = mode ? Bh + (size_t)2 * s->ld32 * 2 * NBO : nullptr;
+
+    // factor the panel [k0, k0 + nbo) for rows k0 .. N
+    auto panel = [&](int k0, hipStream_t q) {
+        const int pe = std::min(k0 + NBO, N);
         for (int kj = k0; kj < pe; kj += NBI) {
             const int nbi = std::min(NBI, pe - kj);
             T* invb = inv + (size_t)(kj / NBI) * 4 * NBI * NBI;
-            hipLaunchKernelGGL(k_diag<T>, dim3(1), dim3(256), 0, st, nbi, Wr, Wi, ld, kj, invb, piv);
+            hipLaunchKernelGGL(k_diag<T>, dim3(1), dim3(DIAG_THREADS), 0, q, nbi, Wr, Wi, ld, kj, invb, piv);
             if (kj + nbi < N)
-                hipLaunchKernelGGL(k_uinv<T>, dim3(dep_cdiv(N - kj - nbi, 16)), dim3(256), 0, st, N, nbi,
+                hipLaunchKernelGGL(k_uinv<T>, dim3(dep_cdiv(N - kj - nbi, 16)), dim3(256), 0, q, N, nbi,
                                    kj + nbi, Wr, Wi, ld, kj, invb + 2 * NBI * NBI);
             if (kj + nbi < pe) {
-                hipLaunchKernelGGL(k_linv<T>, dim3(dep_cdiv(pe - kj - nbi, 16)), dim3(256), 0, st, pe, nbi,
+                hipLaunchKernelGGL(k_linv<T>, dim3(dep_cdiv(pe - kj - nbi, 16)), dim3(256), 0, q, pe, nbi,
                                    kj + nbi, Wr, Wi, ld, kj, invb);
-                hipLaunchKernelGGL(k_gemmT<T>, dim3(dep_cdiv(pe - kj - nbi, 64), dep_cdiv(N - kj - nbi, 64)),
-                                   dim3(256), 0, st, kj + nbi, N, kj + nbi, pe, kj, nbi, Wr, Wi, ld);
+                gemm_small(kj + nbi, N, kj + nbi, pe, kj, nbi, q);
             }
         }
-        if (pe >= N) break;
-        // U12 = L11^-1 A12 by block forward substitution
+    };
+    // U12 = L11^-1 A12 for the columns [c0, c1) by block forward substitution
+    auto u12 = [&](int k0, int c0, int c1, hipStream_t q) {
+        const int pe = std::min(k0 + NBO, N);
         for (int kj = k0; kj < pe; kj += NBI) {
             const int nbi = std::min(NBI, pe - kj);
             const T* invb = inv + (size_t)(kj / NBI) * 4 * NBI * NBI;
-            hipLaunchKernelGGL(k_linv<T>, dim3(dep_cdiv(N - pe, 16)), dim3(256), 0, st, N, nbi, pe, Wr, Wi,
+            hipLaunchKernelGGL(k_linv<T>, dim3(dep_cdiv(c1 - c0, 16)), dim3(256), 0, q, c1, nbi, c0, Wr, Wi,
                                ld, kj, invb);
             if (kj + nbi < pe)
-                hipLaunchKernelGGL(k_gemmT<T>, dim3(dep_cdiv(N - pe, 64), dep_cdiv(pe - kj - nbi, 64)),
-                                   dim3(256), 0, st, kj + nbi, pe, pe, N, kj, nbi, Wr, Wi, ld);
+                gemm_small(kj + nbi, pe, c0, c1, kj, nbi, q);
         }
-        const int M = N - pe, NC = N - pe;
+    };
+    // A' of the panel (low-precision modes)
+    auto pack_a = [&](int k0, hipStream_t q) {
+        const int nbo = std::min(NBO, N - k0), pe = k0 + nbo, M = N - pe, K2p = (2 * nbo + 31) / 32 * 32;
+        const float* Fr = (const float*)Wr;
+        const float* Fi = (const float*)Wi;
+        if (mode == 2)
+            hipLaunchKernelGGL(k_pack_a<__bf16>, dim3(dep_cdiv(K2p, 256), M), dim3(256), 0, q, M, nbo, K2p, pe,
+                               k0, Fr, Fi, ld, (__bf16*)Ah);
+        else
+            hipLaunchKernelGGL(k_pack_a<_Float16>, dim3(dep_cdiv(K2p, 256), M), dim3(256), 0, q, M, nbo, K2p, pe,
+                               k0, Fr, Fi, ld, Ah);
+    };
+    // trailing update of rows pe .. N, columns [c0, c1) by the panel at k0
+    auto trail = [&](int k0, int c0, int c1, _Float16* B, hipStream_t q) {
+        const int nbo = std::min(NBO, N - k0), pe = k0 + nbo, M = N - pe, NC = c1 - c0;
         hipEvent_t ev;
         if (mode == 0) {
-            prof_begin(c, &ev);
+            prof_begin(c, &ev, q);
             dim3 grid(dep_cdiv(NC, 64), dep_cdiv(M, 64));
             if (f64_valu)
-                hipLaunchKernelGGL(k_gemmT<T>, grid, dim3(256), 0, st, pe, N, pe, N, k0, nbo, Wr, Wi, ld);
+                hipLaunchKernelGGL(k_gemmT<T>, grid, dim3(256), 0, q, pe, N, c0, c1, k0, nbo, Wr, Wi, ld);
+            else if (panel_valu)
+                hipLaunchKernelGGL(k_gemm_f64mfma, grid, dim3(256), 0, q, pe, N, c0, c1, k0, nbo, (double*)Wr,
+                                   (double*)Wi, ld);
             else
-                hipLaunchKernelGGL(k_gemm_f64mfma, grid, dim3(256), 0, st, pe, N, pe, N, k0, nbo,
-                                   (double*)Wr, (double*)Wi, ld);
-            prof_end(c, ev, "lu_gemm_fp64", 8.0 * M * (double)NC * nbo, 32.0 * M * (double)NC + 32.0 * nbo * (M + NC));
-        } else {
-            // low-precision operands and the MFMA trailing update
-            const int NCp = (NC + 127) / 128 * 128, K2p = (2 * nbo + 31) / 32 * 32;
-            _Float16* Ah = (_Float16*)s->opB;
-            _Float16* Bh = Ah + (size_t)N * 2 * NBO;
-            const float* Fr = (const float*)Wr;
-            const float* Fi = (const float*)Wi;
-            if (mode == 2) {
-                hipLaunchKernelGGL(k_pack_a<__bf16>, dim3(dep_cdiv(K2p, 256), M), dim3(256), 0, st, M, nbo, K2p,
-                                   pe, k0, Fr, Fi, ld, (__bf16*)Ah);
-                hipLaunchKernelGGL(k_pack_b<__bf16>, dim3(dep_cdiv(NCp, 64)), dim3(256), 0, st, NC, NCp, nbo,
-                                   K2p, k0, pe, Fr, Fi, ld, (__bf16*)Bh);
-            } else {
-                hipLaunchKernelGGL(k_pack_a<_Float16>, dim3(dep_cdiv(K2p, 256), M), dim3(256), 0, st, M, nbo,
-                                   K2p, pe, k0, Fr, Fi, ld, Ah);
-                hipLaunchKernelGGL(k_pack_b<_Float16>, dim3(dep_cdiv(NCp, 64)), dim3(256), 0, st, NC, NCp, nbo,
-                                   K2p, k0, pe, Fr, Fi, ld, Bh);
-            }
-            prof_begin(c, &ev);
-            dim3 grid(2 * NCp / 128, dep_cdiv(M, 128));
-            float* Cr = (float*)Wr;
-            float* Ci = (float*)Wi;
-            if (mode == 2) {
-                if (gemm_v1)
-                    hipLaunchKernelGGL((k_gemm_mfma<__bf16, bf16x8>), grid, dim3(256), 0, st, M, NC, NCp, K2p,
-                                       pe, (const __bf16*)Ah, (const __bf16*)Bh, Cr, Ci, ld);
-                else
-                    hipLaunchKernelGGL((k_gemm_mfma3<__bf16, bf16x8>), grid, dim3(256), 0, st, M, NC, NCp, K2p,
-                                       pe, (const __bf16*)Ah, (const __bf16*)Bh, Cr, Ci, ld);
-            } else {
-                if (gemm_v1)
-                    hipLaunchKernelGGL((k_gemm_mfma<_Float16, f16x8>), grid, dim3(256), 0, st, M, NC, NCp, K2p,
-                                       pe, (const _Float16*)Ah, (const _Float16*)Bh, Cr, Ci, ld);
-                else
-                    hipLaunchKernelGGL((k_gemm_mfma3<_Float16, f16x8>), grid, dim3(256), 0, st, M, NC, NCp,
-                                       K2p, pe, (const _Float16*)Ah, (const _Float16*)Bh, Cr, Ci, ld);
-            }
-            // complex flops 8 M NC nbo; bytes: C read+write FP32 complex + FP16 operands
-            prof_end(c, ev, "lu_gemm_mixed", 8.0 * M * (double)NC * nbo,
-                     16.0 * M * (double)NC + 2.0 * K2p * (M + 2.0 * NCp));
+                hipLaunchKernelGGL(k_gemm_cmfma<T>, grid, dim3(256), 0, q, pe, N, c0, c1, k0, nbo, Wr, Wi, ld);
+            prof_end(c, ev, "lu_gemm_fp64", 8.0 * M * (double)NC * nbo,
+                     32.0 * M * (double)NC + 32.0 * nbo * (M + NC), q);
+            return;
         }
+        const int NCp = (NC + 127) / 128 * 128, K2p = (2 * nbo + 31) / 32 * 32;
+        const float* Fr = (const float*)Wr;
+        const float* Fi = (const float*)Wi;
+        if (mode == 2)
+            hipLaunchKernelGGL(k_pack_b<__bf16>, dim3(dep_cdiv(NCp, 64)), dim3(256), 0, q, NC, NCp, nbo, K2p, k0,
+                               c0, Fr, Fi, ld, (__bf16*)B);
+        else
+            hipLaunchKernelGGL(k_pack_b<_Float16>, dim3(dep_cdiv(NCp, 64)), dim3(256), 0, q, NC, NCp, nbo, K2p,
+                               k0, c0, Fr, Fi, ld, B);
+        prof_begin(c, &ev, q);
+        dim3 grid(2 * NCp / 128, dep_cdiv(M, 128));
+        float* Cr = (float*)Wr;
+        float* Ci = (float*)Wi;
+        if (mode == 2) {
+            if (gemm_v1)
+                hipLaunchKernelGGL((k_gemm_mfma<__bf16, bf16x8>), grid, dim3(256), 0, q, M, NC, NCp, K2p, pe, c0,
+                                   (const __bf16*)Ah, (const __bf16*)B, Cr, Ci, ld);
+            else
+                hipLaunchKernelGGL((k_gemm_mfma3<__bf16, bf16x8>), grid, dim3(256), 0, q, M, NC, NCp, K2p, pe, c0,
+                                   (const __bf16*)Ah, (const __bf16*)B, Cr, Ci, ld);
+        } else {
+            if (gemm_v1)
+                hipLaunchKernelGGL((k_gemm_mfma<_Float16, f16x8>), grid, dim3(256), 0, q, M, NC, NCp, K2p, pe, c0,
+                                   (const _Float16*)Ah, (const _Float16*)B, Cr, Ci, ld);
+            else
+                hipLaunchKernelGGL((k_gemm_mfma3<_Float16, f16x8>), grid, dim3(256), 0, q, M, NC, NCp, K2p, pe,
+                                   c0, (const _Float16*)Ah, (const _Float16*)B, Cr, Ci, ld);
+        }
+        // complex flops 8 M NC nbo; bytes: C read+write FP32 complex + FP16 operands
+        prof_end(c, ev, "lu_gemm_mixed", 8.0 * M * (double)NC * nbo, 16.0 * M * (double)NC + 2.0 * K2p * (M + 2.0 * NCp),
+                 q);
+    };
+
+    panel(0, st);
+    for (int k0 = 0;; k0 += NBO) {
+        const int pe = std::min(k0 + NBO, N);
+        if (pe >= N) break;
+        const int pn = std::min(pe + NBO, N);   // the next panel is [pe, pn)
+        u12(k0, pe, pn, st);
+        if (mode) pack_a(k0, st);
+        if (pn < N) {
+            if ((e = hipEventRecord(c->fork, st)) != hipSuccess) return e;
+            if ((e = hipStreamWaitEvent(st2, c->fork, 0)) != hipSuccess) return e;
+            u12(k0, pn, N, st2);
+            trail(k0, pn, N, Bh, st2);
+            if ((e = hipEventRecord(c->join, st2)) != hipSuccess) return e;
+        }
+        trail(k0, pe, pn, Bn, st);
+        panel(pe, st);
+        if (pn < N && (e = hipStreamWaitEvent(st, c->join, 0)) != hipSuccess) return e;
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     return hipGetLastError();
@@ -765,6 +1048,12 @@ static hipError_t min_pivot_of(dep_ctx* c, const T* piv, int nblk, double* out)
 }
 
 // out = Dc U^-1 L^-1 Dr rhs (Dr, Dc optional); split FP64 vectors; out may alias rhs
+// Sync words of the triangular sweeps, in the reduction scratch s->dwork
+// (65536 doubles; the first half holds pivots and dot products): two
+// (ticket, flags[nblk]) groups, then the sticky error word.
+static const size_t TRSV_SYNC_OFF = 49152;   // in doubles
+static const int TRSV_SYNC_CAP = 2 * (65536 - 49152);   // in ints
+
 template <class T>
 static hipError_t trsv_blocked(dep_sys* s, const T* Wr, const T* Wi, long ld, const T* inv, const double* rs,
                                const double* cs, const double* rhs, double* out)
@@ -773,30 +1062,39 @@ static hipError_t trsv_blocked(dep_sys* s, const T* Wr, const T* Wi, long ld, co
     const int N = g->N;
     hipStream_t st = g->ctx->stream;
     const int nblk = dep_cdiv(N, NBI);
+    if (2 * (nblk + 1) + 1 > TRSV_SYNC_CAP) return hipErrorInvalidValue;
+    int* sync = (int*)(s->dwork + TRSV_SYNC_OFF);
+    int* err = sync + 2 * (nblk + 1);
     double* xr = out;
     double* xi = out + N;
     if (rs)
         hipLaunchKernelGGL(k_vscale, dim3(dep_cdiv(N, 256)), dim3(256), 0, st, N, rs, rhs, out);
     else if (out != rhs)
         hipLaunchKernelGGL(k_copy, dim3(dep_cdiv(2L * N, 256)), dim3(256), 0, st, 2L * N, rhs, out);
-    for (int kb = 0; kb < nblk; kb++) {
-        const int k0 = kb * NBI, nb = std::min(NBI, N - k0);
-        hipLaunchKernelGGL(k_block_applyT<T>, dim3(1), dim3(256), 0, st, nb, inv + (size_t)kb * 4 * NBI * NBI,
-                           xr + k0, xi + k0, 1);
-        if (k0 + nb < N)
-            hipLaunchKernelGGL(k_panel_gemvT<T>, dim3(dep_cdiv(N - k0 - nb, 4)), dim3(256), 0, st, k0 + nb, N,
-                               nb, Wr, Wi, ld, k0, xr, xi);
-    }
-    for (int kb = nblk - 1; kb >= 0; kb--) {
-        const int k0 = kb * NBI, nb = std::min(NBI, N - k0);
-        hipLaunchKernelGGL(k_block_applyT<T>, dim3(1), dim3(256), 0, st, nb,
-                           inv + (size_t)kb * 4 * NBI * NBI + 2 * NBI * NBI, xr + k0, xi + k0, 0);
-        if (k0 > 0)
-            hipLaunchKernelGGL(k_panel_gemvT<T>, dim3(dep_cdiv(k0, 4)), dim3(256), 0, st, 0, k0, nb, Wr, Wi, ld,
-                               k0, xr, xi);
-    }
+    hipLaunchKernelGGL(k_fill_int, dim3(dep_cdiv(2 * (nblk + 1), 256)), dim3(256), 0, st, 2 * (nblk + 1), sync, 0);
+    hipLaunchKernelGGL(k_trsv_sweep<T>, dim3(nblk), dim3(256), 0, st, N, nblk, Wr, Wi, ld, inv, xr, xi, 1, sync,
+                       err);
+    hipLaunchKernelGGL(k_trsv_sweep<T>, dim3(nblk), dim3(256), 0, st, N, nblk, Wr, Wi, ld, inv, xr, xi, 0,
+                       sync + nblk + 1, err);
     if (cs) hipLaunchKernelGGL(k_vscale, dim3(dep_cdiv(N, 256)), dim3(256), 0, st, N, cs, out, out);
     return hipGetLastError();
+}
+
+// Reads and clears the sticky error word of the triangular sweeps (synchronises).
+hipError_t trsv_status(dep_sys* s, int* timed_out)
+{
+    dep_geom* g = s->g;
+    const int nblk = dep_cdiv(g->N, NBI);
+    int* err = (int*)(s->dwork + TRSV_SYNC_OFF) + 2 * (nblk + 1);
+    hipError_t e;
+    if ((e = hipMemcpyAsync(timed_out, err, sizeof(int), hipMemcpyDeviceToHost, g->ctx->stream)) != hipSuccess)
+        return e;
+    if ((e = hipStreamSynchronize(g->ctx->stream)) != hipSuccess) return e;
+    if (*timed_out) {
+        const int zero = 0;
+        if ((e = hipMemcpy(err, &zero, sizeof(int), hipMemcpyHostToDevice)) != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 // ---- entry points -----------------------------------------------------------
