@@ -74,6 +74,7 @@ def lib():
         "dep_factor": ([_p, i], i),
         "dep_solve": ([_p, d, i, _p, ctypes.POINTER(SolveInfo)], i),
         "dep_gmres": ([_p, i, d, i, _p, _p, ctypes.POINTER(SolveInfo), ctypes.c_char_p], i),
+        "dep_gmres_ir": ([_p, d, d, i, i, _p, ctypes.POINTER(SolveInfo)], i),
         "dep_get_solution": ([_p, _f64p], i),
         "dep_set_solution": ([_p, _f64p], i),
         "dep_post_potential": ([_p, i, _f64p, _f64p], i),
@@ -194,6 +195,14 @@ class System:
         self.ctx.check(lib().dep_gmres(self._h, precond, tol, max_iters, x0p, sigma.ctypes.data,
                                        ctypes.byref(info), None if log_path is None else log_path.encode()),
                        "dep_gmres")
+        return sigma, info.as_dict()
+
+    def gmres_ir(self, tol=1e-12, inner_tol=1e-6, max_inner=200, max_outer=10):
+        """Refined FP32 GMRES: FP64 residuals, FP32-operator GMRES corrections."""
+        sigma = np.zeros(2 * self.n)
+        info = SolveInfo()
+        self.ctx.check(lib().dep_gmres_ir(self._h, tol, inner_tol, max_inner, max_outer, sigma.ctypes.data,
+                                          ctypes.byref(info)), "dep_gmres_ir")
         return sigma, info.as_dict()
 
     def solution(self):

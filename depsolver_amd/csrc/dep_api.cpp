@@ -9,12 +9,26 @@
 #include <string.h>
 
 #include <algorithm>
+#include <memory>
 #include <numeric>
 #include <unordered_map>
 
 #include "dep_common.h"
 #include "dep_internal.h"
 #include "dep_post.h"
+
+// The Maxwell-stress evaluation set of a geometry (bc-6 nodes, force elements,
+// their exclusion lists and the device buffers), built on the first
+// dep_post_mst and kept with the geometry so that later calls allocate nothing.
+struct dep_mst_cache {
+    int m = 0, nf = 0;
+    std::vector<int> fel;          // force elements: first node has bc type 6 (force.c:519)
+    double Xcm[3] = {0.0, 0.0, 0.0};
+    dep_points P;
+    int *dnodes = nullptr, *dfel = nullptr;
+    double *Efn = nullptr, *res = nullptr;   // res: Fce (3 nf), Xce (3 nf), Fcm (3)
+};
+
 
 static const double kC = 1.0 / (DEP_PI4 * DEP_EPS0);
 
@@ -96,6 +110,8 @@ extern "C" int dep_ctx_create(int device, dep_ctx** out)
         dep_ctx_destroy(c);
         return DEP_ERR_HIP;
     }
+    if (hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || c->ncu <= 0)
+        c->ncu = 256;
     *out = c;
     return DEP_OK;
 }
@@ -312,14 +328,19 @@ extern "C" int dep_geom_create(dep_ctx* c, int N, const double* xyz, int E, int 
                 if (first) inc_ek[fill[v]++] = e * 8 + k;
             }
     }
-    // column patches
-    g->n_patch = dep_cdiv(N, DEP_TC);
+    // column patches of tc consecutive (Morton-ordered) nodes
+    {
+        const char* tsel = getenv("DEPSOLVER_ASM_TC");
+        const int tc = tsel ? atoi(tsel) : DEP_TC;
+        g->tc = (tc == 16 || tc == 32 || tc == 64) ? tc : DEP_TC;
+    }
+    g->n_patch = dep_cdiv(N, g->tc);
     std::vector<int> pptr(g->n_patch + 1, 0), pel;
     std::vector<int8_t> pslot;
     {
         std::vector<int> mark(E, -1);
         for (int p = 0; p < g->n_patch; p++) {
-            const int c0 = p * DEP_TC, c1 = std::min(N, c0 + DEP_TC);
+            const int c0 = p * g->tc, c1 = std::min(N, c0 + g->tc);
             std::vector<int> els;
             for (int v = c0; v < c1; v++)
                 for (int t = cnt[v]; t < cnt[v + 1]; t++) {
@@ -382,6 +403,7 @@ extern "C" int dep_geom_destroy(dep_geom* g)
     if (!g) return DEP_OK;
     (void)hipStreamSynchronize(g->ctx->stream);
     free_all(g);
+    delete g->mst;
     delete g;
     return DEP_OK;
 }
@@ -390,6 +412,7 @@ extern "C" int dep_geom_reassemble(dep_geom* g)
 {
     if (!g) return DEP_ERR_ARG;
     g->assembled = false;
+    g->p32_valid = false;
     return dep_geom_assemble(g);
 }
 
@@ -888,13 +911,15 @@ extern "C" int dep_post_field(dep_sys* s, int m, const double* pts, double* ef)
     return eval_points(s, 1, m, pts, ef);
 }
 
-extern "C" int dep_post_mst(dep_sys* s, double eps, double* Fce, double* Xce, double* Fcm, double* Xcm)
+static int mst_cache(dep_geom* g, dep_mst_cache** out)
 {
-    if (!s || !Fce || !Xce || !Fcm || !Xcm) return DEP_ERR_ARG;
-    dep_geom* g = s->g;
+    if (g->mst) {
+        *out = g->mst;
+        return DEP_OK;
+    }
     dep_ctx* c = g->ctx;
     const int N = g->N, E = g->E, npe = g->npe;
-    // bc-6 nodes (internal ids) and their coordinates
+    std::unique_ptr<dep_mst_cache> mc(new dep_mst_cache());
     std::vector<int> nodes;
     std::vector<double> pts;
     for (int i = 0; i < N; i++)
@@ -902,63 +927,64 @@ extern "C" int dep_post_mst(dep_sys* s, double eps, double* Fce, double* Xce, do
             nodes.push_back(i);
             for (int k = 0; k < 3; k++) pts.push_back(g->hxyz[3 * i + k]);
         }
-    const int m = (int)nodes.size();
-    if (m == 0) return dep_fail(c, DEP_ERR_ARG, "dep_post_mst: no node has bc type 6");
+    mc->m = (int)nodes.size();
+    if (mc->m == 0) return dep_fail(c, DEP_ERR_ARG, "dep_post_mst: no node has bc type 6");
     // Xcm: mean of the bc-6 nodes in the caller's node order (force.c:493-507)
-    Xcm[0] = Xcm[1] = Xcm[2] = 0.0;
     for (int o = 0; o < N; o++) {
         const int i = g->iperm[o];
         if (!g->hbc6[i]) continue;
-        for (int k = 0; k < 3; k++) Xcm[k] += g->hxyz[3 * i + k];
+        for (int k = 0; k < 3; k++) mc->Xcm[k] += g->hxyz[3 * i + k];
     }
-    for (int k = 0; k < 3; k++) Xcm[k] = Xcm[k] / m;
-    // force elements: first node has bc type 6 (force.c:519)
-    std::vector<int> fel;
+    for (int k = 0; k < 3; k++) mc->Xcm[k] = mc->Xcm[k] / mc->m;
     for (int e = 0; e < E; e++)
-        if (g->hbc6[g->hconn[(size_t)e * npe]]) fel.push_back(e);
-    const int nf = (int)fel.size();
-    dep_points P;
-    DEP_CHECK(dep_points_prepare(g, m, pts.data(), &P));
-    struct Tmp {
-        dep_ctx* c;
-        std::vector<std::pair<void*, size_t>> allocs;
-        dep_ctx* ctx_of() { return c; }
-    } t{c, {}};
-    int* dnodes = nullptr;
-    int* dfel = nullptr;
-    double *Efn = nullptr, *dF = nullptr, *dX = nullptr, *dcm = nullptr;
-    int r = DEP_OK;
-    if (r == DEP_OK) r = dep_obj_alloc(&t, &dnodes, (size_t)m);
-    if (r == DEP_OK) r = dep_obj_alloc(&t, &dfel, (size_t)std::max(nf, 1));
-    if (r == DEP_OK) r = dep_obj_alloc(&t, &Efn, 3 * (size_t)N);
-    if (r == DEP_OK) r = dep_obj_alloc(&t, &dF, 3 * (size_t)std::max(nf, 1));
-    if (r == DEP_OK) r = dep_obj_alloc(&t, &dX, 3 * (size_t)std::max(nf, 1));
-    if (r == DEP_OK) r = dep_obj_alloc(&t, &dcm, 3);
-    if (r == DEP_OK) r = upload(c, dnodes, nodes.data(), nodes.size());
-    if (r == DEP_OK && nf) r = upload(c, dfel, fel.data(), fel.size());
-    if (r == DEP_OK) {
-        hipError_t e = hipMemsetAsync(Efn, 0, sizeof(double) * 3 * N, c->stream);
-        if (e == hipSuccess) e = launch_sigma_at_points(s);
-        if (e == hipSuccess) e = launch_points(s, 1, &P, P.out);
-        if (e == hipSuccess) e = launch_mst(s, &P, P.out, dnodes, nf, dfel, 0.5 * eps, Efn, dF, dX, dcm);
-        if (e != hipSuccess) r = dep_fail(c, DEP_ERR_HIP, "MST kernels: %s", hipGetErrorString(e));
-    }
-    std::vector<double> hF(3 * (size_t)std::max(nf, 1)), hX(3 * (size_t)std::max(nf, 1));
-    if (r == DEP_OK) r = download(c, hF.data(), dF, hF.size());
-    if (r == DEP_OK) r = download(c, hX.data(), dX, hX.size());
-    if (r == DEP_OK) r = download(c, Fcm, dcm, 3);
-    if (r == DEP_OK) {
-        memset(Fce, 0, sizeof(double) * 3 * E);
-        memset(Xce, 0, sizeof(double) * 3 * E);
-        for (int k = 0; k < nf; k++)
-            for (int q = 0; q < 3; q++) {
-                Fce[3 * (size_t)fel[k] + q] = hF[3 * k + q];
-                Xce[3 * (size_t)fel[k] + q] = hX[3 * k + q];
-            }
-    }
-    for (auto& a : t.allocs) dep_free(c, a.first, a.second);
-    dep_points_release(&P);
-    return r;
+        if (g->hbc6[g->hconn[(size_t)e * npe]]) mc->fel.push_back(e);
+    mc->nf = (int)mc->fel.size();
+    const size_t nf1 = std::max(mc->nf, 1);
+    DEP_CHECK(dep_points_prepare(g, mc->m, pts.data(), &mc->P));
+    // the point set's buffers live as long as the geometry
+    g->allocs.insert(g->allocs.end(), mc->P.allocs.begin(), mc->P.allocs.end());
+    mc->P.allocs.clear();
+    DEP_CHECK(dep_obj_alloc(g, &mc->dnodes, (size_t)mc->m));
+    DEP_CHECK(dep_obj_alloc(g, &mc->dfel, nf1));
+    DEP_CHECK(dep_obj_alloc(g, &mc->Efn, 3 * (size_t)N));
+    DEP_CHECK(dep_obj_alloc(g, &mc->res, 6 * nf1 + 3));
+    DEP_CHECK(upload(c, mc->dnodes, nodes.data(), nodes.size()));
+    if (mc->nf) DEP_CHECK(upload(c, mc->dfel, mc->fel.data(), mc->fel.size()));
+    g->mst = mc.release();
+    *out = g->mst;
+    return DEP_OK;
+}
+
+extern "C" int dep_post_mst(dep_sys* s, double eps, double* Fce, double* Xce, double* Fcm, double* Xcm)
+{
+    if (!s || !Fce || !Xce || !Fcm || !Xcm) return DEP_ERR_ARG;
+    dep_geom* g = s->g;
+    dep_ctx* c = g->ctx;
+    const int N = g->N, E = g->E;
+    dep_mst_cache* mc = nullptr;
+    DEP_CHECK(mst_cache(g, &mc));
+    const int nf = mc->nf;
+    const size_t nf1 = std::max(nf, 1);
+    for (int k = 0; k < 3; k++) Xcm[k] = mc->Xcm[k];
+    double* dF = mc->res;
+    double* dX = dF + 3 * nf1;
+    double* dcm = dX + 3 * nf1;
+    hipError_t e = hipMemsetAsync(mc->Efn, 0, sizeof(double) * 3 * N, c->stream);
+    if (e == hipSuccess) e = launch_sigma_at_points(s);
+    if (e == hipSuccess) e = launch_points(s, 1, &mc->P, mc->P.out);
+    if (e == hipSuccess) e = launch_mst(s, &mc->P, mc->P.out, mc->dnodes, nf, mc->dfel, 0.5 * eps, mc->Efn, dF, dX, dcm);
+    if (e != hipSuccess) return dep_fail(c, DEP_ERR_HIP, "MST kernels: %s", hipGetErrorString(e));
+    std::vector<double> h(6 * nf1 + 3);
+    DEP_CHECK(download(c, h.data(), mc->res, h.size()));
+    memset(Fce, 0, sizeof(double) * 3 * E);
+    memset(Xce, 0, sizeof(double) * 3 * E);
+    for (int k = 0; k < nf; k++)
+        for (int q = 0; q < 3; q++) {
+            Fce[3 * (size_t)mc->fel[k] + q] = h[3 * k + q];
+            Xce[3 * (size_t)mc->fel[k] + q] = h[3 * nf1 + 3 * k + q];
+        }
+    for (int k = 0; k < 3; k++) Fcm[k] = h[6 * nf1 + k];
+    return DEP_OK;
 }
 
 // Multipolar DEP force on a sphere of radius R at each point (force.c:572-825):
@@ -1076,6 +1102,17 @@ static void sys_release(dep_sys* s, void* p)
         }
 }
 
+// FP32 copy of the geometric blocks (N x ldP), made on demand after each assembly
+static int ensure_p32(dep_geom* g)
+{
+    if (g->P32 && g->p32_valid) return DEP_OK;
+    DEP_CHECK(dep_geom_assemble(g));
+    if (!g->P32) DEP_CHECK(dep_obj_alloc(g, &g->P32, (size_t)g->N * g->ldP));
+    DEP_HIP(g->ctx, launch_to_f32(g->ctx, (long)g->N * g->ldP, g->P, g->P32));
+    g->p32_valid = true;
+    return DEP_OK;
+}
+
 static int ensure_gmres_storage(dep_sys* s, int max_iters)
 {
     if (s->gm_V && s->gm_cap >= max_iters) return DEP_OK;
@@ -1093,102 +1130,75 @@ static int ensure_gmres_storage(dep_sys* s, int max_iters)
     return DEP_OK;
 }
 
-extern "C" int dep_gmres(dep_sys* s, int precond, double tol_rel, int max_iters, const double* x0,
-                         double* sigma, dep_solve_info* info, const char* log_path)
+// Jacobi scaling of GMRES: d_i = 1 / Z_ii's real part (the reference divides row
+// i by A[i][i], gmres_el.c:87-93); the scaled weights d w_r, d w_i, d shift go
+// to gw, gwi, gsh on the device and d to the host vector.
+static int jacobi_weights(dep_sys* s, int precond, std::vector<double>& d, double* gw, double* gwi, double* gsh)
 {
-    if (!s || max_iters <= 0 || !(tol_rel > 0.0)) return DEP_ERR_ARG;
     dep_geom* g = s->g;
     dep_ctx* c = g->ctx;
     const int N = g->N;
-    const long n2 = 2L * N;
-    // Jacobi scaling: row i (both halves) and b[i] by 1/A[i][i], not b[i+N] (gmres_el.c:87-93)
-    std::vector<double> d(N, 1.0);
+    d.assign(N, 1.0);
     if (precond == 1) {
-        if (s->dense) return dep_fail(c, DEP_ERR_ARG, "dep_gmres: scale a dense system before the call");
+        // diagonal of P gathered on the device, then one contiguous copy
         std::vector<double> pd(N);
-        DEP_HIP(c, hipMemcpy2DAsync(pd.data(), sizeof(double), g->P, sizeof(double) * (g->ldP + 1),
-                                    sizeof(double), N, hipMemcpyDeviceToHost, c->stream));
-        DEP_HIP(c, hipStreamSynchronize(c->stream));
+        DEP_HIP(c, launch_diag_gather(c, N, g->P, g->ldP, gw));
+        DEP_CHECK(download(c, pd.data(), gw, N));
         for (int i = 0; i < N; i++) d[i] = 1.0 / (s->hwr[i] * pd[i] + s->hshift[i]);
     }
-    DEP_CHECK(ensure_gmres_storage(s, max_iters));
-    double* gw = s->gm_vec;
-    double* gwi = gw + N;
-    double* gsh = gwi + N;
-    double* b = gsh + N;
-    double* x = b + n2;
-    double* w = x + n2;
+    std::vector<double> a(N), bi(N), sh(N);
+    for (int i = 0; i < N; i++) {
+        a[i] = d[i] * s->hwr[i];
+        bi[i] = d[i] * s->hwi[i];
+        sh[i] = d[i] * s->hshift[i];
+    }
+    DEP_CHECK(upload(c, gw, a.data(), N));
+    DEP_CHECK(upload(c, gwi, bi.data(), N));
+    DEP_CHECK(upload(c, gsh, sh.data(), N));
+    return DEP_OK;
+}
+
+// One unrestarted GMRES run on the device (Krylov basis in s->gm_V, CGS2
+// orthogonalisation, Givens rotations and the small triangular solve on the
+// host, gmres_el.c:140-260): solves Op x = b from the x given (x_zero: from 0)
+// until the residual estimate drops below tol (absolute) or after M steps.
+// rc: 0 converged, 1 iteration limit, 2 stagnation (gmres_el.c:241).
+struct GmresRun {
+    int iters = 0, rc = 1;
+    double beta = 0.0, err = 0.0;
+};
+template <class MV>
+static int gmres_run(dep_sys* s, MV matvec, const double* b, bool x_zero, double* x, double* w, double tol, int M,
+                     FILE* fg, GmresRun* out)
+{
+    dep_ctx* c = s->g->ctx;
+    const long n2 = 2L * s->g->N;
     double* V = s->gm_V;
-    // reductions: [0] |w|^2 before, [1] after orthogonalisation, then the two CGS passes'
-    // coefficients (cap + 1 each), then the dot-product partials
     const int cap = s->gm_cap;
     double* red = s->gm_red;
     double* h1 = red + 2;
     double* h2 = h1 + (cap + 1);
     double* part = h2 + (cap + 1);
-    double* h = h1;
-    std::vector<double> hb(n2);
-    DEP_CHECK(download(c, hb.data(), s->b, n2));
-    if (!s->dense) {
-        std::vector<double> a(N), bi(N), sh(N);
-        for (int i = 0; i < N; i++) {
-            a[i] = d[i] * s->hwr[i];
-            bi[i] = d[i] * s->hwi[i];
-            sh[i] = d[i] * s->hshift[i];
-            hb[i] = hb[i] * d[i];
-        }
-        DEP_CHECK(upload(c, gw, a.data(), N));
-        DEP_CHECK(upload(c, gwi, bi.data(), N));
-        DEP_CHECK(upload(c, gsh, sh.data(), N));
-    }
-    DEP_CHECK(upload(c, b, hb.data(), n2));
-    auto matvec = [&](const double* in, const double* rhs, double* out) -> hipError_t {
-        if (s->dense) return launch_dense_zmatvec(c, N, s->Zr, s->Zi, s->ldZ, in, rhs, out);
-        return launch_zmatvec_w(s, gw, gwi, gsh, in, rhs, out);
-    };
-    auto norm = [&](const double* v, double* out) -> int {
-        DEP_HIP(c, launch_dots(c, 1, n2, v, v, red, part));
-        double hv = 0.0;
-        DEP_CHECK(download(c, &hv, red, 1));
-        *out = sqrt(hv);
-        return DEP_OK;
-    };
-    double bnorm = 0.0;
-    for (double v : hb) bnorm += v * v;
-    const double tol = sqrt(bnorm) * tol_rel;
-    FILE* fg = log_path ? fopen(log_path, "w") : nullptr;
-    if (fg) {
-        fprintf(fg, "\n\nRELATIVE TOLERANCE = %10.4e\n", tol);
-        fprintf(fg, "\n%10s%21s\n", "ITER #    ", "RELATIVE RESIDUAL");
-        fprintf(fg, "%10s%21s\n", "***************", "*******************");
-    }
-    DEP_HIP(c, hipEventRecord(c->ev0, c->stream));
-    // initial residual (gmres_el.c:272-294)
-    double xn = 0.0;
-    if (x0) {
-        std::vector<double> hx(n2);
-        for (int i = 0; i < N; i++) {
-            hx[i] = x0[g->perm[i]];
-            hx[N + i] = x0[N + g->perm[i]];
-        }
-        DEP_CHECK(upload(c, x, hx.data(), n2));
-        for (double v : hx) xn += v * v;
-    } else {
+    if (M > cap) return dep_fail(c, DEP_ERR_ARG, "gmres_run: %d iterations exceed the workspace", M);
+    if (x_zero) {
         DEP_HIP(c, hipMemsetAsync(x, 0, sizeof(double) * n2, c->stream));
+        DEP_HIP(c, hipMemcpyAsync(V, b, sizeof(double) * n2, hipMemcpyDeviceToDevice, c->stream));
+    } else {
+        DEP_HIP(c, matvec(x, b, V));   // r0 = b - Op x0 (gmres_el.c:272-294)
     }
-    if (xn != 0.0) DEP_HIP(c, matvec(x, b, V));
-    else DEP_HIP(c, hipMemcpyAsync(V, b, sizeof(double) * n2, hipMemcpyDeviceToDevice, c->stream));
     double beta = 0.0;
-    DEP_CHECK(norm(V, &beta));
-    DEP_HIP(c, launch_scale(c, n2, V, 1.0 / beta, V));
-    const int M = max_iters;
-    std::vector<double> H((size_t)(M + 1) * (M + 1), 0.0), cs(M + 1, 0.0), sn(M + 1, 0.0),
-        res(M + 2, 0.0), hh(2 + 2 * (size_t)(cap + 1));
+    DEP_HIP(c, launch_dots(c, 1, n2, V, V, red, part));
+    DEP_CHECK(download(c, &beta, red, 1));
+    beta = sqrt(beta);
+    out->beta = beta;
+    std::vector<double> H((size_t)(M + 1) * (M + 1), 0.0), cs(M + 1, 0.0), sn(M + 1, 0.0), res(M + 2, 0.0),
+        hh(2 + 2 * (size_t)(cap + 1));
 #define HH(r, col) H[(size_t)(r) * (M + 1) + (col)]
     res[0] = beta;
     double err0 = beta, err1 = beta;
     int iters = 0, rc = 1;
     if (err0 < tol) rc = 0;
+    if (rc == 1 && beta > 0.0) DEP_HIP(c, launch_scale(c, n2, V, 1.0 / beta, V));
     for (int i = 0; rc == 1 && i < M; i++) {
         iters++;
         double* vi = V + (size_t)i * n2;
@@ -1207,10 +1217,10 @@ extern "C" int dep_gmres(dep_sys* s, int precond, double tol_rel, int max_iters,
         HH(i + 1, i) = hn;
         if (hn != 0.0) DEP_HIP(c, launch_scale(c, n2, w, 1.0 / hn, V + (size_t)(i + 1) * n2));
         for (int j = 0; j <= i - 1; j++) {
-            const double h1 = cs[j] * HH(j, i) + sn[j] * HH(j + 1, i);
-            const double h2 = -sn[j] * HH(j, i) + cs[j] * HH(j + 1, i);
-            HH(j, i) = h1;
-            HH(j + 1, i) = h2;
+            const double g1 = cs[j] * HH(j, i) + sn[j] * HH(j + 1, i);
+            const double g2 = -sn[j] * HH(j, i) + cs[j] * HH(j + 1, i);
+            HH(j, i) = g1;
+            HH(j + 1, i) = g2;
         }
         const double htmp = HH(i + 1, i) / HH(i, i);
         if (wn != 0.0) {
@@ -1232,31 +1242,165 @@ extern "C" int dep_gmres(dep_sys* s, int precond, double tol_rel, int max_iters,
             for (int j = i + 1; j <= iters; j++) res[i] -= HH(i, j) * res[j];
             res[i] = res[i] / HH(i, i);
         }
-        DEP_CHECK(upload(c, h, res.data(), iters));
-        DEP_HIP(c, launch_combine(c, iters, n2, V, h, x));
+        DEP_CHECK(upload(c, h1, res.data(), iters));
+        DEP_HIP(c, launch_combine(c, iters, n2, V, h1, x));
     }
 #undef HH
+    out->iters = iters;
+    out->rc = rc;
+    out->err = err1;
+    return DEP_OK;
+}
+
+extern "C" int dep_gmres(dep_sys* s, int precond, double tol_rel, int max_iters, const double* x0,
+                         double* sigma, dep_solve_info* info, const char* log_path)
+{
+    if (!s || max_iters <= 0 || !(tol_rel > 0.0)) return DEP_ERR_ARG;
+    dep_geom* g = s->g;
+    dep_ctx* c = g->ctx;
+    const int N = g->N;
+    const long n2 = 2L * N;
+    if (precond == 1 && s->dense) return dep_fail(c, DEP_ERR_ARG, "dep_gmres: scale a dense system before the call");
+    DEP_CHECK(ensure_gmres_storage(s, max_iters));
+    double* gw = s->gm_vec;
+    double* gwi = gw + N;
+    double* gsh = gwi + N;
+    double* b = gsh + N;
+    double* x = b + n2;
+    double* w = x + n2;
+    // Jacobi scaling of the rows and of b[i] only, not b[i+N] (gmres_el.c:87-93)
+    std::vector<double> d;
+    std::vector<double> hb(n2);
+    DEP_CHECK(download(c, hb.data(), s->b, n2));
+    if (!s->dense) {
+        DEP_CHECK(jacobi_weights(s, precond, d, gw, gwi, gsh));
+        for (int i = 0; i < N; i++) hb[i] = hb[i] * d[i];
+    }
+    DEP_CHECK(upload(c, b, hb.data(), n2));
+    auto matvec = [&](const double* in, const double* rhs, double* out) -> hipError_t {
+        if (s->dense) return launch_dense_zmatvec(c, N, s->Zr, s->Zi, s->ldZ, in, rhs, out);
+        return launch_zmatvec_w(s, gw, gwi, gsh, in, rhs, out);
+    };
+    double bnorm = 0.0;
+    for (double v : hb) bnorm += v * v;
+    const double tol = sqrt(bnorm) * tol_rel;
+    FILE* fg = log_path ? fopen(log_path, "w") : nullptr;
+    if (fg) {
+        fprintf(fg, "\n\nRELATIVE TOLERANCE = %10.4e\n", tol);
+        fprintf(fg, "\n%10s%21s\n", "ITER #    ", "RELATIVE RESIDUAL");
+        fprintf(fg, "%10s%21s\n", "***************", "*******************");
+    }
+    DEP_HIP(c, hipEventRecord(c->ev0, c->stream));
+    bool x_zero = true;
+    if (x0) {   // warm start (solution.init, gmres_el.c:272-294)
+        std::vector<double> hx(n2);
+        double xn = 0.0;
+        for (int i = 0; i < N; i++) {
+            hx[i] = x0[g->perm[i]];
+            hx[N + i] = x0[N + g->perm[i]];
+        }
+        for (double v : hx) xn += v * v;
+        DEP_CHECK(upload(c, x, hx.data(), n2));
+        x_zero = xn == 0.0;
+    }
+    GmresRun run;
+    const int r = gmres_run(s, matvec, b, x_zero, x, w, tol, max_iters, fg, &run);
+    if (r != DEP_OK) {
+        if (fg) fclose(fg);
+        return r;
+    }
     DEP_HIP(c, hipEventRecord(c->ev1, c->stream));
     DEP_HIP(c, hipEventSynchronize(c->ev1));
     float ms = 0.0f;
     DEP_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     if (fg) {
-        if (rc == 0) fprintf(fg, "\n\n***PROBLEM CONVERGES IN %d ITERATIONS***\n\n", iters);
-        else fprintf(fg, "Problem fails to vonverge to given tolerance in %d iterations\n\n", M);
+        if (run.rc == 0) fprintf(fg, "\n\n***PROBLEM CONVERGES IN %d ITERATIONS***\n\n", run.iters);
+        else fprintf(fg, "Problem fails to vonverge to given tolerance in %d iterations\n\n", max_iters);
         fclose(fg);
     }
-    if (rc == 0) {
+    if (run.rc == 0) {
         DEP_HIP(c, hipMemcpyAsync(s->x, x, sizeof(double) * n2, hipMemcpyDeviceToDevice, c->stream));
         if (sigma) DEP_CHECK(dep_get_solution(s, sigma));
     }
     if (info) {
         memset(info, 0, sizeof(*info));
-        info->iterations = iters;
-        info->converged = rc == 0;
-        info->residual = (bnorm > 0 ? err1 / sqrt(bnorm) : err1);
+        info->iterations = run.iters;
+        info->converged = run.rc == 0;
+        info->residual = (bnorm > 0 ? run.err / sqrt(bnorm) : run.err);
         info->solve_ms = ms;
     }
-    if (rc == 2) return dep_fail(c, DEP_ERR_NOCONV, "GMRES: CONVERGENCE RATE TOO SLOW after %d iterations", iters);
-    if (rc == 1) return dep_fail(c, DEP_ERR_NOCONV, "GMRES: no convergence in %d iterations", M);
+    if (run.rc == 2)
+        return dep_fail(c, DEP_ERR_NOCONV, "GMRES: CONVERGENCE RATE TOO SLOW after %d iterations", run.iters);
+    if (run.rc == 1) return dep_fail(c, DEP_ERR_NOCONV, "GMRES: no convergence in %d iterations", max_iters);
+    return DEP_OK;
+}
+
+// Refined FP32 GMRES: FP64 residuals r = b - Z x from the FP64 blocks P; each
+// correction solves D Z32 d = D r by GMRES on the FP32 copy of P (half the bytes
+// per product) with the Jacobi scaling D of dep_gmres applied to both halves of
+// r, to the relative tolerance inner_tol; x += d until |r| <= tol |b|.
+extern "C" int dep_gmres_ir(dep_sys* s, double tol_rel, double inner_tol, int max_inner, int max_outer,
+                            double* sigma, dep_solve_info* info)
+{
+    if (!s || max_inner <= 0 || max_outer <= 0 || !(tol_rel > 0.0) || !(inner_tol > 0.0)) return DEP_ERR_ARG;
+    dep_geom* g = s->g;
+    dep_ctx* c = g->ctx;
+    if (s->dense) return dep_fail(c, DEP_ERR_ARG, "dep_gmres_ir: needs the geometric blocks (not a dense system)");
+    const int N = g->N;
+    const long n2 = 2L * N;
+    DEP_CHECK(ensure_gmres_storage(s, max_inner));
+    DEP_CHECK(ensure_p32(g));
+    double* gw = s->gm_vec;
+    double* gwi = gw + N;
+    double* gsh = gwi + N;
+    double* rs = gsh + N;      // scaled residual (inner right-hand side)
+    double* d = rs + n2;       // inner solution
+    double* w = d + n2;
+    double* r = s->work;       // FP64 residual
+    double* dsc = s->work + n2;   // Jacobi factors on the device (N)
+    std::vector<double> hd;
+    DEP_CHECK(jacobi_weights(s, 1, hd, gw, gwi, gsh));
+    DEP_CHECK(upload(c, dsc, hd.data(), N));
+    auto matvec32 = [&](const double* in, const double* rhs, double* out) -> hipError_t {
+        return launch_zmatvec32_w(s, gw, gwi, gsh, in, rhs, out);
+    };
+    DEP_HIP(c, hipEventRecord(c->ev0, c->stream));
+    double bn = 0.0, res = 0.0;
+    DEP_CHECK(dev_norm(s, s->b, &bn));
+    if (bn == 0.0) bn = 1.0;
+    DEP_HIP(c, hipMemsetAsync(s->x, 0, sizeof(double) * n2, c->stream));
+    int outer = 0, inner_total = 0;
+    bool stalled = false;
+    for (;;) {
+        DEP_HIP(c, launch_residual(s, s->x, r));
+        DEP_CHECK(dev_norm(s, r, &res));
+        res /= bn;
+        if (!(res > tol_rel) || outer >= max_outer || stalled) break;
+        DEP_HIP(c, launch_rowscale(c, N, dsc, r, rs));
+        double rsn = 0.0;
+        DEP_CHECK(dev_norm(s, rs, &rsn));
+        GmresRun run;
+        DEP_CHECK(gmres_run(s, matvec32, rs, true, d, w, inner_tol * rsn, max_inner, nullptr, &run));
+        inner_total += run.iters;
+        if (run.iters == 0) stalled = true;
+        DEP_HIP(c, launch_axpy(c, n2, d, s->x));
+        outer++;
+    }
+    DEP_HIP(c, hipEventRecord(c->ev1, c->stream));
+    DEP_HIP(c, hipEventSynchronize(c->ev1));
+    float ms = 0.0f;
+    DEP_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    if (info) {
+        memset(info, 0, sizeof(*info));
+        info->iterations = outer;
+        info->inner_iterations = inner_total;
+        info->residual = res;
+        info->converged = !(res > tol_rel);
+        info->solve_ms = ms;
+    }
+    if (sigma) DEP_CHECK(dep_get_solution(s, sigma));
+    if (!std::isfinite(res)) return dep_fail(c, DEP_ERR_SINGULAR, "dep_gmres_ir: non-finite residual");
+    if (res > tol_rel)
+        return dep_fail(c, DEP_ERR_NOCONV, "dep_gmres_ir: residual %.3e after %d refinements", res, outer);
     return DEP_OK;
 }

@@ -89,7 +89,7 @@ struct ShapeTable {
     }
 };
 
-template <int NPE, int TR>
+template <int NPE, int TR, int TC>
 __global__ __launch_bounds__(TR) void k_assemble_regular(
     int N, const double* __restrict__ xyz, const int* __restrict__ conn,
     const int8_t* __restrict__ kind, const double* __restrict__ normal,
@@ -98,13 +98,13 @@ __global__ __launch_bounds__(TR) void k_assemble_regular(
     double* __restrict__ P, long ldP)
 {
     constexpr ShapeTable<NPE> SF{};
-    __shared__ double acc[DEP_TC][TR + 1];
+    __shared__ double acc[TC][TR + 1];
     const int patch = blockIdx.x;
     const int t = threadIdx.x;
     const int row = blockIdx.y * TR + t;
     const bool active = row < N;
 #pragma unroll 8
-    for (int c = 0; c < DEP_TC; c++) acc[c][t] = 0.0;
+    for (int c = 0; c < TC; c++) acc[c][t] = 0.0;
 
     double xe0 = 0.0, xe1 = 0.0, xe2 = 0.0, n0 = 0.0, n1 = 0.0, n2 = 0.0;
     bool iface = false;
@@ -154,10 +154,10 @@ __global__ __launch_bounds__(TR) void k_assemble_regular(
         }
     }
     __syncthreads();
-    // coalesced write-out of the TR x DEP_TC tile
-    const int c0 = patch * DEP_TC;
-    constexpr int RPP = TR / DEP_TC;   // rows per pass
-    const int cc = t % DEP_TC, rr = t / DEP_TC;
+    // coalesced write-out of the TR x TC tile
+    const int c0 = patch * TC;
+    constexpr int RPP = TR / TC;   // rows per pass
+    const int cc = t % TC, rr = t / TC;
     for (int r = rr; r < TR; r += RPP) {
         const int gi = blockIdx.y * TR + r, gc = c0 + cc;
         if (gi < N && gc < N) P[(long)gi * ldP + gc] = acc[cc][r];
@@ -219,9 +219,18 @@ hipError_t assemble_impl(dep_geom* g)
     dim3 grid(g->n_patch, dep_cdiv(g->N, TR));
     hipEvent_t ev;
     prof_begin(g->ctx, &ev);
-    hipLaunchKernelGGL((k_assemble_regular<NPE, TR>), grid, dim3(TR), 0, st, g->N, g->xyz, g->conn,
-                       g->kind, g->normal, (const double4*)g->q, g->patch_ptr, g->patch_elem,
-                       g->patch_slot, g->P, g->ldP);
+    if (g->tc == 16)
+        hipLaunchKernelGGL((k_assemble_regular<NPE, TR, 16>), grid, dim3(TR), 0, st, g->N, g->xyz, g->conn,
+                           g->kind, g->normal, (const double4*)g->q, g->patch_ptr, g->patch_elem,
+                           g->patch_slot, g->P, g->ldP);
+    else if (g->tc == 32)
+        hipLaunchKernelGGL((k_assemble_regular<NPE, TR, 32>), grid, dim3(TR), 0, st, g->N, g->xyz, g->conn,
+                           g->kind, g->normal, (const double4*)g->q, g->patch_ptr, g->patch_elem,
+                           g->patch_slot, g->P, g->ldP);
+    else
+        hipLaunchKernelGGL((k_assemble_regular<NPE, TR, 64>), grid, dim3(TR), 0, st, g->N, g->xyz, g->conn,
+                           g->kind, g->normal, (const double4*)g->q, g->patch_ptr, g->patch_elem,
+                           g->patch_slot, g->P, g->ldP);
     // algorithmic flops per (row, element) pair of the 7-point rule, sqrt and
     // division counted as one flop each: 7 (10 + 2 npe) for G rows, 7 (18 + 2 npe)
     // for dG/dn rows; bytes: the N x N FP64 blocks written once

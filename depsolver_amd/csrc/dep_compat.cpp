@@ -1,10 +1,11 @@
 // dep_compat.cpp -- drop-in shims with the reference's signatures.
 //
 // A driver with src/depSolver.c's flow links these instead of the reference's
-// assembly_el.o, gaussBksb.o and gmres_el.o:
+// assembly_el.o, gaussBksb.o, gmres_el.o and postProcess.o:
 //   electricForm[Full]A_tria{3,6}  src/assembly_el.c:63, :197, :329, :470
 //   gaussBksb                      src/gaussBksb.c:39
 //   solverGMRES_el                 src/gmres_el.c:66
+//   [vtk]postProcess_tria{3,6}     src/postProcess.c:256, :372, :488, :626
 // They read the sizes from the executable's globals (src/depSolver.c:46-50),
 // flatten the row-pointer arrays, run the HIP path and write back in the
 // reference's layouts.  The last assembled system stays on the device, so
@@ -24,6 +25,8 @@ extern "C" {
 extern unsigned int nNodes __attribute__((weak));
 extern unsigned int nElems __attribute__((weak));
 extern unsigned int nInterfaces __attribute__((weak));
+extern unsigned int nInternalPoints __attribute__((weak));
+extern unsigned int nFPoints __attribute__((weak));
 extern FILE* file_log __attribute__((weak));
 }
 
@@ -299,4 +302,131 @@ extern "C" void solverGMRES_el(unsigned int preCond, unsigned int nInit, double*
         fatal("CONVERGENCE RATE TOO SLOW");
     }
     check(rc, "dep_gmres");
+}
+
+namespace {
+
+// postProcess_tria{3,6} (src/postProcess.c:256-470, headers :58-116): potential
+// and field at the internal points, the Maxwell-stress force (ANALYSIS 3, 4) or
+// the multipolar force at the force points (ANALYSIS 5-9), written in the
+// reference's formats.  Runs on the system of the preceding electricForm* call
+// with vB as its solution.
+int post(int npe, unsigned ANALYSIS, const char* type, const double* axis, double** XF, double** Xinner,
+         double** vMatParam, const unsigned* vBCType, unsigned** mElems, const double* vB)
+{
+    if (!st.sys) fatal("postProcess: no assembled system on the device (electricForm* runs first)");
+    if (!type || strcmp(type, "STD") != 0) fatal("postProcess: only the STD output type is available");
+    if (ANALYSIS == 10 || ANALYSIS == 11)
+        fatal("postProcess: the dipolar ellipsoid force (ANALYSIS 10/11) is not available in this build");
+    dep_sys* sy = st.sys;
+    const int E = st.geom->E;
+    check(dep_set_solution(sy, vB), "dep_set_solution");
+    FILE* fp[3] = {nullptr, nullptr, nullptr};
+    if (ANALYSIS == 0) {
+        fp[0] = fopen("potential.dat", "w");
+        fprintf(fp[0], "#x\t\ty\t\tz\t\tRe[Pot(x,y,z)]\tIm[Pot(x,y,z)]\n");
+    } else if (ANALYSIS == 1) {
+        fp[1] = fopen("field.dat", "w");
+        fprintf(fp[1], "#x\t\ty\t\tz\t\tRe[Ex(x,y,z)]\tIm[Ex(x,y,z)]\tRe[Ey(x,y,z)]");
+        fprintf(fp[1], "\tIm[Ey(x,y,z)]\tRe[Ez(x,y,z)]\tIm[Ez(x,y,z)]\n");
+    } else {
+        fp[0] = fopen("potential.dat", "w");
+        fp[1] = fopen("field.dat", "w");
+        fprintf(fp[0], "#x\t\ty\t\tz\t\tRe[Pot(x,y,z)]\tIm[Pot(x,y,z)]\n");
+        fprintf(fp[1], "#x\t\ty\t\tz\t\tRe[Ex(x,y,z)] \tIm[Ex(x,y,z)]\tRe[Ey(x,y,z)]");
+        fprintf(fp[1], "\tIm[Ey(x,y,z)] \tRe[Ez(x,y,z)]\tIm[Ez(x,y,z)]\n");
+    }
+    if (ANALYSIS == 3 || ANALYSIS == 4) {
+        fp[2] = fopen("force-mst.dat", "w");
+        fprintf(fp[2], "#Xcm\t\tYcm\t\tZcm\t\tFXcm\t\tFYcm\t\tFZcm\n");
+    } else if (ANALYSIS > 4) {
+        fp[2] = fopen("force-mp.dat", "w");
+        fprintf(fp[2], "#Xcm\t\tYcm\t\tZcm\t\tFXcm\t\tFYcm\t\tFZcm\n");
+    }
+    const int nP = &nInternalPoints ? (int)nInternalPoints : 0;
+    // tria3 evaluates nothing for ANALYSIS 4, tria6 does (postProcess.c:298 and :414)
+    if (nP > 0 && !(npe == 3 && ANALYSIS == 4)) {
+        std::vector<double> pts(3 * (size_t)nP), pot, ef;
+        for (int i = 0; i < nP; i++)
+            for (int k = 0; k < 3; k++) pts[3 * i + k] = Xinner[i][k];
+        if (ANALYSIS != 1) {
+            pot.resize(2 * (size_t)nP);
+            check(dep_post_potential(sy, nP, pts.data(), pot.data()), "potential");
+        }
+        if (ANALYSIS != 0) {
+            ef.resize(6 * (size_t)nP);
+            check(dep_post_field(sy, nP, pts.data(), ef.data()), "field");
+        }
+        for (int i = 0; i < nP; i++) {
+            const double* x = &pts[3 * i];
+            if (!pot.empty()) {
+                fprintf(fp[0], "%e\t%e\t%e\t", x[0], x[1], x[2]);
+                fprintf(fp[0], "%e\t%e\n", pot[2 * i], pot[2 * i + 1]);
+            }
+            if (!ef.empty()) {
+                fprintf(fp[1], "%e\t%e\t%e\t", x[0], x[1], x[2]);
+                fprintf(fp[1], "%e\t%e\t%e\t", ef[6 * i], ef[6 * i + 1], ef[6 * i + 2]);
+                fprintf(fp[1], "%e\t%e\t%e\n", ef[6 * i + 3], ef[6 * i + 4], ef[6 * i + 5]);
+            }
+        }
+    }
+    if (ANALYSIS == 3 || ANALYSIS == 4) {
+        std::vector<double> Fce(3 * (size_t)E), Xce(3 * (size_t)E);
+        double Fcm[3], Xcm[3];
+        check(dep_post_mst(sy, vMatParam[0][1] * DEP_EPS0, Fce.data(), Xce.data(), Fcm, Xcm), "MST force");
+        fprintf(fp[2], "%e\t%e\t%e\t%e\t%e\t%e\n", Xcm[0], Xcm[1], Xcm[2], Fcm[0], Fcm[1], Fcm[2]);
+        fprintf(fp[2], "#Xce\t\tYce\t\tZce\t\tFXce\t\tFYce\t\tFZce\n");
+        for (int e = 0; e < E; e++)
+            if (vBCType[mElems[e][0] - 1] == 6) {
+                fprintf(fp[2], "%e\t%e\t%e\t", Xce[3 * e], Xce[3 * e + 1], Xce[3 * e + 2]);
+                fprintf(fp[2], "%e\t%e\t%e\n", Fce[3 * e], Fce[3 * e + 1], Fce[3 * e + 2]);
+            }
+    } else if (ANALYSIS > 4) {
+        const int nF = &nFPoints ? (int)nFPoints : 0;
+        const int order = (int)ANALYSIS - 4;   // multipoleSetup, postProcess.c:194-236
+        std::vector<double> pts(3 * (size_t)nF), F(3 * (size_t)nF);
+        for (int i = 0; i < nF; i++)
+            for (int k = 0; k < 3; k++) pts[3 * i + k] = XF[i][k];
+        if (nF) check(dep_post_multipole(sy, order, axis[0], nF, pts.data(), F.data()), "multipolar force");
+        for (int i = 0; i < nF; i++) {
+            fprintf(fp[2], "%e\t%e\t%e\t", pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
+            fprintf(fp[2], "%e\t%e\t%e\n", F[3 * i], F[3 * i + 1], F[3 * i + 2]);
+        }
+    }
+    for (FILE* f : fp)
+        if (f) fclose(f);
+    return 0;
+}
+
+}  // namespace
+
+extern "C" int postProcess_tria3(unsigned int ANALYSIS, char* cOutputType, double* axis, double** XF,
+                                 double** Xinner, double** mNodes, unsigned int** mElems, double** vMatParam,
+                                 double* vProbParam, unsigned int* vBCType, double* vB)
+{
+    (void)mNodes;
+    (void)vProbParam;
+    return post(3, ANALYSIS, cOutputType, axis, XF, Xinner, vMatParam, vBCType, mElems, vB);
+}
+
+extern "C" int postProcess_tria6(unsigned int ANALYSIS, char* cOutputType, double* axis, double** XF,
+                                 double** Xinner, double** mNodes, unsigned int** mElems, double** vMatParam,
+                                 double* vProbParam, unsigned int* vBCType, double* vB)
+{
+    (void)mNodes;
+    (void)vProbParam;
+    return post(6, ANALYSIS, cOutputType, axis, XF, Xinner, vMatParam, vBCType, mElems, vB);
+}
+
+// The VTK writer is not part of this library (src/postProcess.c:488-738).
+extern "C" int vtkPostProcess_tria3(unsigned int, char*, double*, double**, double**, double**, unsigned int**,
+                                    double**, double*, unsigned int*, double*)
+{
+    fatal("vtkPostProcess_tria3: VTK output is not available in this build");
+}
+
+extern "C" int vtkPostProcess_tria6(unsigned int, char*, double*, double**, double**, double**, unsigned int**,
+                                    double**, double*, unsigned int*, double*)
+{
+    fatal("vtkPostProcess_tria6: VTK output is not available in this build");
 }

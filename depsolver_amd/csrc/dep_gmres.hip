@@ -55,6 +55,28 @@ __global__ void k_update(int nvec, long n2, const double* __restrict__ V, const 
     w[t] = s;
 }
 
+// out[i] = P[i][i]
+__global__ void k_diag_gather(int N, const double* __restrict__ P, long ldP, double* __restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N) out[i] = P[(long)i * ldP + i];
+}
+
+__global__ void k_to_f32(long n, const double* __restrict__ in, float* __restrict__ out)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (float)in[i];
+}
+
+__global__ void k_rowscale(int N, const double* __restrict__ d, const double* __restrict__ in,
+                           double* __restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    out[i] = d[i] * in[i];
+    out[N + i] = d[i] * in[N + i];
+}
+
 // out = w * alpha
 __global__ void k_scale(long n2, const double* __restrict__ w, double alpha, double* __restrict__ out)
 {
@@ -113,6 +135,24 @@ hipError_t launch_dots(dep_ctx* c, int nvec, long n2, const double* V, const dou
 {
     hipLaunchKernelGGL(k_dots_partial, dim3(DOT_CHUNKS, nvec), dim3(256), 0, c->stream, n2, V, w, part);
     hipLaunchKernelGGL(k_dots_reduce, dim3(dep_cdiv(nvec, 256)), dim3(256), 0, c->stream, nvec, part, h);
+    return hipGetLastError();
+}
+
+hipError_t launch_diag_gather(dep_ctx* c, int N, const double* P, long ldP, double* out)
+{
+    hipLaunchKernelGGL(k_diag_gather, dim3(dep_cdiv(N, 256)), dim3(256), 0, c->stream, N, P, ldP, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_to_f32(dep_ctx* c, long n, const double* in, float* out)
+{
+    hipLaunchKernelGGL(k_to_f32, dim3((unsigned)dep_cdiv(n, 256)), dim3(256), 0, c->stream, n, in, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_rowscale(dep_ctx* c, int N, const double* d, const double* in, double* out)
+{
+    hipLaunchKernelGGL(k_rowscale, dim3(dep_cdiv(N, 256)), dim3(256), 0, c->stream, N, d, in, out);
     return hipGetLastError();
 }
 

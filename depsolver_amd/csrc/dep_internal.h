@@ -34,6 +34,7 @@ struct dep_kstat {
 
 struct dep_ctx {
     int device = 0;
+    int ncu = 0;                             // compute units of the device
     hipStream_t stream = nullptr;
     hipStream_t stream2 = nullptr;           // second stream of the LU's lookahead
     hipEvent_t fork = nullptr, join = nullptr;
@@ -83,6 +84,10 @@ struct dep_geom {
     double* P = nullptr;
     long ldP = 0;
     bool assembled = false;
+    int tc = DEP_TC;                        // columns per assembly patch (16, 32 or 64)
+    struct dep_mst_cache* mst = nullptr;   // Maxwell-stress point set, built on first use
+    float* P32 = nullptr;                   // FP32 copy of P for refined FP32 GMRES
+    bool p32_valid = false;
     std::vector<std::pair<void*, size_t>> allocs;
     dep_ctx* ctx_of() { return ctx; }
 };
@@ -172,6 +177,13 @@ hipError_t launch_zmatvec_w(dep_sys* s, const double* wr, const double* wi, cons
 #define DEP_DOT_CHUNKS 32
 hipError_t launch_dots(dep_ctx* c, int nvec, long n2, const double* V, const double* w, double* h,
                        double* part);
+hipError_t launch_diag_gather(dep_ctx* c, int N, const double* P, long ldP, double* out);
+hipError_t launch_to_f32(dep_ctx* c, long n, const double* in, float* out);
+// out[i] = d[i] in[i], out[N + i] = d[i] in[N + i]
+hipError_t launch_rowscale(dep_ctx* c, int N, const double* d, const double* in, double* out);
+// y = Z x (b == nullptr) or b - Z x with the FP32 blocks P32 and the given weights
+hipError_t launch_zmatvec32_w(dep_sys* s, const double* wr, const double* wi, const double* shift,
+                              const double* x, const double* b, double* y);
 hipError_t launch_update(dep_ctx* c, int nvec, long n2, const double* V, const double* h, double* w);
 hipError_t launch_scale(dep_ctx* c, long n2, const double* w, double alpha, double* out);
 hipError_t launch_combine(dep_ctx* c, int nvec, long n2, const double* V, const double* y, double* x);

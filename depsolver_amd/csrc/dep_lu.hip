@@ -356,8 +356,8 @@ struct mfma16<float> {
 // K chunk is loaded into registers while the current one feeds the MFMAs (two LDS
 // buffers, one barrier per chunk).
 template <class T>
-__global__ __launch_bounds__(256) void k_gemm_cmfma(int ra, int rb, int ca, int cb, int kc, int nbk,
-                                                    T* __restrict__ Zr, T* __restrict__ Zi, long ldZ)
+__device__ __forceinline__ void gemm_cmfma_tile(int bx, int by, int ra, int rb, int ca, int cb, int kc, int nbk,
+                                                T* __restrict__ Zr, T* __restrict__ Zi, long ldZ)
 {
     constexpr int BM = 64, BN = 64, BK = 16;
     typedef typename mfma16<T>::acc acc;
@@ -365,7 +365,7 @@ __global__ __launch_bounds__(256) void k_gemm_cmfma(int ra, int rb, int ca, int 
     __shared__ T Bsr[2][BK][BN + 1], Bsi[2][BK][BN + 1];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
-    const int m0 = ra + blockIdx.y * BM, n0 = ca + blockIdx.x * BN;
+    const int m0 = ra + by * BM, n0 = ca + bx * BN;
     const int li = lane & 15, lk = lane >> 4;
     // staging: A rows t/4, k (t%4)*4 .. +3; B row k = t/16, columns (t%16)*4 .. +3
     const int a_r = t >> 2, a_k = (t & 3) * 4, b_k = t >> 4, b_c = (t & 15) * 4;
@@ -452,6 +452,110 @@ __global__ __launch_bounds__(256) void k_gemm_cmfma(int ra, int rb, int ca, int 
                     Zi[(long)gi * ldZ + gc] -= ci[a][b][v];
                 }
             }
+}
+
+// persistent form: workgroups loop over the 64x64 tiles (grid may be smaller than the tile count)
+template <class T>
+__global__ __launch_bounds__(256) void k_gemm_cmfma(int ra, int rb, int ca, int cb, int kc, int nbk,
+                                                    T* __restrict__ Zr, T* __restrict__ Zi, long ldZ)
+{
+    const int ntx = (cb - ca + 63) / 64, ntiles = ntx * ((rb - ra + 63) / 64);
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x)
+        gemm_cmfma_tile<T>(tile % ntx, tile / ntx, ra, rb, ca, cb, kc, nbk, Zr, Zi, ldZ);
+}
+
+// Multiply panels by an inverse diagonal block on 16x16x4 MFMA, in place:
+//   RIGHT: W[i, k0:k0+nb] <- W[i, k0:k0+nb] Uinv for rows i in [r0, e)  (64-row tiles)
+//   LEFT:  W[k0:k0+nb, j] <- Linv W[k0:k0+nb, j] for columns j in [r0, e)  (64-column tiles)
+// Binv is DIAG_NB x DIAG_NB complex interleaved.  A workgroup stages its whole
+// 64 x 64 panel tile and the block in LDS before it writes, so the update is in place.
+template <class T, bool RIGHT>
+__global__ __launch_bounds__(256) void k_apply_inv(int r0, int e, int nb, T* __restrict__ Zr, T* __restrict__ Zi,
+                                                   long ldZ, int k0, const T* __restrict__ Binv)
+{
+    typedef typename mfma16<T>::acc acc;
+    __shared__ T Xr[NBI][NBI + 1], Xi[NBI][NBI + 1], Yr[NBI][NBI + 1], Yi[NBI][NBI + 1];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32, li = lane & 15, lk = lane >> 4;
+    const int ntiles = (e - r0 + 63) / 64;
+    // the block operand: Y = Uinv (RIGHT) or X = Linv (LEFT)
+    for (int idx = t; idx < NBI * NBI; idx += 256) {
+        const int r = idx >> 6, c = idx & 63;
+        const T vr = Binv[2 * idx], vi = Binv[2 * idx + 1];
+        if (RIGHT) {
+            Yr[r][c] = vr;
+            Yi[r][c] = vi;
+        } else {
+            Xr[r][c] = vr;
+            Xi[r][c] = vi;
+        }
+    }
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int p0 = r0 + 64 * tile;   // first row (RIGHT) or column (LEFT) of the tile
+        __syncthreads();                 // previous tile's reads of the panel operand are done
+        for (int idx = t; idx < NBI * NBI; idx += 256) {
+            const int r = idx >> 6, c = idx & 63;
+            if (RIGHT) {   // X = W[p0 + r, k0 + c]
+                const bool ok = p0 + r < e && c < nb;
+                Xr[r][c] = ok ? Zr[(long)(p0 + r) * ldZ + k0 + c] : T(0);
+                Xi[r][c] = ok ? Zi[(long)(p0 + r) * ldZ + k0 + c] : T(0);
+            } else {       // Y = W[k0 + r, p0 + c]
+                const bool ok = r < nb && p0 + c < e;
+                Yr[r][c] = ok ? Zr[(long)(k0 + r) * ldZ + p0 + c] : T(0);
+                Yi[r][c] = ok ? Zi[(long)(k0 + r) * ldZ + p0 + c] : T(0);
+            }
+        }
+        __syncthreads();
+        acc cr[2][2], ci[2][2];
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int b = 0; b < 2; b++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) cr[a][b][v] = ci[a][b][v] = T(0);
+        for (int ks = 0; ks < NBI; ks += 4) {
+            T ar[2], ai[2], nai[2], br[2], bi[2];
+#pragma unroll
+            for (int a = 0; a < 2; a++) {
+                ar[a] = Xr[wm + 16 * a + li][ks + lk];
+                ai[a] = Xi[wm + 16 * a + li][ks + lk];
+                nai[a] = -ai[a];
+                br[a] = Yr[ks + lk][wn + 16 * a + li];
+                bi[a] = Yi[ks + lk][wn + 16 * a + li];
+            }
+#pragma unroll
+            for (int a = 0; a < 2; a++)
+#pragma unroll
+                for (int b = 0; b < 2; b++) {
+                    cr[a][b] = mfma16<T>::op(ar[a], br[b], cr[a][b]);
+                    ci[a][b] = mfma16<T>::op(ar[a], bi[b], ci[a][b]);
+                }
+#pragma unroll
+            for (int a = 0; a < 2; a++)
+#pragma unroll
+                for (int b = 0; b < 2; b++) {
+                    cr[a][b] = mfma16<T>::op(nai[a], bi[b], cr[a][b]);
+                    ci[a][b] = mfma16<T>::op(ai[a], br[b], ci[a][b]);
+                }
+        }
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int b = 0; b < 2; b++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int r = wm + 16 * a + mfma16<T>::row(v, lane), c = wn + 16 * b + li;
+                    if (RIGHT) {
+                        if (p0 + r < e && c < nb) {
+                            Zr[(long)(p0 + r) * ldZ + k0 + c] = cr[a][b][v];
+                            Zi[(long)(p0 + r) * ldZ + k0 + c] = ci[a][b][v];
+                        }
+                    } else if (r < nb && p0 + c < e) {
+                        Zr[(long)(k0 + r) * ldZ + p0 + c] = cr[a][b][v];
+                        Zi[(long)(k0 + r) * ldZ + p0 + c] = ci[a][b][v];
+                    }
+                }
+    }
 }
 
 // ---- low-precision operands and the MFMA trailing update --------------------
@@ -595,17 +699,16 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(int M, int NC, int NCp, int K
 // current buffer feeds the MFMAs, then stored into the other buffer, so each
 // chunk costs one barrier.
 template <class H, class V8>
-__global__ __launch_bounds__(256) void k_gemm_mfma3(int M, int NC, int NCp, int K2p, int r0, int c0,
-                                                    const H* __restrict__ A, const H* __restrict__ BT,
-                                                    float* __restrict__ Zr, float* __restrict__ Zi,
-                                                    long ldZ)
+__device__ __forceinline__ void gemm_mfma3_tile(int bx, int by, int M, int NC, int NCp, int K2p, int r0,
+                                                int c0, const H* __restrict__ A, const H* __restrict__ BT,
+                                                float* __restrict__ Zr, float* __restrict__ Zi, long ldZ)
 {
     constexpr int TK = 32, LDA = TK + 8;
     __shared__ H As[2][128][LDA], Bs[2][128][LDA];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
-    const int m0 = blockIdx.y * 128;
-    const int n0 = blockIdx.x * 128;        // in [0, 2*NCp)
+    const int m0 = by * 128;
+    const int n0 = bx * 128;                // in [0, 2*NCp)
     f32x16 acc[2][2];
 #pragma unroll
     for (int a = 0; a < 2; a++)
@@ -676,6 +779,33 @@ __global__ __launch_bounds__(256) void k_gemm_mfma3(int M, int NC, int NCp, int 
                 if (m < M && n < NC) C[(long)(r0 + m) * ldZ + c0 + n] -= acc[a][b][v];
             }
 }
+
+template <class H, class V8>
+__global__ __launch_bounds__(256) void k_gemm_mfma3(int M, int NC, int NCp, int K2p, int r0, int c0,
+                                                    const H* __restrict__ A, const H* __restrict__ BT,
+                                                    float* __restrict__ Zr, float* __restrict__ Zi,
+                                                    long ldZ, int grouped)
+{
+    const int ntx = 2 * NCp / 128, nty = (M + 127) / 128, ntiles = ntx * nty;
+    if (!grouped) {
+        for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x)
+            gemm_mfma3_tile<H, V8>(tile % ntx, tile / ntx, M, NC, NCp, K2p, r0, c0, A, BT, Zr, Zi, ldZ);
+        return;
+    }
+    // Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2):
+    // work item w belongs to XCD w % 8, and each XCD takes a contiguous range of
+    // logical tiles, walked in groups of GM tile rows (column by column inside a
+    // group), so the group's A' rows and the current B' columns stay in that L2.
+    constexpr int GM = 8;
+    const int per = (ntiles + 7) / 8;
+    for (int w = blockIdx.x; w < 8 * per; w += gridDim.x) {
+        const int L = (w & 7) * per + (w >> 3);
+        if ((w >> 3) >= per || L >= ntiles) continue;
+        const int g0 = (L / (GM * ntx)) * GM, rows = min(GM, nty - g0), loc = L % (GM * ntx);
+        gemm_mfma3_tile<H, V8>(loc / rows, g0 + loc % rows, M, NC, NCp, K2p, r0, c0, A, BT, Zr, Zi, ldZ);
+    }
+}
+
 
 // ---- sync-free blocked triangular sweep ------------------------------------
 // One workgroup per 64-row block.  Workgroups take their block from a ticket
@@ -901,18 +1031,44 @@ static hipError_t lu_blocked(dep_sys* s, T* Wr, T* Wi, long ld, T* inv, T* piv, 
     const int N = g->N;
     hipError_t e;
     const char* gsel = getenv("DEPSOLVER_MIXED_GEMM");
-    const bool gemm_v1 = gsel && strcmp(gsel, "v1") == 0;
+    const bool gemm_v1 = gsel && strcmp(gsel, "v1") == 0, v1 = gemm_v1;
     const char* fsel = getenv("DEPSOLVER_FP64_GEMM");
     const bool f64_valu = fsel && strcmp(fsel, "valu") == 0;
     const char* psel = getenv("DEPSOLVER_PANEL_GEMM");
     const bool panel_valu = psel && strcmp(psel, "valu") == 0;
     // W[ra:rb, ca:cb] -= W[ra:rb, kc:kc+nbk] W[kc:kc+nbk, ca:cb] inside the panel and for U12
+    // Persistent GEMM grids: at most gemm_cap workgroups loop over the tiles, which
+    // leaves room on every CU for the panel kernels of the other stream.
+    const char* csel = getenv("DEPSOLVER_GEMM_CAP");    // workgroups per CU (0: one per tile)
+    const int cap_mult = csel ? atoi(csel) : 0;
+    const char* osel = getenv("DEPSOLVER_GEMM_ORDER");  // 1: XCD-grouped tile order
+    const int grouped = osel ? atoi(osel) : 0;
+    const long gemm_cap = cap_mult > 0 ? (long)cap_mult * c->ncu : (1L << 30);
+    auto tiles = [&](long n) { return dim3((unsigned)std::min<long>(n, gemm_cap)); };
     auto gemm_small = [&](int ra, int rb, int ca, int cb, int kc, int nbk, hipStream_t q) {
-        dim3 grid(dep_cdiv(cb - ca, 64), dep_cdiv(rb - ra, 64));
         if (panel_valu)
-            hipLaunchKernelGGL(k_gemmT<T>, grid, dim3(256), 0, q, ra, rb, ca, cb, kc, nbk, Wr, Wi, ld);
+            hipLaunchKernelGGL(k_gemmT<T>, dim3(dep_cdiv(cb - ca, 64), dep_cdiv(rb - ra, 64)), dim3(256), 0, q,
+                               ra, rb, ca, cb, kc, nbk, Wr, Wi, ld);
         else
-            hipLaunchKernelGGL(k_gemm_cmfma<T>, grid, dim3(256), 0, q, ra, rb, ca, cb, kc, nbk, Wr, Wi, ld);
+            hipLaunchKernelGGL(k_gemm_cmfma<T>, tiles((long)dep_cdiv(cb - ca, 64) * dep_cdiv(rb - ra, 64)),
+                               dim3(256), 0, q, ra, rb, ca, cb, kc, nbk, Wr, Wi, ld);
+    };
+    // panel rows [r0, e) times Uinv, and Linv times panel columns [c0, e)
+    auto apply_u = [&](int r0, int e, int nbi, int kj, const T* B, hipStream_t q) {
+        if (panel_valu)
+            hipLaunchKernelGGL(k_uinv<T>, dim3(dep_cdiv(e - r0, 16)), dim3(256), 0, q, e, nbi, r0, Wr, Wi, ld, kj,
+                               B);
+        else
+            hipLaunchKernelGGL((k_apply_inv<T, true>), tiles(dep_cdiv(e - r0, 64)), dim3(256), 0, q, r0, e, nbi, Wr,
+                               Wi, ld, kj, B);
+    };
+    auto apply_l = [&](int c0, int e, int nbi, int kj, const T* B, hipStream_t q) {
+        if (panel_valu)
+            hipLaunchKernelGGL(k_linv<T>, dim3(dep_cdiv(e - c0, 16)), dim3(256), 0, q, e, nbi, c0, Wr, Wi, ld, kj,
+                               B);
+        else
+            hipLaunchKernelGGL((k_apply_inv<T, false>), tiles(dep_cdiv(e - c0, 64)), dim3(256), 0, q, c0, e, nbi, Wr,
+                               Wi, ld, kj, B);
     };
     // low-precision modes: A' = [Lr | Li] of the panel (N x 2 NBO), B' of the columns beyond
     // the next panel (2 ld32 x 2 NBO) and B' of the next panel's columns (2 NBO x 2 NBO)
@@ -927,12 +1083,9 @@ static hipError_t lu_blocked(dep_sys* s, T* Wr, T* Wi, long ld, T* inv, T* piv, 
             const int nbi = std::min(NBI, pe - kj);
             T* invb = inv + (size_t)(kj / NBI) * 4 * NBI * NBI;
             hipLaunchKernelGGL(k_diag<T>, dim3(1), dim3(DIAG_THREADS), 0, q, nbi, Wr, Wi, ld, kj, invb, piv);
-            if (kj + nbi < N)
-                hipLaunchKernelGGL(k_uinv<T>, dim3(dep_cdiv(N - kj - nbi, 16)), dim3(256), 0, q, N, nbi,
-                                   kj + nbi, Wr, Wi, ld, kj, invb + 2 * NBI * NBI);
+            if (kj + nbi < N) apply_u(kj + nbi, N, nbi, kj, invb + 2 * NBI * NBI, q);
             if (kj + nbi < pe) {
-                hipLaunchKernelGGL(k_linv<T>, dim3(dep_cdiv(pe - kj - nbi, 16)), dim3(256), 0, q, pe, nbi,
-                                   kj + nbi, Wr, Wi, ld, kj, invb);
+                apply_l(kj + nbi, pe, nbi, kj, invb, q);
                 gemm_small(kj + nbi, N, kj + nbi, pe, kj, nbi, q);
             }
         }
@@ -943,8 +1096,7 @@ static hipError_t lu_blocked(dep_sys* s, T* Wr, T* Wi, long ld, T* inv, T* piv, 
         for (int kj = k0; kj < pe; kj += NBI) {
             const int nbi = std::min(NBI, pe - kj);
             const T* invb = inv + (size_t)(kj / NBI) * 4 * NBI * NBI;
-            hipLaunchKernelGGL(k_linv<T>, dim3(dep_cdiv(c1 - c0, 16)), dim3(256), 0, q, c1, nbi, c0, Wr, Wi,
-                               ld, kj, invb);
+            apply_l(c0, c1, nbi, kj, invb, q);
             if (kj + nbi < pe)
                 gemm_small(kj + nbi, pe, c0, c1, kj, nbi, q);
         }
@@ -974,7 +1126,8 @@ static hipError_t lu_blocked(dep_sys* s, T* Wr, T* Wi, long ld, T* inv, T* piv, 
                 hipLaunchKernelGGL(k_gemm_f64mfma, grid, dim3(256), 0, q, pe, N, c0, c1, k0, nbo, (double*)Wr,
                                    (double*)Wi, ld);
             else
-                hipLaunchKernelGGL(k_gemm_cmfma<T>, grid, dim3(256), 0, q, pe, N, c0, c1, k0, nbo, Wr, Wi, ld);
+                hipLaunchKernelGGL(k_gemm_cmfma<T>, tiles((long)grid.x * grid.y), dim3(256), 0, q, pe, N, c0, c1,
+                                   k0, nbo, Wr, Wi, ld);
             prof_end(c, ev, "lu_gemm_fp64", 8.0 * M * (double)NC * nbo,
                      32.0 * M * (double)NC + 32.0 * nbo * (M + NC), q);
             return;
@@ -989,7 +1142,7 @@ static hipError_t lu_blocked(dep_sys* s, T* Wr, T* Wi, long ld, T* inv, T* piv, 
             hipLaunchKernelGGL(k_pack_b<_Float16>, dim3(dep_cdiv(NCp, 64)), dim3(256), 0, q, NC, NCp, nbo, K2p,
                                k0, c0, Fr, Fi, ld, B);
         prof_begin(c, &ev, q);
-        dim3 grid(2 * NCp / 128, dep_cdiv(M, 128));
+        const dim3 grid = v1 ? dim3(2 * NCp / 128, dep_cdiv(M, 128)) : tiles((long)(2 * NCp / 128) * dep_cdiv(M, 128));
         float* Cr = (float*)Wr;
         float* Ci = (float*)Wi;
         if (mode == 2) {
@@ -998,14 +1151,14 @@ static hipError_t lu_blocked(dep_sys* s, T* Wr, T* Wi, long ld, T* inv, T* piv, 
                                    (const __bf16*)Ah, (const __bf16*)B, Cr, Ci, ld);
             else
                 hipLaunchKernelGGL((k_gemm_mfma3<__bf16, bf16x8>), grid, dim3(256), 0, q, M, NC, NCp, K2p, pe, c0,
-                                   (const __bf16*)Ah, (const __bf16*)B, Cr, Ci, ld);
+                                   (const __bf16*)Ah, (const __bf16*)B, Cr, Ci, ld, grouped);
         } else {
             if (gemm_v1)
                 hipLaunchKernelGGL((k_gemm_mfma<_Float16, f16x8>), grid, dim3(256), 0, q, M, NC, NCp, K2p, pe, c0,
                                    (const _Float16*)Ah, (const _Float16*)B, Cr, Ci, ld);
             else
                 hipLaunchKernelGGL((k_gemm_mfma3<_Float16, f16x8>), grid, dim3(256), 0, q, M, NC, NCp, K2p, pe,
-                                   c0, (const _Float16*)Ah, (const _Float16*)B, Cr, Ci, ld);
+                                   c0, (const _Float16*)Ah, (const _Float16*)B, Cr, Ci, ld, grouped);
         }
         // complex flops 8 M NC nbo; bytes: C read+write FP32 complex + FP16 operands
         prof_end(c, ev, "lu_gemm_mixed", 8.0 * M * (double)NC * nbo, 16.0 * M * (double)NC + 2.0 * K2p * (M + 2.0 * NCp),

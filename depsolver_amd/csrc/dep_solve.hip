@@ -27,7 +27,14 @@ __global__ void k_weight(int N, const double* __restrict__ P, long ldP, const do
 // ---- products with P -------------------------------------------------------
 // y = Z x with Z = w .* P + diag(shift); x, y split (re block, im block) of length N.
 // One wave per row, FP64.
-__global__ __launch_bounds__(256) void k_zmatvec(int N, const double* __restrict__ P, long ldP,
+// y = Z x (or b - Z x) with Z = diag(w) P + diag(shift), P real (FP64 or FP32
+// storage, FP64 arithmetic): each wave takes ZROWS rows and walks their columns
+// in chunks of 256 (4 consecutive columns per lane, vector loads), so every
+// chunk of x is read once per ZROWS rows; a scalar loop finishes the columns
+// past the last full chunk.
+#define ZROWS 4
+template <class TP>
+__global__ __launch_bounds__(256) void k_zmatvec(int N, const TP* __restrict__ P, long ldP,
                                                  const double* __restrict__ wr,
                                                  const double* __restrict__ wi,
                                                  const double* __restrict__ shift,
@@ -35,31 +42,65 @@ __global__ __launch_bounds__(256) void k_zmatvec(int N, const double* __restrict
                                                  const double* __restrict__ b,
                                                  double* __restrict__ y)
 {
+    typedef TP p4 __attribute__((ext_vector_type(4)));
+    typedef double d4 __attribute__((ext_vector_type(4)));
     const int lane = threadIdx.x & 63;
-    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (row >= N) return;
-    const double* p = P + (long)row * ldP;
-    double sr = 0.0, si = 0.0;
-    for (int c = lane; c < N; c += 64) {
-        const double v = p[c];
-        sr += v * x[c];
-        si += v * x[N + c];
+    const int row0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * ZROWS;
+    if (row0 >= N) return;
+    const int nr = min(ZROWS, N - row0);
+    double sr[ZROWS], si[ZROWS];
+#pragma unroll
+    for (int k = 0; k < ZROWS; k++) sr[k] = si[k] = 0.0;
+    const int full = N / 256 * 256;
+    for (int c = 4 * lane; c < full; c += 256) {
+        const d4 xr = *(const d4*)(x + c), xi = *(const d4*)(x + N + c);
+#pragma unroll
+        for (int k = 0; k < ZROWS; k++) {
+            if (k < nr) {
+                const p4 v = *(const p4*)(P + (long)(row0 + k) * ldP + c);
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    sr[k] += (double)v[e] * xr[e];
+                    si[k] += (double)v[e] * xi[e];
+                }
+            }
+        }
+    }
+    for (int c = full + lane; c < N; c += 64) {
+        const double xr = x[c], xi = x[N + c];
+#pragma unroll
+        for (int k = 0; k < ZROWS; k++)
+            if (k < nr) {
+                const double v = (double)P[(long)(row0 + k) * ldP + c];
+                sr[k] += v * xr;
+                si[k] += v * xi;
+            }
     }
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        sr += __shfl_xor(sr, o);
-        si += __shfl_xor(si, o);
-    }
-    if (lane == 0) {
-        const double a = wr[row], bb = wi[row], s = shift[row];
-        double zr = a * sr - bb * si + s * x[row];
-        double zi = a * si + bb * sr + s * x[N + row];
-        if (b) {   // residual b - Z x
-            zr = b[row] - zr;
-            zi = b[N + row] - zi;
+    for (int k = 0; k < ZROWS; k++)
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            sr[k] += __shfl_xor(sr[k], o);
+            si[k] += __shfl_xor(si[k], o);
         }
-        y[row] = zr;
-        y[N + row] = zi;
+    if (lane < nr) {
+        double zr = 0.0, zi = 0.0;
+#pragma unroll
+        for (int k = 0; k < ZROWS; k++)
+            if (k == lane) {
+                zr = sr[k];
+                zi = si[k];
+            }
+        const int row = row0 + lane;
+        const double a = wr[row], bb = wi[row], sh = shift[row];
+        double yr = a * zr - bb * zi + sh * x[row];
+        double yi = a * zi + bb * zr + sh * x[N + row];
+        if (b) {   // residual b - Z x
+            yr = b[row] - yr;
+            yi = b[N + row] - yi;
+        }
+        y[row] = yr;
+        y[N + row] = yi;
     }
 }
 
@@ -85,10 +126,23 @@ hipError_t launch_zmatvec_w(dep_sys* s, const double* wr, const double* wi, cons
     dep_geom* g = s->g;
     hipEvent_t ev;
     prof_begin(g->ctx, &ev);
-    hipLaunchKernelGGL(k_zmatvec, dim3(dep_cdiv(g->N, 4)), dim3(256), 0, g->ctx->stream, g->N, g->P,
-                       g->ldP, wr, wi, shift, x, b, y);
+    hipLaunchKernelGGL(k_zmatvec<double>, dim3(dep_cdiv(g->N, 4 * ZROWS)), dim3(256), 0, g->ctx->stream, g->N,
+                       g->P, g->ldP, wr, wi, shift, x, b, y);
     const double n = g->N;
     prof_end(g->ctx, ev, "zmatvec", 4.0 * n * n, 8.0 * n * n + 48.0 * n);
+    return hipGetLastError();
+}
+
+hipError_t launch_zmatvec32_w(dep_sys* s, const double* wr, const double* wi, const double* shift,
+                              const double* x, const double* b, double* y)
+{
+    dep_geom* g = s->g;
+    hipEvent_t ev;
+    prof_begin(g->ctx, &ev);
+    hipLaunchKernelGGL(k_zmatvec<float>, dim3(dep_cdiv(g->N, 4 * ZROWS)), dim3(256), 0, g->ctx->stream, g->N,
+                       g->P32, g->ldP, wr, wi, shift, x, b, y);
+    const double n = g->N;
+    prof_end(g->ctx, ev, "zmatvec32", 4.0 * n * n, 4.0 * n * n + 48.0 * n);
     return hipGetLastError();
 }
 

@@ -12,8 +12,9 @@
  *
  *  2. Drop-in shims with the reference's exact signatures (bottom of this
  *     file): electricForm[Full]A_tria{3,6}, gaussBksb, solverGMRES_el and
- *     [vtk]postProcess_tria{3,6} as src/depSolver.c calls them, reading the
- *     same globals (nNodes, nElems, nInterfaces, ...) from the executable.
+ *     postProcess_tria{3,6} as src/depSolver.c calls them, reading the same
+ *     globals (nNodes, nElems, nInterfaces, nInternalPoints, nFPoints) from
+ *     the executable; vtkPostProcess_tria{3,6} exit with a message.
  *
  * Conventions of the native API
  *   - node ids in `conn` are 0-based; material and interface ids in
@@ -125,6 +126,15 @@ int dep_solve(dep_sys *s, double tol, int max_steps, double *sigma, dep_solve_in
 int dep_gmres(dep_sys *s, int precond, double tol, int max_iters, const double *x0,
               double *sigma, dep_solve_info *info, const char *log_path);
 
+/* Refined FP32 GMRES (no counterpart in the reference; the refined-FP32 solve
+ * path): FP64 residuals from the FP64 geometric blocks, corrections by Jacobi-
+ * scaled GMRES on an FP32 copy of the blocks to the relative tolerance
+ * inner_tol (max_inner iterations each), at most max_outer corrections, until
+ * ||b - Z x|| <= tol ||b||.  info: iterations = corrections, inner_iterations =
+ * GMRES iterations.  The solution stays on the device (dep_get_solution). */
+int dep_gmres_ir(dep_sys *sys, double tol, double inner_tol, int max_inner, int max_outer,
+                 double *sigma, dep_solve_info *info);
+
 int dep_get_solution(dep_sys *s, double *sigma);
 int dep_set_solution(dep_sys *s, const double *sigma);
 
@@ -153,6 +163,24 @@ int electricFormFullA_tria6(double **mNodes, unsigned int **mElems, unsigned int
                             double **mA, double **mBC, double *vB);
 int gaussBksb(unsigned int N, double **A, double *B);
 void solverGMRES_el(unsigned int preCond, unsigned int nInit, double **A, double *B);
+/* src/postProcess.c:256 and :372: potential/field at the nInternalPoints points,
+ * MST force (ANALYSIS 3, 4) or multipolar force at the nFPoints points (5-9), on the
+ * system of the preceding electricForm* call with vB as its solution; the reference's
+ * output files and formats.  ANALYSIS 10/11 (ellipsoid) are not available. */
+int postProcess_tria3(unsigned int ANALYSIS, char *cOutputType, double *axis, double **XF,
+                      double **Xinner, double **mNodes, unsigned int **mElems, double **vMatParam,
+                      double *vProbParam, unsigned int *vBCType, double *vB);
+int postProcess_tria6(unsigned int ANALYSIS, char *cOutputType, double *axis, double **XF,
+                      double **Xinner, double **mNodes, unsigned int **mElems, double **vMatParam,
+                      double *vProbParam, unsigned int *vBCType, double *vB);
+/* src/postProcess.c:488 and :626: the VTK writer is not part of this library; these
+ * exit with a message, as the reference does on errors. */
+int vtkPostProcess_tria3(unsigned int ANALYSIS, char *cOutputType, double *axis, double **XF,
+                         double **Xinner, double **mNodes, unsigned int **mElems, double **vMatParam,
+                         double *vProbParam, unsigned int *vBCType, double *vB);
+int vtkPostProcess_tria6(unsigned int ANALYSIS, char *cOutputType, double *axis, double **XF,
+                         double **Xinner, double **mNodes, unsigned int **mElems, double **vMatParam,
+                         double *vProbParam, unsigned int *vBCType, double *vB);
 
 #ifdef __cplusplus
 }

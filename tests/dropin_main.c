@@ -15,7 +15,7 @@
 #include "../include/depsolver_amd.h"
 
 FILE *file_log;
-unsigned int nNodes, nElems, nInterfaces;
+unsigned int nNodes, nElems, nInterfaces, nInternalPoints, nFPoints;
 
 static double **dmat(unsigned r, unsigned c)
 {
@@ -50,6 +50,12 @@ int main(int argc, char **argv)
     }
     for (unsigned j = 0; j < 2 * N; j++)
         if (fscanf(f, "%u %le %le %le", &bct[j], &bc[j][0], &bc[j][1], &bc[j][2]) != 4) return 8;
+    double **Xin = NULL;
+    if (fscanf(f, "%u", &nInternalPoints) == 1 && nInternalPoints) {
+        Xin = dmat(nInternalPoints, 3);
+        for (unsigned i = 0; i < nInternalPoints; i++)
+            if (fscanf(f, "%le %le %le", &Xin[i][0], &Xin[i][1], &Xin[i][2]) != 3) return 9;
+    }
     fclose(f);
     file_log = fopen("/dev/null", "w");
     const int lu = strcmp(argv[3], "lu") == 0;
@@ -68,5 +74,9 @@ int main(int argc, char **argv)
     FILE *o = fopen(argv[2], "w");
     for (unsigned j = 0; j < 2 * N; j++) fprintf(o, "%.17e\n", B[j]);
     fclose(o);
+    double axis[3] = {1.0, 1.0, 1.0};
+    char std_type[] = "STD";
+    if (npe == 6) postProcess_tria6(3, std_type, axis, NULL, Xin, nodes, el, mats, prob, bct, B);
+    else postProcess_tria3(3, std_type, axis, NULL, Xin, nodes, el, mats, prob, bct, B);
     return 0;
 }

@@ -18,7 +18,8 @@ def declared():
 def test_header_declares_the_entry_points():
     names = declared()
     for n in ("dep_ctx_create", "dep_geom_create", "dep_factor", "dep_solve", "dep_gmres",
-              "dep_post_mst", "electricFormFullA_tria6", "gaussBksb", "solverGMRES_el"):
+              "dep_post_mst", "electricFormFullA_tria6", "gaussBksb", "solverGMRES_el",
+              "postProcess_tria6"):
         assert n in names
 
 

@@ -42,6 +42,24 @@ def test_gmres_tight_tolerance_reaches_lu_solution(ctx, name):
         g.close()
 
 
+@pytest.mark.parametrize("name", ["tiny_tria6", "tiny_tria3", "config1"])
+def test_refined_fp32_gmres_reaches_reference(ctx, name):
+    # FP32-operator GMRES corrections refined with FP64 residuals: same sigma as the
+    # reference's LU to the FP64 path's tolerance
+    import depsolver_amd as dep
+    p, z = load_case(name)
+    g = dep.Geometry(ctx, p)
+    s = dep.System(g)
+    try:
+        sigma, info = s.gmres_ir(tol=1e-12, inner_tol=1e-6)
+        assert info["converged"] == 1 and info["residual"] <= 1e-12
+        assert 1 <= info["iterations"] <= 4
+        assert rel(sigma, z["sigma"]) < 1e-10
+    finally:
+        s.close()
+        g.close()
+
+
 @pytest.mark.parametrize("name", ["tiny_tria6", "tiny_tria3"])
 def test_gmres_follows_the_reference_iteration(ctx, orc, name, tmp_path):
     # same Krylov iteration as gmres_el at its 1e-6 tolerance: same iteration count,
@@ -156,6 +174,7 @@ def test_reference_signature_shims(tmp_path, orc, solver):
                     f"-L{lib}", "-ldepsolver_amd", f"-Wl,-rpath,{lib}"], check=True)
     p = mesh.sphere_between_plates(1, 4)
     N = p.n_nodes
+    pts = np.array([[0.1, 0.2, 1.6], [0.3, -0.2, -2.1]])
     with open(tmp_path / "problem.txt", "w") as f:
         f.write(f"{N} {p.n_elems} {p.npe} {len(p.mats)} {len(p.ifaces)} {p.freq:.17e}\n")
         for x in p.xyz:
@@ -168,6 +187,9 @@ def test_reference_signature_shims(tmp_path, orc, solver):
             f.write("%d %d\n" % tuple(i))
         for j in range(2 * N):
             f.write("%d %.17e %.17e %.17e\n" % (p.bctype[j], *p.bc[j]))
+        f.write("%d\n" % len(pts))
+        for x in pts:
+            f.write("%.17e %.17e %.17e\n" % tuple(x))
     r = subprocess.run([str(exe), str(tmp_path / "problem.txt"), str(tmp_path / "x.txt"), solver],
                        capture_output=True, text=True, timeout=120, cwd=tmp_path)
     assert r.returncode == 0, r.stderr
@@ -178,3 +200,15 @@ def test_reference_signature_shims(tmp_path, orc, solver):
     assert rel(x, want) < tol
     if solver == "gmres":
         assert "PROBLEM CONVERGES IN" in open(tmp_path / "gmres.log").read()
+    # postProcess_tria6 (ANALYSIS 3) on the shim's system with the solution x; files print 7 digits
+    pot = np.loadtxt(tmp_path / "potential.dat")
+    fld = np.loadtxt(tmp_path / "field.dat")
+    for k, pt in enumerate(pts):
+        wp, wf = orc.potential(p, x, pt), orc.field(p, x, pt)
+        assert np.abs(pot[k, 3:] - wp).max() <= 1e-6 * np.abs(wp).max()
+        assert np.abs(fld[k, 3:] - wf).max() <= 1e-6 * np.abs(wf).max()
+    Fce, Xce, Fcm, Xcm = orc.force_mst(p, x)
+    rows = np.loadtxt(tmp_path / "force-mst.dat")
+    sel = p.bctype[p.conn[:, 0]] == 6
+    assert np.abs(rows[1:, 3:] - Fce[sel]).max() <= 1e-6 * np.abs(Fce).max()
+    assert np.abs(rows[1:, :3] - Xce[sel]).max() <= 1e-6 * np.abs(Xce).max()
