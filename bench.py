@@ -4,15 +4,20 @@ Workload (one step = one solve of a new request, inputs resident in HBM):
   dielectric sphere (icosphere level 4, 5,120 tria6) between two 10x10 plates
   (24x24 quadratic grids) at z = +-3, +-1 V; medium eps_r 78 / 1e-3 S/m,
   particle eps_r 2.5 / 1e-4 S/m, f = 1 MHz; N = 15,044 nodes, E = 7,424.
-  step = FP64 collocation assembly -> factor (refined mixed precision, or
-  FP64 LU) -> FP64 solve/refinement to 1e-12 -> Maxwell-stress force.
+  step = FP64 collocation assembly -> solve to an FP64 residual of 1e-12 ->
+  Maxwell-stress force.
 
-Prints ONE JSON line.  `value` is refined-mixed solves/sec (FP64-LU solves/sec
-when the mixed factor is unavailable); the FP64 path, p50/p99 latencies,
-errors against the FP64 solve, the dominant kernel's roofline (timed with HIP
-events on the library's stream over the timed region) and a CPU baseline (the
-oracle's restatement of the reference on a bounded sample, extrapolated to
-one full solve) are in the same line.
+Solvers: gmres-ir-fp32 (refined FP32: GMRES corrections on an FP32 copy of the
+geometric blocks, FP64 residuals; the default), lu-fp16 / lu-bf16 (FP16/BF16
+MFMA LU factors with FP32 panels, FP64 refinement), and the plain FP64 paths
+gmres-fp64 and lu-fp64.
+
+Prints ONE JSON line.  `value` is solves/sec of --solver; every other path is
+timed beside it (`paths`, with errors against the FP64 LU), with p50/p99
+latencies, the dominant kernel's roofline (timed with HIP events on the
+library's stream over the timed region) and a CPU baseline (the oracle's
+restatement of the reference on a bounded sample, extrapolated to one full
+solve).
 """
 import argparse
 import json
@@ -31,7 +36,7 @@ PEAKS = {  # MI355X published dense peaks
     "hbm": (8000.0, "GB/s", "hbm"),
 }
 KERNEL_PEAK = {"lu_gemm_fp64": "fp64", "lu_gemm_mixed": "fp16", "assemble_regular": "fp64",
-               "points_field": "fp64", "points_potential": "fp64", "zmatvec": "hbm"}
+               "points_field": "fp64", "points_potential": "fp64", "zmatvec": "hbm", "zmatvec32": "hbm"}
 
 
 def log(*a):
@@ -113,14 +118,19 @@ def cpu_baseline(p, s, rows=48, pivots=6, points=24):
             "seconds_per_solve": total}
 
 
+SOLVERS = ("gmres-ir-fp32", "lu-fp16", "lu-bf16", "gmres-fp64", "lu-fp64")
+DTYPE = {"gmres-ir-fp32": "fp32/f64", "lu-fp16": "fp16/fp32+f64", "lu-bf16": "bf16/fp32+f64",
+         "gmres-fp64": "f64", "lu-fp64": "f64"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--fp64-steps", type=int, default=2)
-    ap.add_argument("--precision", default="fp16", choices=["fp16", "bf16", "fp64"])
-    ap.add_argument("--gmres-steps", type=int, default=2)
+    ap.add_argument("--solver", default="gmres-ir-fp32", choices=SOLVERS,
+                    help="the path timed for `value` (the others are reported beside it)")
+    ap.add_argument("--side-steps", type=int, default=2, help="timed steps of each other path")
     ap.add_argument("--config", type=int, default=2, choices=[1, 2])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -139,121 +149,99 @@ def main():
     s = dep.System(g)
     log(f"bench: {p.name} N={N} E={p.n_elems}")
 
-    def step(prec):
+    def step(solver):
+        """One request: FP64 assembly, solve to a 1e-12 FP64 residual, Maxwell-stress force."""
         t0 = time.perf_counter()
         dep.lib().dep_geom_reassemble(g._h)
         s.set_frequency(p.freq)
-        s.factor(prec)
-        sigma, info = s.solve(tol=1e-12, max_steps=30)
+        if solver == "gmres-ir-fp32":
+            sigma, info = s.gmres_ir(tol=1e-12, inner_tol=1e-6)
+        elif solver == "gmres-fp64":
+            sigma, info = s.gmres(precond=1, tol=1e-12, max_iters=600)
+        else:
+            s.factor(solver[3:])
+            sigma, info = s.solve(tol=1e-12, max_steps=30)
         Fce, Xce, Fcm, Xcm = s.mst()
         return time.perf_counter() - t0, sigma, info, Fce, Fcm
 
-    def step_gmres_ir():
-        t0 = time.perf_counter()
-        dep.lib().dep_geom_reassemble(g._h)
-        s.set_frequency(p.freq)
-        sigma, info = s.gmres_ir(tol=1e-12, inner_tol=1e-6)
-        Fce, Xce, Fcm, Xcm = s.mst()
-        return time.perf_counter() - t0, sigma, info, Fce, Fcm
-
-    def step_gmres():
-        t0 = time.perf_counter()
-        dep.lib().dep_geom_reassemble(g._h)
-        s.set_frequency(p.freq)
-        sigma, info = s.gmres(precond=1, tol=1e-12, max_iters=600)
-        Fce, Xce, Fcm, Xcm = s.mst()
-        return time.perf_counter() - t0, sigma, info, Fce, Fcm
-
-    # FP64 reference solve (also the accuracy yardstick for the mixed path)
+    # FP64 LU: the accuracy yardstick for every other path
+    step("lu-fp64")
     lat64 = []
-    for k in range(max(1, args.fp64_steps)):
-        dt, sig64, info64, Fce64, Fcm64 = step("fp64")
+    for k in range(max(1, args.side_steps)):
+        dt, sig64, info64, Fce64, Fcm64 = step("lu-fp64")
         lat64.append(dt)
-        log(f"bench: fp64 step {k} {dt * 1e3:.1f} ms residual {info64['residual']:.2e}")
-    # FP64 GMRES (Jacobi, unrestarted, tolerance 1e-12): the other plain FP64 path.  Its
-    # first call allocates the Krylov workspace, so that call is not timed.
-    latg, gm = [], None
-    if args.gmres_steps:
-        step_gmres()
-    for k in range(args.gmres_steps):
-        dt, sigg, infog, Fceg, Fcmg = step_gmres()
-        latg.append(dt)
-        gm = {"iterations": infog["iterations"], "converged": infog["converged"],
-              "sigma_rel_vs_lu": float(np.abs(sigg - sig64).max() / np.abs(sig64).max())}
-        log(f"bench: fp64 gmres step {k} {dt * 1e3:.1f} ms iterations {infog['iterations']}")
-    # refined FP32 GMRES (FP32-operator corrections, FP64 residuals); first call untimed
-    lati, gi = [], None
-    if args.gmres_steps:
-        step_gmres_ir()
-    for k in range(args.gmres_steps):
-        dt, sigi, infoi, Fcei, Fcmi = step_gmres_ir()
-        lati.append(dt)
-        gi = {"refinements": infoi["iterations"], "inner_iterations": infoi["inner_iterations"],
-              "converged": infoi["converged"],
-              "sigma_rel_vs_lu": float(np.abs(sigi - sig64).max() / np.abs(sig64).max()),
-              "force_element_rel_vs_lu": float(np.abs(Fcei - Fce64).max() / np.abs(Fce64).max())}
-        log(f"bench: gmres-ir step {k} {dt * 1e3:.1f} ms refinements {infoi['iterations']} "
-            f"inner {infoi['inner_iterations']}")
-    prec = args.precision
-    try:
-        step(prec)
-    except dep.DepError as e:
-        log(f"bench: precision {prec} unavailable ({e}); timing fp64")
-        prec = "fp64"
+        log(f"bench: lu-fp64 step {k} {dt * 1e3:.1f} ms residual {info64['residual']:.2e}")
+    fscale = np.abs(Fce64).max()
+
+    def errors(sig, Fce, Fcm):
+        return {"sigma_rel": float(np.abs(sig - sig64).max() / np.abs(sig64).max()),
+                "force_element_rel": float(np.abs(Fce - Fce64).max() / fscale),
+                "force_total_rel": float(np.abs(Fcm - Fcm64).max() / np.abs(Fce64).sum())}
+
+    paths = {"lu-fp64": {"p50_ms": float(np.median(lat64) * 1e3), "solves_per_sec": 1.0 / float(np.median(lat64)),
+                         "residual": info64["residual"]}}
+    for name in SOLVERS:
+        if name in (args.solver, "lu-fp64"):
+            continue
+        step(name)   # first call: workspace allocation, factor storage
+        lat = []
+        for k in range(args.side_steps):
+            dt, sig, info, Fce, Fcm = step(name)
+            lat.append(dt)
+            log(f"bench: {name} step {k} {dt * 1e3:.1f} ms residual {info['residual']:.2e} "
+                f"iterations {info['iterations']}")
+        paths[name] = dict(errors(sig, Fce, Fcm), p50_ms=float(np.median(lat) * 1e3),
+                           solves_per_sec=1.0 / float(np.median(lat)), iterations=info["iterations"],
+                           inner_iterations=info["inner_iterations"], residual=info["residual"])
+
+    # the headline path: warmup, then the timed region with kernel timing on the library's stream
+    step(args.solver)
     for _ in range(args.warmup):
-        step(prec)
+        step(args.solver)
     ctx.synchronize()
     dep.lib().dep_ctx_set_profiling(ctx._h, 1)
     dep.lib().dep_ctx_reset_stats(ctx._h)
     lat = []
     t0 = time.perf_counter()
     for k in range(args.steps):
-        dt, sig, info, Fce, Fcm = step(prec)
+        dt, sig, info, Fce, Fcm = step(args.solver)
         lat.append(dt)
-        log(f"bench: {prec} step {k} {dt * 1e3:.1f} ms residual {info['residual']:.2e} "
-            f"iters {info['iterations']} fallback {info['fallback']}")
+        log(f"bench: {args.solver} step {k} {dt * 1e3:.1f} ms residual {info['residual']:.2e} "
+            f"iterations {info['iterations']} inner {info['inner_iterations']} fallback {info['fallback']}")
     ctx.synchronize()
     elapsed = time.perf_counter() - t0
     stats = kernel_stats(dep, ctx, list(KERNEL_PEAK))
     dep.lib().dep_ctx_set_profiling(ctx._h, 0)
     ms = elapsed / args.steps * 1e3
-    fscale = np.abs(Fce64).max()
-    errors = {
-        "sigma_rel": float(np.abs(sig - sig64).max() / np.abs(sig64).max()),
-        "force_element_rel": float(np.abs(Fce - Fce64).max() / fscale),
-        "force_total_rel": float(np.abs(Fcm - Fcm64).max() / np.abs(Fce64).sum()),
-    }
+    lat_ms = np.array(lat) * 1e3
+    paths[args.solver] = dict(errors(sig, Fce, Fcm), p50_ms=float(np.percentile(lat_ms, 50)),
+                              solves_per_sec=1e3 / ms, iterations=info["iterations"],
+                              inner_iterations=info["inner_iterations"], residual=info["residual"])
     cpu = None
     if not args.no_cpu_baseline:
         log("bench: cpu baseline (bounded sample)")
         cpu = cpu_baseline(p, s)
-    lat_ms = np.array(lat) * 1e3
+    value = 1e3 / ms
     out = {
         "metric": "solves/sec at N~12k BEM nodes, refined-FP32 vs FP64; p50/p99 solve latency",
-        "value": 1e3 / ms, "unit": "solves/sec", "n_gpus": 1, "steps": args.steps,
+        "value": value, "unit": "solves/sec", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": {"fp16": "fp16/fp32+f64", "bf16": "bf16/fp32+f64",
-                                       "fp64": "f64"}[prec],
+        "vs_baseline": None, "dtype": DTYPE[args.solver],
         "data": "synthetic (seeded mesh generator, config geometry)",
         "config": {"workload": f"{p.name}: sphere L4 + 24x24 plates, N={N}, E={p.n_elems}, "
-                               "assemble+factor+refine(1e-12)+MST per step",
-                   "precision": prec, "nodes": N, "elements": p.n_elems},
+                               "assemble+solve(FP64 residual 1e-12)+MST per step",
+                   "solver": args.solver, "nodes": N, "elements": p.n_elems},
         "p50_ms": float(np.percentile(lat_ms, 50)), "p99_ms": float(np.percentile(lat_ms, 99)),
-        "fp64_solves_per_sec": 1.0 / float(np.median(lat64)),
-        "fp64_p50_ms": float(np.median(lat64) * 1e3),
-        "speedup_vs_fp64": float(np.median(lat64) / np.median(lat)),
-        "fp64_gmres": None if not latg else dict(gm, solves_per_sec=1.0 / float(np.median(latg)),
-                                                  p50_ms=float(np.median(latg) * 1e3),
-                                                  speedup=float(np.median(latg) / np.median(lat))),
-        "gmres_ir_fp32": None if not lati else dict(gi, solves_per_sec=1.0 / float(np.median(lati)),
-                                                     p50_ms=float(np.median(lati) * 1e3)),
-        "errors_vs_fp64": errors,
+        "errors_vs_fp64_lu": errors(sig, Fce, Fcm),
+        "speedup_vs_fp64_lu": value / paths["lu-fp64"]["solves_per_sec"],
+        "speedup_vs_fp64_gmres": (value / paths["gmres-fp64"]["solves_per_sec"]) if "gmres-fp64" in paths else None,
+        "paths": paths,
         "kernels": {k: {"launches": v["launches"], "avg_ms": v["ms"] / v["launches"]} for k, v in stats.items()},
         "roofline": roofline(stats),
         "cpu_baseline": cpu,
     }
     if cpu:
-        out["speedup_vs_cpu"] = out["value"] / cpu["value"]
+        out["speedup_vs_cpu"] = value / cpu["value"]
     print(json.dumps(out), flush=True)
     s.close()
     g.close()

@@ -332,7 +332,7 @@ extern "C" int dep_geom_create(dep_ctx* c, int N, const double* xyz, int E, int 
     {
         const char* tsel = getenv("DEPSOLVER_ASM_TC");
         const int tc = tsel ? atoi(tsel) : DEP_TC;
-        g->tc = (tc == 16 || tc == 32 || tc == 64) ? tc : DEP_TC;
+        g->tc = (tc == 4 || tc == 8 || tc == 16 || tc == 32 || tc == 64) ? tc : DEP_TC;
     }
     g->n_patch = dep_cdiv(N, g->tc);
     std::vector<int> pptr(g->n_patch + 1, 0), pel;
@@ -1115,6 +1115,7 @@ static int ensure_p32(dep_geom* g)
 
 static int ensure_gmres_storage(dep_sys* s, int max_iters)
 {
+    if (max_iters > 1000) return dep_fail(s->g->ctx, DEP_ERR_ARG, "GMRES: at most 1000 iterations per run");
     if (s->gm_V && s->gm_cap >= max_iters) return DEP_OK;
     const size_t N = s->g->N, n2 = 2 * N;
     if (s->gm_V) {
@@ -1125,7 +1126,9 @@ static int ensure_gmres_storage(dep_sys* s, int max_iters)
     }
     if (!s->gm_vec) DEP_CHECK(dep_obj_alloc(s, &s->gm_vec, 3 * N + 3 * n2));
     DEP_CHECK(dep_obj_alloc(s, &s->gm_V, (size_t)(max_iters + 1) * n2));
-    DEP_CHECK(dep_obj_alloc(s, &s->gm_red, 2 + (size_t)(max_iters + 1) * (2 + DEP_DOT_CHUNKS)));
+    // reductions: h1 (cap + 2), h2 (cap + 2), DEP_UPD_BLOCKS norm partials, dot partials
+    DEP_CHECK(dep_obj_alloc(s, &s->gm_red,
+                            2 * (size_t)(max_iters + 2) + DEP_UPD_BLOCKS + (size_t)(max_iters + 2) * DEP_DOT_CHUNKS));
     s->gm_cap = max_iters;
     return DEP_OK;
 }
@@ -1171,14 +1174,16 @@ template <class MV>
 static int gmres_run(dep_sys* s, MV matvec, const double* b, bool x_zero, double* x, double* w, double tol, int M,
                      FILE* fg, GmresRun* out)
 {
+    (void)w;
     dep_ctx* c = s->g->ctx;
     const long n2 = 2L * s->g->N;
     double* V = s->gm_V;
     const int cap = s->gm_cap;
     double* red = s->gm_red;
-    double* h1 = red + 2;
-    double* h2 = h1 + (cap + 1);
-    double* part = h2 + (cap + 1);
+    double* h1 = red;                      // pass 1: h[0..i], |w|^2 before
+    double* h2 = h1 + (cap + 2);           // pass 2
+    double* np = h2 + (cap + 2);           // norm partials of the orthogonalised w
+    double* part = np + DEP_UPD_BLOCKS;    // dot-product partials
     if (M > cap) return dep_fail(c, DEP_ERR_ARG, "gmres_run: %d iterations exceed the workspace", M);
     if (x_zero) {
         DEP_HIP(c, hipMemsetAsync(x, 0, sizeof(double) * n2, c->stream));
@@ -1187,12 +1192,13 @@ static int gmres_run(dep_sys* s, MV matvec, const double* b, bool x_zero, double
         DEP_HIP(c, matvec(x, b, V));   // r0 = b - Op x0 (gmres_el.c:272-294)
     }
     double beta = 0.0;
-    DEP_HIP(c, launch_dots(c, 1, n2, V, V, red, part));
-    DEP_CHECK(download(c, &beta, red, 1));
+    DEP_HIP(c, launch_dots(c, 1, n2, V, V, h1, part));
+    DEP_CHECK(download(c, &beta, h1, 1));
     beta = sqrt(beta);
     out->beta = beta;
+    const size_t nred = 2 * (size_t)(cap + 2) + DEP_UPD_BLOCKS;
     std::vector<double> H((size_t)(M + 1) * (M + 1), 0.0), cs(M + 1, 0.0), sn(M + 1, 0.0), res(M + 2, 0.0),
-        hh(2 + 2 * (size_t)(cap + 1));
+        hh(nred);
 #define HH(r, col) H[(size_t)(r) * (M + 1) + (col)]
     res[0] = beta;
     double err0 = beta, err1 = beta;
@@ -1202,20 +1208,22 @@ static int gmres_run(dep_sys* s, MV matvec, const double* b, bool x_zero, double
     for (int i = 0; rc == 1 && i < M; i++) {
         iters++;
         double* vi = V + (size_t)i * n2;
-        DEP_HIP(c, matvec(vi, nullptr, w));
-        // CGS2 on the device; one transfer brings back both passes and both norms
-        DEP_HIP(c, launch_dots(c, 1, n2, w, w, red, part));
-        DEP_HIP(c, launch_dots(c, i + 1, n2, V, w, h1, part));
-        DEP_HIP(c, launch_update(c, i + 1, n2, V, h1, w));
-        DEP_HIP(c, launch_dots(c, i + 1, n2, V, w, h2, part));
-        DEP_HIP(c, launch_update(c, i + 1, n2, V, h2, w));
-        DEP_HIP(c, launch_dots(c, 1, n2, w, w, red + 1, part));
-        DEP_CHECK(download(c, hh.data(), red, 2 + 2 * (size_t)(cap + 1)));
-        const double wn = sqrt(hh[0]);
-        for (int j = 0; j <= i; j++) HH(j, i) = hh[2 + j] + hh[2 + (cap + 1) + j];
-        const double hn = sqrt(hh[1]);
+        double* wv = V + (size_t)(i + 1) * n2;   // the new vector is built in place in V[i+1]
+        DEP_HIP(c, matvec(vi, nullptr, wv));
+        // CGS2: each pass is one dot launch over V[0 .. i+1] (the last gives |w|^2) and one
+        // update launch that folds the reductions; one transfer per iteration
+        DEP_HIP(c, launch_dot_partials(c, i + 2, n2, V, wv, part));
+        DEP_HIP(c, launch_update_fused(c, i + 1, n2, V, part, wv, h1, nullptr));
+        DEP_HIP(c, launch_dot_partials(c, i + 2, n2, V, wv, part));
+        DEP_HIP(c, launch_update_fused(c, i + 1, n2, V, part, wv, h2, np));
+        DEP_CHECK(download(c, hh.data(), red, nred));
+        const double wn = sqrt(hh[i + 1]);
+        for (int j = 0; j <= i; j++) HH(j, i) = hh[j] + hh[(cap + 2) + j];
+        double hn2 = 0.0;
+        for (int k = 0; k < DEP_UPD_BLOCKS; k++) hn2 += hh[2 * (cap + 2) + k];
+        const double hn = sqrt(hn2);
         HH(i + 1, i) = hn;
-        if (hn != 0.0) DEP_HIP(c, launch_scale(c, n2, w, 1.0 / hn, V + (size_t)(i + 1) * n2));
+        if (hn != 0.0) DEP_HIP(c, launch_scale(c, n2, wv, 1.0 / hn, wv));
         for (int j = 0; j <= i - 1; j++) {
             const double g1 = cs[j] * HH(j, i) + sn[j] * HH(j + 1, i);
             const double g2 = -sn[j] * HH(j, i) + cs[j] * HH(j + 1, i);

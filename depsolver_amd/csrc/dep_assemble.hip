@@ -219,7 +219,15 @@ hipError_t assemble_impl(dep_geom* g)
     dim3 grid(g->n_patch, dep_cdiv(g->N, TR));
     hipEvent_t ev;
     prof_begin(g->ctx, &ev);
-    if (g->tc == 16)
+    if (g->tc == 4)
+        hipLaunchKernelGGL((k_assemble_regular<NPE, TR, 4>), grid, dim3(TR), 0, st, g->N, g->xyz, g->conn,
+                           g->kind, g->normal, (const double4*)g->q, g->patch_ptr, g->patch_elem,
+                           g->patch_slot, g->P, g->ldP);
+    else if (g->tc == 8)
+        hipLaunchKernelGGL((k_assemble_regular<NPE, TR, 8>), grid, dim3(TR), 0, st, g->N, g->xyz, g->conn,
+                           g->kind, g->normal, (const double4*)g->q, g->patch_ptr, g->patch_elem,
+                           g->patch_slot, g->P, g->ldP);
+    else if (g->tc == 16)
         hipLaunchKernelGGL((k_assemble_regular<NPE, TR, 16>), grid, dim3(TR), 0, st, g->N, g->xyz, g->conn,
                            g->kind, g->normal, (const double4*)g->q, g->patch_ptr, g->patch_elem,
                            g->patch_slot, g->P, g->ldP);

@@ -44,6 +44,40 @@ __global__ void k_dots_reduce(int nvec, const double* __restrict__ part, double*
     h[j] = s;
 }
 
+// One CGS pass with the reductions folded in: h[j] = sum of the DOT_CHUNKS
+// partials of j (fixed order), w -= sum_{j < nv} h[j] V[j]; block 0 writes
+// h[0 .. nv] (h[nv] is |w|^2 before the pass) to hout; with normp each block
+// writes the sum of squares of its part of the updated w to normp[block].
+#define UPD_MAXV 1024
+__global__ __launch_bounds__(256) void k_update_fused(int nv, long n2, const double* __restrict__ V,
+                                                      const double* __restrict__ part, double* __restrict__ w,
+                                                      double* __restrict__ hout, double* __restrict__ normp)
+{
+    __shared__ double h[UPD_MAXV + 1];
+    __shared__ double red[4];
+    for (int j = threadIdx.x; j <= nv; j += 256) {
+        double s = 0.0;
+        for (int c = 0; c < DOT_CHUNKS; c++) s += part[(long)j * DOT_CHUNKS + c];
+        h[j] = s;
+    }
+    __syncthreads();
+    if (blockIdx.x == 0)
+        for (int j = threadIdx.x; j <= nv; j += 256) hout[j] = h[j];
+    double sq = 0.0;
+    for (long t = (long)blockIdx.x * 256 + threadIdx.x; t < n2; t += (long)gridDim.x * 256) {
+        double v = w[t];
+        for (int j = 0; j < nv; j++) v -= h[j] * V[(long)j * n2 + t];
+        w[t] = v;
+        sq += v * v;
+    }
+    if (!normp) return;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) sq += __shfl_xor(sq, o);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = sq;
+    __syncthreads();
+    if (threadIdx.x == 0) normp[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
 // w -= sum_j h[j] V[j]
 __global__ void k_update(int nvec, long n2, const double* __restrict__ V, const double* __restrict__ h,
                          double* __restrict__ w)
@@ -153,6 +187,21 @@ hipError_t launch_to_f32(dep_ctx* c, long n, const double* in, float* out)
 hipError_t launch_rowscale(dep_ctx* c, int N, const double* d, const double* in, double* out)
 {
     hipLaunchKernelGGL(k_rowscale, dim3(dep_cdiv(N, 256)), dim3(256), 0, c->stream, N, d, in, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_dot_partials(dep_ctx* c, int nvec, long n2, const double* V, const double* w, double* part)
+{
+    hipLaunchKernelGGL(k_dots_partial, dim3(DOT_CHUNKS, nvec), dim3(256), 0, c->stream, n2, V, w, part);
+    return hipGetLastError();
+}
+
+hipError_t launch_update_fused(dep_ctx* c, int nv, long n2, const double* V, const double* part, double* w,
+                               double* hout, double* normp)
+{
+    if (nv > UPD_MAXV) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_update_fused, dim3(DEP_UPD_BLOCKS), dim3(256), 0, c->stream, nv, n2, V, part, w, hout,
+                       normp);
     return hipGetLastError();
 }
 

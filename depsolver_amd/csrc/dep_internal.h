@@ -22,7 +22,7 @@
 
 #include "../../include/depsolver_amd.h"
 
-#define DEP_TC 64   // columns per assembly patch
+#define DEP_TC 16   // columns per assembly patch (the LDS tile TR x DEP_TC sets occupancy)
 
 struct dep_kstat {
     long launches = 0;
@@ -185,6 +185,12 @@ hipError_t launch_rowscale(dep_ctx* c, int N, const double* d, const double* in,
 hipError_t launch_zmatvec32_w(dep_sys* s, const double* wr, const double* wi, const double* shift,
                               const double* x, const double* b, double* y);
 hipError_t launch_update(dep_ctx* c, int nvec, long n2, const double* V, const double* h, double* w);
+// partials only (DEP_DOT_CHUNKS per vector), consumed by launch_update_fused
+hipError_t launch_dot_partials(dep_ctx* c, int nvec, long n2, const double* V, const double* w, double* part);
+// CGS pass from dot partials (see dep_gmres.hip); DEP_UPD_BLOCKS workgroups, nv <= 1024
+#define DEP_UPD_BLOCKS 128
+hipError_t launch_update_fused(dep_ctx* c, int nv, long n2, const double* V, const double* part, double* w,
+                               double* hout, double* normp);
 hipError_t launch_scale(dep_ctx* c, long n2, const double* w, double alpha, double* out);
 hipError_t launch_combine(dep_ctx* c, int nvec, long n2, const double* V, const double* y, double* x);
 size_t mixed_bytes(int N);

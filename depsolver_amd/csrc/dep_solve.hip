@@ -31,9 +31,9 @@ __global__ void k_weight(int N, const double* __restrict__ P, long ldP, const do
 // storage, FP64 arithmetic): each wave takes ZROWS rows and walks their columns
 // in chunks of 256 (4 consecutive columns per lane, vector loads), so every
 // chunk of x is read once per ZROWS rows; a scalar loop finishes the columns
-// past the last full chunk.
-#define ZROWS 4
-template <class TP>
+// past the last full chunk.  ZROWS = 4 (8 rows per wave measured slower for the
+// FP32 blocks: fewer waves in flight).
+template <class TP, int ZROWS>
 __global__ __launch_bounds__(256) void k_zmatvec(int N, const TP* __restrict__ P, long ldP,
                                                  const double* __restrict__ wr,
                                                  const double* __restrict__ wi,
@@ -126,7 +126,7 @@ hipError_t launch_zmatvec_w(dep_sys* s, const double* wr, const double* wi, cons
     dep_geom* g = s->g;
     hipEvent_t ev;
     prof_begin(g->ctx, &ev);
-    hipLaunchKernelGGL(k_zmatvec<double>, dim3(dep_cdiv(g->N, 4 * ZROWS)), dim3(256), 0, g->ctx->stream, g->N,
+    hipLaunchKernelGGL((k_zmatvec<double, 4>), dim3(dep_cdiv(g->N, 16)), dim3(256), 0, g->ctx->stream, g->N,
                        g->P, g->ldP, wr, wi, shift, x, b, y);
     const double n = g->N;
     prof_end(g->ctx, ev, "zmatvec", 4.0 * n * n, 8.0 * n * n + 48.0 * n);
@@ -139,7 +139,7 @@ hipError_t launch_zmatvec32_w(dep_sys* s, const double* wr, const double* wi, co
     dep_geom* g = s->g;
     hipEvent_t ev;
     prof_begin(g->ctx, &ev);
-    hipLaunchKernelGGL(k_zmatvec<float>, dim3(dep_cdiv(g->N, 4 * ZROWS)), dim3(256), 0, g->ctx->stream, g->N,
+    hipLaunchKernelGGL((k_zmatvec<float, 4>), dim3(dep_cdiv(g->N, 16)), dim3(256), 0, g->ctx->stream, g->N,
                        g->P32, g->ldP, wr, wi, shift, x, b, y);
     const double n = g->N;
     prof_end(g->ctx, ev, "zmatvec32", 4.0 * n * n, 4.0 * n * n + 48.0 * n);
