@@ -55,6 +55,34 @@ def kernel_stats(dep, ctx, names):
     return out
 
 
+def calibration_copy(mib):
+    """One device-to-device copy of `mib` MiB (larger than the 256 MB last-level cache) through the HIP
+    runtime's copy kernel: the known-size reference for scaling FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(n)) or hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(n)):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, ctypes.c_size_t(n))
+    hip.hipDeviceSynchronize()
+    hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3)   # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed PMC profile, if one was recorded."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_config2.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        rec = json.load(f)
+    k = rec.get("kernels", {}).get(kernel)
+    return None if k is None else k["hbm_bytes_per_launch"]
+
+
 def roofline(stats):
     if not stats:
         return None
@@ -68,7 +96,7 @@ def roofline(stats):
     else:
         achieved = st["flops"] / st["launches"] / per_launch_s / 1e12
     return {"kernel": name, "bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
-            "frac": achieved / peak, "traffic": None, "launches": st["launches"],
+            "frac": achieved / peak, "traffic": measured_traffic(name), "launches": st["launches"],
             "avg_launch_ms": st["ms"] / st["launches"],
             "algorithmic_per_launch": (st["bytes"] if bound == "hbm" else st["flops"]) / st["launches"]}
 
@@ -130,9 +158,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--solver", default="gmres-ir-fp32", choices=SOLVERS,
                     help="the path timed for `value` (the others are reported beside it)")
-    ap.add_argument("--side-steps", type=int, default=2, help="timed steps of each other path")
+    ap.add_argument("--side-steps", type=int, default=2,
+                    help="timed steps of each other path (0: time only --solver and the FP64 LU yardstick)")
     ap.add_argument("--config", type=int, default=2, choices=[1, 2])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--pmc-calibration-mib", type=int, default=0,
+                    help="first copy this many MiB device-to-device (counter calibration runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -142,6 +173,8 @@ def main():
     import depsolver_amd as dep
     from depsolver_amd import mesh
 
+    if args.pmc_calibration_mib:
+        calibration_copy(args.pmc_calibration_mib)
     p = mesh.config2() if args.config == 2 else mesh.config1()
     N = p.n_nodes
     ctx = dep.Context(0)
@@ -181,7 +214,7 @@ def main():
     paths = {"lu-fp64": {"p50_ms": float(np.median(lat64) * 1e3), "solves_per_sec": 1.0 / float(np.median(lat64)),
                          "residual": info64["residual"]}}
     for name in SOLVERS:
-        if name in (args.solver, "lu-fp64"):
+        if name in (args.solver, "lu-fp64") or args.side_steps == 0:
             continue
         step(name)   # first call: workspace allocation, factor storage
         lat = []

@@ -12,6 +12,7 @@
 // gaussBksb / solverGMRES_el called on the matrix just assembled solve it
 // there without re-uploading it.  On failure they print the reference's
 // messages and exit(1), as the reference does.
+#include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -27,6 +28,14 @@ extern unsigned int nElems __attribute__((weak));
 extern unsigned int nInterfaces __attribute__((weak));
 extern unsigned int nInternalPoints __attribute__((weak));
 extern unsigned int nFPoints __attribute__((weak));
+extern unsigned int nCols __attribute__((weak));
+extern unsigned int nColType __attribute__((weak));
+extern unsigned int NX __attribute__((weak));
+extern unsigned int NY __attribute__((weak));
+extern unsigned int NZ __attribute__((weak));
+extern double deltaX __attribute__((weak));
+extern double deltaY __attribute__((weak));
+extern double deltaZ __attribute__((weak));
 extern FILE* file_log __attribute__((weak));
 }
 
@@ -306,35 +315,55 @@ extern "C" void solverGMRES_el(unsigned int preCond, unsigned int nInit, double*
 
 namespace {
 
-// postProcess_tria{3,6} (src/postProcess.c:256-470, headers :58-116): potential
-// and field at the internal points, the Maxwell-stress force (ANALYSIS 3, 4) or
-// the multipolar force at the force points (ANALYSIS 5-9), written in the
-// reference's formats.  Runs on the system of the preceding electricForm* call
-// with vB as its solution.
-int post(int npe, unsigned ANALYSIS, const char* type, const double* axis, double** XF, double** Xinner,
+// postProcess_tria{3,6} and vtkPostProcess_tria{3,6} (src/postProcess.c:256-738,
+// headers :58-116): potential and field at the internal points (STD files, or
+// one results.vtk structured-points file), the Maxwell-stress force (ANALYSIS
+// 3, 4) or the multipolar force at the force points (ANALYSIS 5-9), written in
+// the reference's formats.  The header rules follow headerSetup: an output
+// type other than STD or VTK opens no potential/field file (the driver passes
+// such a type when the input has no internal points).  Runs on the system of
+// the preceding electricForm* call with vB as its solution.
+int post(int npe, bool vtk, unsigned ANALYSIS, const char* type, const double* axis, double** XF, double** Xinner,
          double** vMatParam, const unsigned* vBCType, unsigned** mElems, const double* vB)
 {
     if (!st.sys) fatal("postProcess: no assembled system on the device (electricForm* runs first)");
-    if (!type || strcmp(type, "STD") != 0) fatal("postProcess: only the STD output type is available");
     if (ANALYSIS == 10 || ANALYSIS == 11)
         fatal("postProcess: the dipolar ellipsoid force (ANALYSIS 10/11) is not available in this build");
     dep_sys* sy = st.sys;
     const int E = st.geom->E;
+    const bool is_std = type && strcmp(type, "STD") == 0, is_vtk = type && strcmp(type, "VTK") == 0;
     check(dep_set_solution(sy, vB), "dep_set_solution");
+    const int nP = &nInternalPoints ? (int)nInternalPoints : 0;
     FILE* fp[3] = {nullptr, nullptr, nullptr};
-    if (ANALYSIS == 0) {
-        fp[0] = fopen("potential.dat", "w");
-        fprintf(fp[0], "#x\t\ty\t\tz\t\tRe[Pot(x,y,z)]\tIm[Pot(x,y,z)]\n");
-    } else if (ANALYSIS == 1) {
-        fp[1] = fopen("field.dat", "w");
-        fprintf(fp[1], "#x\t\ty\t\tz\t\tRe[Ex(x,y,z)]\tIm[Ex(x,y,z)]\tRe[Ey(x,y,z)]");
-        fprintf(fp[1], "\tIm[Ey(x,y,z)]\tRe[Ez(x,y,z)]\tIm[Ez(x,y,z)]\n");
-    } else {
-        fp[0] = fopen("potential.dat", "w");
-        fp[1] = fopen("field.dat", "w");
-        fprintf(fp[0], "#x\t\ty\t\tz\t\tRe[Pot(x,y,z)]\tIm[Pot(x,y,z)]\n");
-        fprintf(fp[1], "#x\t\ty\t\tz\t\tRe[Ex(x,y,z)] \tIm[Ex(x,y,z)]\tRe[Ey(x,y,z)]");
-        fprintf(fp[1], "\tIm[Ey(x,y,z)] \tRe[Ez(x,y,z)]\tIm[Ez(x,y,z)]\n");
+    if (is_std) {
+        if (ANALYSIS == 0) {
+            fp[0] = fopen("potential.dat", "w");
+            fprintf(fp[0], "#x\t\ty\t\tz\t\tRe[Pot(x,y,z)]\tIm[Pot(x,y,z)]\n");
+        } else if (ANALYSIS == 1) {
+            fp[1] = fopen("field.dat", "w");
+            fprintf(fp[1], "#x\t\ty\t\tz\t\tRe[Ex(x,y,z)]\tIm[Ex(x,y,z)]\tRe[Ey(x,y,z)]");
+            fprintf(fp[1], "\tIm[Ey(x,y,z)]\tRe[Ez(x,y,z)]\tIm[Ez(x,y,z)]\n");
+        } else {
+            fp[0] = fopen("potential.dat", "w");
+            fp[1] = fopen("field.dat", "w");
+            fprintf(fp[0], "#x\t\ty\t\tz\t\tRe[Pot(x,y,z)]\tIm[Pot(x,y,z)]\n");
+            fprintf(fp[1], "#x\t\ty\t\tz\t\tRe[Ex(x,y,z)] \tIm[Ex(x,y,z)]\tRe[Ey(x,y,z)]");
+            fprintf(fp[1], "\tIm[Ey(x,y,z)] \tRe[Ez(x,y,z)]\tIm[Ez(x,y,z)]\n");
+        }
+    } else if (is_vtk) {
+        if (!&NX || !&NY || !&NZ || !&deltaX || !&deltaY || !&deltaZ)
+            fatal("vtkPostProcess: the executable does not define NX/NY/NZ/deltaX/deltaY/deltaZ");
+        if (nP == 0) fatal("vtkPostProcess: VTK output needs internal points");
+        fp[0] = fopen("results.vtk", "w");
+        fprintf(fp[0], "# vtk DataFile Version 2.0\n");
+        fprintf(fp[0], "depSolver v2.0\n");
+        fprintf(fp[0], "ASCII\n\n");
+        fprintf(fp[0], "DATASET STRUCTURED_POINTS\n");
+        fprintf(fp[0], "DIMENSIONS %d %d %d\n", NX, NY, NZ);
+        fprintf(fp[0], "ORIGIN %e %e %e\n", Xinner[0][0], Xinner[0][1], Xinner[0][2]);
+        fprintf(fp[0], "SPACING %e %e %e\n", deltaX, deltaY, deltaZ);
+        fprintf(fp[0], "\n");
+        fprintf(fp[0], "POINT_DATA %d\n", NX * NY * NZ);
     }
     if (ANALYSIS == 3 || ANALYSIS == 4) {
         fp[2] = fopen("force-mst.dat", "w");
@@ -343,30 +372,42 @@ int post(int npe, unsigned ANALYSIS, const char* type, const double* axis, doubl
         fp[2] = fopen("force-mp.dat", "w");
         fprintf(fp[2], "#Xcm\t\tYcm\t\tZcm\t\tFXcm\t\tFYcm\t\tFZcm\n");
     }
-    const int nP = &nInternalPoints ? (int)nInternalPoints : 0;
-    // tria3 evaluates nothing for ANALYSIS 4, tria6 does (postProcess.c:298 and :414)
-    if (nP > 0 && !(npe == 3 && ANALYSIS == 4)) {
+    if (nP > 0 && (is_vtk || is_std)) {
         std::vector<double> pts(3 * (size_t)nP), pot, ef;
         for (int i = 0; i < nP; i++)
             for (int k = 0; k < 3; k++) pts[3 * i + k] = Xinner[i][k];
-        if (ANALYSIS != 1) {
+        // STD: tria3 evaluates nothing for ANALYSIS 4, tria6 does (postProcess.c:298 and :414)
+        const bool want_pot = vtk || (ANALYSIS != 1 && !(npe == 3 && ANALYSIS == 4));
+        const bool want_fld = vtk || (ANALYSIS != 0 && !(npe == 3 && ANALYSIS == 4));
+        if (want_pot) {
             pot.resize(2 * (size_t)nP);
             check(dep_post_potential(sy, nP, pts.data(), pot.data()), "potential");
         }
-        if (ANALYSIS != 0) {
+        if (want_fld) {
             ef.resize(6 * (size_t)nP);
             check(dep_post_field(sy, nP, pts.data(), ef.data()), "field");
         }
-        for (int i = 0; i < nP; i++) {
-            const double* x = &pts[3 * i];
-            if (!pot.empty()) {
-                fprintf(fp[0], "%e\t%e\t%e\t", x[0], x[1], x[2]);
-                fprintf(fp[0], "%e\t%e\n", pot[2 * i], pot[2 * i + 1]);
-            }
-            if (!ef.empty()) {
-                fprintf(fp[1], "%e\t%e\t%e\t", x[0], x[1], x[2]);
-                fprintf(fp[1], "%e\t%e\t%e\t", ef[6 * i], ef[6 * i + 1], ef[6 * i + 2]);
-                fprintf(fp[1], "%e\t%e\t%e\n", ef[6 * i + 3], ef[6 * i + 4], ef[6 * i + 5]);
+        if (vtk) {
+            fprintf(fp[0], "\nSCALARS Re[V] double\nLOOKUP_TABLE default\n");
+            for (int i = 0; i < nP; i++) fprintf(fp[0], "%e\n", pot[2 * i]);
+            fprintf(fp[0], "\nSCALARS Im[V] double\nLOOKUP_TABLE default\n");
+            for (int i = 0; i < nP; i++) fprintf(fp[0], "%e\n", pot[2 * i + 1]);
+            fprintf(fp[0], "\nVECTORS Re[E] double\n");
+            for (int i = 0; i < nP; i++) fprintf(fp[0], "%e\t%e\t%e\n", ef[6 * i], ef[6 * i + 2], ef[6 * i + 4]);
+            fprintf(fp[0], "\nVECTORS Im[E] double\n");
+            for (int i = 0; i < nP; i++) fprintf(fp[0], "%e\t%e\t%e\n", ef[6 * i + 1], ef[6 * i + 3], ef[6 * i + 5]);
+        } else {
+            for (int i = 0; i < nP; i++) {
+                const double* x = &pts[3 * i];
+                if (!pot.empty()) {
+                    fprintf(fp[0], "%e\t%e\t%e\t", x[0], x[1], x[2]);
+                    fprintf(fp[0], "%e\t%e\n", pot[2 * i], pot[2 * i + 1]);
+                }
+                if (!ef.empty()) {
+                    fprintf(fp[1], "%e\t%e\t%e\t", x[0], x[1], x[2]);
+                    fprintf(fp[1], "%e\t%e\t%e\t", ef[6 * i], ef[6 * i + 1], ef[6 * i + 2]);
+                    fprintf(fp[1], "%e\t%e\t%e\n", ef[6 * i + 3], ef[6 * i + 4], ef[6 * i + 5]);
+                }
             }
         }
     }
@@ -406,7 +447,7 @@ extern "C" int postProcess_tria3(unsigned int ANALYSIS, char* cOutputType, doubl
 {
     (void)mNodes;
     (void)vProbParam;
-    return post(3, ANALYSIS, cOutputType, axis, XF, Xinner, vMatParam, vBCType, mElems, vB);
+    return post(3, false, ANALYSIS, cOutputType, axis, XF, Xinner, vMatParam, vBCType, mElems, vB);
 }
 
 extern "C" int postProcess_tria6(unsigned int ANALYSIS, char* cOutputType, double* axis, double** XF,
@@ -415,18 +456,60 @@ extern "C" int postProcess_tria6(unsigned int ANALYSIS, char* cOutputType, doubl
 {
     (void)mNodes;
     (void)vProbParam;
-    return post(6, ANALYSIS, cOutputType, axis, XF, Xinner, vMatParam, vBCType, mElems, vB);
+    return post(6, false, ANALYSIS, cOutputType, axis, XF, Xinner, vMatParam, vBCType, mElems, vB);
 }
 
-// The VTK writer is not part of this library (src/postProcess.c:488-738).
-extern "C" int vtkPostProcess_tria3(unsigned int, char*, double*, double**, double**, double**, unsigned int**,
-                                    double**, double*, unsigned int*, double*)
+// postProcessSetup (src/postProcess.c:123-190): with STD output and columns
+// defined, keep only the internal points outside every column (circular
+// columns for nColType 1, square otherwise), flag each point in its fourth
+// entry (+1 kept, -1 dropped) and update nInternalPoints; otherwise copy the
+// points.  depSolver.c calls it before postProcess_tria*.
+extern "C" int postProcessSetup(char* cOutputType, double** xCols, double** XinnerTemp, double** Xinner)
 {
-    fatal("vtkPostProcess_tria3: VTK output is not available in this build");
+    if (!&nInternalPoints || nInternalPoints == 0) return 0;
+    const unsigned nc = &nCols ? nCols : 0;
+    if (nc > 0 && cOutputType && strcmp(cOutputType, "STD") == 0) {
+        const bool round = &nColType && nColType == 1;
+        unsigned kept = 0;
+        for (unsigned i = 0; i < nInternalPoints; i++) {
+            const double x = XinnerTemp[i][0], y = XinnerTemp[i][1];
+            unsigned outside = 0;
+            for (unsigned j = 0; j < nc; j++) {
+                const double dx = fabs(x - xCols[j][0]), dy = fabs(y - xCols[j][1]), rad = xCols[j][2];
+                if (round ? (dx * dx + dy * dy > rad * rad) : (dx > rad && dy > rad)) outside++;
+            }
+            XinnerTemp[i][3] = outside == nc ? 1.0 : -1.0;
+            if (outside == nc) {
+                for (int k = 0; k < 3; k++) Xinner[kept][k] = XinnerTemp[i][k];
+                kept++;
+            }
+        }
+        nInternalPoints = kept;
+    } else {
+        for (unsigned i = 0; i < nInternalPoints; i++)
+            for (int k = 0; k < 3; k++) Xinner[i][k] = XinnerTemp[i][k];
+    }
+    return 0;
 }
 
-extern "C" int vtkPostProcess_tria6(unsigned int, char*, double*, double**, double**, double**, unsigned int**,
-                                    double**, double*, unsigned int*, double*)
+// vtkPostProcess_tria{3,6} (src/postProcess.c:488, :626): potential and field
+// at the internal-point grid in one structured-points results.vtk file, then
+// the forces as postProcess_tria*; the driver calls these for VTK output and
+// when the input has no internal points.
+extern "C" int vtkPostProcess_tria3(unsigned int ANALYSIS, char* cOutputType, double* axis, double** XF,
+                                    double** Xinner, double** mNodes, unsigned int** mElems, double** vMatParam,
+                                    double* vProbParam, unsigned int* vBCType, double* vB)
 {
-    fatal("vtkPostProcess_tria6: VTK output is not available in this build");
+    (void)mNodes;
+    (void)vProbParam;
+    return post(3, true, ANALYSIS, cOutputType, axis, XF, Xinner, vMatParam, vBCType, mElems, vB);
+}
+
+extern "C" int vtkPostProcess_tria6(unsigned int ANALYSIS, char* cOutputType, double* axis, double** XF,
+                                    double** Xinner, double** mNodes, unsigned int** mElems, double** vMatParam,
+                                    double* vProbParam, unsigned int* vBCType, double* vB)
+{
+    (void)mNodes;
+    (void)vProbParam;
+    return post(6, true, ANALYSIS, cOutputType, axis, XF, Xinner, vMatParam, vBCType, mElems, vB);
 }

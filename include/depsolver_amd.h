@@ -11,10 +11,11 @@
  *     independent; one context must not be used by two threads at once.
  *
  *  2. Drop-in shims with the reference's exact signatures (bottom of this
- *     file): electricForm[Full]A_tria{3,6}, gaussBksb, solverGMRES_el and
- *     postProcess_tria{3,6} as src/depSolver.c calls them, reading the same
- *     globals (nNodes, nElems, nInterfaces, nInternalPoints, nFPoints) from
- *     the executable; vtkPostProcess_tria{3,6} exit with a message.
+ *     file): electricForm[Full]A_tria{3,6}, gaussBksb, solverGMRES_el,
+ *     postProcessSetup and [vtk]postProcess_tria{3,6} as src/depSolver.c calls
+ *     them, reading the same globals (nNodes, nElems, nInterfaces,
+ *     nInternalPoints, nFPoints, nCols, nColType, NX, NY, NZ, deltaX/Y/Z)
+ *     from the executable.
  *
  * Conventions of the native API
  *   - node ids in `conn` are 0-based; material and interface ids in
@@ -173,8 +174,13 @@ int postProcess_tria3(unsigned int ANALYSIS, char *cOutputType, double *axis, do
 int postProcess_tria6(unsigned int ANALYSIS, char *cOutputType, double *axis, double **XF,
                       double **Xinner, double **mNodes, unsigned int **mElems, double **vMatParam,
                       double *vProbParam, unsigned int *vBCType, double *vB);
-/* src/postProcess.c:488 and :626: the VTK writer is not part of this library; these
- * exit with a message, as the reference does on errors. */
+/* src/postProcess.c:123: internal-point column filter for STD output (reads nCols, nColType;
+ * updates nInternalPoints), as depSolver.c calls it before postProcess_tria*. */
+int postProcessSetup(char *cOutputType, double **xCols, double **XinnerTemp, double **Xinner);
+/* src/postProcess.c:488 and :626: potential and field at the internal-point grid
+ * in one structured-points results.vtk (reads NX, NY, NZ, deltaX/Y/Z), then the forces
+ * as postProcess_tria*; src/depSolver.c calls these for VTK output and for inputs without
+ * internal points. */
 int vtkPostProcess_tria3(unsigned int ANALYSIS, char *cOutputType, double *axis, double **XF,
                          double **Xinner, double **mNodes, unsigned int **mElems, double **vMatParam,
                          double *vProbParam, unsigned int *vBCType, double *vB);

@@ -30,6 +30,9 @@ def build(reference: bool = True) -> None:
     subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], check=True)
     if reference and os.path.isdir(REF_SRC):
         subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
+        # the reference's driver linked against the library (needs libdepsolver_amd.so built)
+        if os.path.exists(os.path.join(os.path.dirname(HERE), "depsolver_amd", "libdepsolver_amd.so")):
+            subprocess.run(["make", "-s", "-B", "-C", HERE, "dropin"], check=True)
 
 
 def _c64(a):

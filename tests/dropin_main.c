@@ -3,7 +3,9 @@
  * libdepsolver_amd.so the way src/depSolver.c:284-310 does: globals nNodes,
  * nElems, nInterfaces and file_log defined here, row-pointer matrices,
  * 1-based connectivity, then electricForm[Full]A_tria{3,6} and gaussBksb or
- * solverGMRES_el.  Input: a text problem written by the test; output: the
+ * solverGMRES_el, then postProcess_tria* ("STD" files) and, when internal
+ * points are given, vtkPostProcess_tria* on an nInternalPoints x 1 x 1 grid
+ * (results.vtk).  Input: a text problem written by the test; output: the
  * solution vector (2N values), one per line.
  *
  *   dropin_main problem.txt solution.txt lu|gmres
@@ -16,6 +18,8 @@
 
 FILE *file_log;
 unsigned int nNodes, nElems, nInterfaces, nInternalPoints, nFPoints;
+unsigned int NX, NY, NZ;            /* the VTK grid (src/depSolver.c globals) */
+double deltaX, deltaY, deltaZ;
 
 static double **dmat(unsigned r, unsigned c)
 {
@@ -78,5 +82,14 @@ int main(int argc, char **argv)
     char std_type[] = "STD";
     if (npe == 6) postProcess_tria6(3, std_type, axis, NULL, Xin, nodes, el, mats, prob, bct, B);
     else postProcess_tria3(3, std_type, axis, NULL, Xin, nodes, el, mats, prob, bct, B);
+    if (nInternalPoints) {
+        char vtk_type[] = "VTK";
+        NX = nInternalPoints;
+        NY = NZ = 1;
+        deltaX = 0.5;
+        deltaY = deltaZ = 1.0;
+        if (npe == 6) vtkPostProcess_tria6(3, vtk_type, axis, NULL, Xin, nodes, el, mats, prob, bct, B);
+        else vtkPostProcess_tria3(3, vtk_type, axis, NULL, Xin, nodes, el, mats, prob, bct, B);
+    }
     return 0;
 }

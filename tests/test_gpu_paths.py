@@ -212,3 +212,62 @@ def test_reference_signature_shims(tmp_path, orc, solver):
     sel = p.bctype[p.conn[:, 0]] == 6
     assert np.abs(rows[1:, 3:] - Fce[sel]).max() <= 1e-6 * np.abs(Fce).max()
     assert np.abs(rows[1:, :3] - Xce[sel]).max() <= 1e-6 * np.abs(Xce).max()
+    # vtkPostProcess_tria6 on a 2 x 1 x 1 grid: header, then Re/Im V and Re/Im E per point
+    vtk = open(tmp_path / "results.vtk").read().split("\n")
+    assert vtk[:5] == ["# vtk DataFile Version 2.0", "depSolver v2.0", "ASCII", "",
+                       "DATASET STRUCTURED_POINTS"]
+    assert vtk[5] == "DIMENSIONS 2 1 1"
+    assert vtk[6] == "ORIGIN %e %e %e" % tuple(pts[0])
+    assert vtk[7] == "SPACING %e %e %e" % (0.5, 1.0, 1.0)
+    assert vtk[9] == "POINT_DATA 2"
+    blk = {}
+    for name in ("SCALARS Re[V] double", "SCALARS Im[V] double", "VECTORS Re[E] double", "VECTORS Im[E] double"):
+        i = vtk.index(name) + (2 if name.startswith("SCALARS") else 1)
+        blk[name.split()[1]] = np.array([[float(v) for v in vtk[i + k].split()] for k in range(len(pts))])
+    assert np.array_equal(blk["Re[V]"][:, 0], pot[:, 3]) and np.array_equal(blk["Im[V]"][:, 0], pot[:, 4])
+    assert np.array_equal(blk["Re[E]"], fld[:, [3, 5, 7]]) and np.array_equal(blk["Im[E]"], fld[:, [4, 6, 8]])
+
+
+DROPIN = os.path.join(ROOT, "oracle", "_ref", "depSolver-dropin")
+
+
+@pytest.mark.skipif(not os.path.exists(DROPIN + "-sft"), reason="reference driver not built here")
+def test_reference_driver_linked_to_the_library_reproduces_dep_trap_2(tmp_path):
+    # the reference's own depSolver-shift.c main, with assembly_el/gmres_el/postProcess from the library
+    import shutil
+    for f in ("input.bem", "nodes.bem", "elems.bem", "bcs.bem"):
+        shutil.copy(os.path.join(GOLD, "dep-trap-2", f), tmp_path / f)
+    r = subprocess.run([DROPIN + "-sft"], capture_output=True, text=True, timeout=240, cwd=tmp_path)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    want = np.loadtxt(os.path.join(GOLD, "dep-trap-2", "solution.dat"))
+    got = np.loadtxt(tmp_path / "solution.dat")
+    assert np.abs(got[:, :3] - want[:, :3]).max() <= 1e-6 * np.abs(want[:, :3]).max()
+    assert np.abs(got[:, 3:] - want[:, 3:]).max() <= 1e-5 * np.abs(want[:, 3:]).max()
+    W = np.loadtxt(os.path.join(GOLD, "dep-trap-2", "force-mst.dat"))
+    G = np.loadtxt(tmp_path / "force-mst.dat")
+    assert W.shape == G.shape
+    assert np.abs(G[1:, 3:] - W[1:, 3:]).max() <= 1e-4 * np.abs(W[1:, 3:]).max()
+
+
+@pytest.mark.skipif(not os.path.exists(DROPIN), reason="reference driver not built here")
+def test_reference_driver_linked_to_the_library_lu_path(tmp_path, orc):
+    # gaussBksb path with internal points and the MST force through the reference's main
+    p = mesh.sphere_between_plates(1, 4)
+    pts = np.array([[0.1, 0.2, 1.6], [0.0, 0.0, 0.0], [1.0, -2.0, 2.0]])
+    mesh.write_bem(str(tmp_path), p, solver="gaussBksb", analysis=3, internal_points=pts)
+    r = subprocess.run([DROPIN], capture_output=True, text=True, timeout=240, cwd=tmp_path)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    A, b = orc.assemble(p, True)
+    x = orc.gauss_bksb(A, b)
+    N = p.n_nodes
+    sol = np.loadtxt(tmp_path / "solution.dat")
+    assert np.abs(sol[:, 3] - x[:N]).max() <= 1e-6 * np.abs(x).max()
+    pot = np.loadtxt(tmp_path / "potential.dat")
+    fld = np.loadtxt(tmp_path / "field.dat")
+    for k, pt in enumerate(pts):
+        wp, wf = orc.potential(p, x, pt), orc.field(p, x, pt)
+        assert np.abs(pot[k, 3:] - wp).max() <= 1e-6 * np.abs(wp).max()
+        assert np.abs(fld[k, 3:] - wf).max() <= 1e-6 * np.abs(wf).max()
+    Fce, Xce, Fcm, Xcm = orc.force_mst(p, x)
+    rows = np.loadtxt(tmp_path / "force-mst.dat")
+    assert np.abs(rows[1:, 3:] - Fce[p.bctype[p.conn[:, 0]] == 6]).max() <= 1e-6 * np.abs(Fce).max()
