@@ -36,7 +36,8 @@ PEAKS = {  # MI355X published dense peaks
     "hbm": (8000.0, "GB/s", "hbm"),
 }
 KERNEL_PEAK = {"lu_gemm_fp64": "fp64", "lu_gemm_mixed": "fp16", "assemble_regular": "fp64",
-               "points_field": "fp64", "points_potential": "fp64", "zmatvec": "hbm", "zmatvec32": "hbm"}
+               "points_field": "fp64", "points_potential": "fp64", "zmatvec": "hbm", "zmatvec32": "hbm",
+               "bgemm32": "fp32", "bgemm64": "fp64"}
 
 
 def log(*a):
@@ -146,6 +147,69 @@ def cpu_baseline(p, s, rows=48, pivots=6, points=24):
             "seconds_per_solve": total}
 
 
+def sweep_bench(dep, ctx, steps, nf=64):
+    """Frequency sweep on the config-3 geometry (sphere L3 + 16x16 plates, N = 4,740): nf log-spaced
+    frequencies 1 kHz - 100 MHz, geometric blocks assembled once, all systems solved together
+    (refined FP32 GMRES, batched products) + total MST force per frequency; beside it the same
+    frequencies solved one at a time (dep_gmres_ir) and errors against the FP64 LU."""
+    from depsolver_amd import mesh
+    p = mesh.config3_geometry()
+    freqs = mesh.sweep_frequencies(nf)
+    g = dep.Geometry(ctx, p)
+    t = time.perf_counter()
+    dep.lib().dep_geom_reassemble(g._h)
+    ctx.synchronize()
+    t_asm = time.perf_counter() - t
+    w = dep.Sweep(g, freqs)
+    w.solve(tol=1e-12, want_sigma=False)
+    ctx.synchronize()
+    dep.lib().dep_ctx_set_profiling(ctx._h, 1)
+    dep.lib().dep_ctx_reset_stats(ctx._h)
+    lat = []
+    for k in range(steps):
+        t0 = time.perf_counter()
+        sig, info = w.solve(tol=1e-12)
+        F, Xcm = w.mst()
+        lat.append(time.perf_counter() - t0)
+        log(f"bench: sweep step {k} {lat[-1] * 1e3:.1f} ms ({nf} frequencies), max residual "
+            f"{max(x['residual'] for x in info):.2e}, inner iterations "
+            f"{min(x['inner_iterations'] for x in info)}-{max(x['inner_iterations'] for x in info)}")
+    ctx.synchronize()
+    stats = kernel_stats(dep, ctx, ["bgemm32", "bgemm64"])
+    dep.lib().dep_ctx_set_profiling(ctx._h, 0)
+    s = dep.System(g)
+    s.set_frequency(freqs[0])
+    s.gmres_ir(tol=1e-12)
+    t0 = time.perf_counter()
+    for f in freqs:
+        s.set_frequency(f)
+        s.gmres_ir(tol=1e-12)
+        s.mst()
+    t_seq = time.perf_counter() - t0
+    err_s, err_f = 0.0, 0.0
+    for k in (0, nf // 3, 2 * nf // 3, nf - 1):
+        s.set_frequency(freqs[k])
+        s.factor("fp64")
+        want, _ = s.solve(tol=1e-13, max_steps=3)
+        Fce, Xce, Fcm, Xc = s.mst()
+        err_s = max(err_s, float(np.abs(sig[k] - want).max() / np.abs(want).max()))
+        err_f = max(err_f, float(np.abs(F[k] - Fcm).max() / np.abs(Fce).sum()))
+    s.close()
+    w.close()
+    g.close()
+    p50 = float(np.median(lat))
+    return {"workload": f"config3: sphere L3 + 16x16 plates, N={p.n_nodes}, E={p.n_elems}, {nf} frequencies "
+                        "1 kHz-100 MHz, one assembly, refined FP32 GMRES (FP64 residual 1e-12) + MST per frequency",
+            "frequencies": nf, "p50_ms": p50 * 1e3, "solves_per_sec": nf / p50,
+            "sequential_solves_per_sec": nf / t_seq, "batched_speedup": t_seq / p50,
+            "assembly_ms": t_asm * 1e3,
+            "inner_iterations": [int(min(x["inner_iterations"] for x in info)),
+                                 int(max(x["inner_iterations"] for x in info))],
+            "errors_vs_fp64_lu": {"sigma_rel": err_s, "force_total_rel": err_f},
+            "kernels": {k: {"launches": v["launches"], "avg_ms": v["ms"] / v["launches"]} for k, v in stats.items()},
+            "roofline": roofline(stats)}
+
+
 SOLVERS = ("gmres-ir-fp32", "lu-fp16", "lu-bf16", "gmres-fp64", "lu-fp64")
 DTYPE = {"gmres-ir-fp32": "fp32/f64", "lu-fp16": "fp16/fp32+f64", "lu-bf16": "bf16/fp32+f64",
          "gmres-fp64": "f64", "lu-fp64": "f64"}
@@ -162,6 +226,8 @@ def main():
                     help="timed steps of each other path (0: time only --solver and the FP64 LU yardstick)")
     ap.add_argument("--config", type=int, default=2, choices=[1, 2])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--sweep-steps", type=int, default=2,
+                    help="timed 64-frequency sweeps on the config-3 geometry (0: skip)")
     ap.add_argument("--pmc-calibration-mib", type=int, default=0,
                     help="first copy this many MiB device-to-device (counter calibration runs)")
     args = ap.parse_args()
@@ -250,6 +316,10 @@ def main():
     paths[args.solver] = dict(errors(sig, Fce, Fcm), p50_ms=float(np.percentile(lat_ms, 50)),
                               solves_per_sec=1e3 / ms, iterations=info["iterations"],
                               inner_iterations=info["inner_iterations"], residual=info["residual"])
+    sweep = None
+    if args.sweep_steps > 0:
+        log("bench: frequency sweep (config 3)")
+        sweep = sweep_bench(dep, ctx, args.sweep_steps)
     cpu = None
     if not args.no_cpu_baseline:
         log("bench: cpu baseline (bounded sample)")
@@ -272,6 +342,7 @@ def main():
         "kernels": {k: {"launches": v["launches"], "avg_ms": v["ms"] / v["launches"]} for k, v in stats.items()},
         "roofline": roofline(stats),
         "cpu_baseline": cpu,
+        "sweep": sweep,
     }
     if cpu:
         out["speedup_vs_cpu"] = value / cpu["value"]

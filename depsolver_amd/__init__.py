@@ -81,6 +81,12 @@ def lib():
         "dep_post_field": ([_p, i, _f64p, _f64p], i),
         "dep_post_mst": ([_p, d, _f64p, _f64p, _f64p, _f64p], i),
         "dep_post_multipole": ([_p, i, d, i, _f64p, _f64p], i),
+        "dep_sweep_create": ([_p, i, _f64p, i, _u32p, i, _f64p, _f64p, ctypes.POINTER(_p)], i),
+        "dep_sweep_destroy": ([_p], i),
+        "dep_sweep_solve": ([_p, d, d, i, i, _p, _p], i),
+        "dep_sweep_get_solution": ([_p, i, _f64p], i),
+        "dep_sweep_system": ([_p, i, ctypes.POINTER(_p)], i),
+        "dep_sweep_mst": ([_p, d, _f64p, _f64p], i),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -243,4 +249,47 @@ class System:
     def close(self):
         if self._h:
             lib().dep_sys_destroy(self._h)
+            self._h = _p()
+
+
+class Sweep:
+    """Frequency sweep on one geometry: nf <= 64 systems sharing the geometric
+    blocks, solved together by refined FP32 GMRES (dep_sweep)."""
+
+    def __init__(self, geom: Geometry, freqs, rhs=None):
+        p = geom.problem
+        self.geom, self.ctx, self.n = geom, geom.ctx, p.n_nodes
+        self.freqs = _c64(freqs).reshape(-1)
+        rhs = p.rhs() if rhs is None else _c64(rhs)
+        self._h = _p()
+        self.ctx.check(lib().dep_sweep_create(geom._h, len(p.mats), _c64(p.mats), len(p.ifaces),
+                                              np.ascontiguousarray(p.ifaces.reshape(-1), dtype=np.uint32),
+                                              len(self.freqs), self.freqs, rhs, ctypes.byref(self._h)),
+                       "dep_sweep_create")
+
+    def solve(self, tol=1e-12, inner_tol=1e-6, max_inner=200, max_outer=10, want_sigma=True):
+        nf = len(self.freqs)
+        sigma = np.zeros((nf, 2 * self.n)) if want_sigma else None
+        info = (SolveInfo * nf)()
+        self.ctx.check(lib().dep_sweep_solve(self._h, tol, inner_tol, max_inner, max_outer,
+                                             None if sigma is None else sigma.ctypes.data,
+                                             ctypes.cast(info, _p)), "dep_sweep_solve")
+        return sigma, [x.as_dict() for x in info]
+
+    def solution(self, f):
+        s = np.zeros(2 * self.n)
+        self.ctx.check(lib().dep_sweep_get_solution(self._h, f, s), "dep_sweep_get_solution")
+        return s
+
+    def mst(self, eps=None):
+        """Total Maxwell-stress force per frequency (nf x 3) and its centre."""
+        p = self.geom.problem
+        eps = p.mats[0, 1] * 8.854187817e-12 if eps is None else eps
+        F, X = np.zeros((len(self.freqs), 3)), np.zeros(3)
+        self.ctx.check(lib().dep_sweep_mst(self._h, eps, F, X), "dep_sweep_mst")
+        return F, X
+
+    def close(self):
+        if self._h:
+            lib().dep_sweep_destroy(self._h)
             self._h = _p()

@@ -450,45 +450,55 @@ extern "C" int dep_geom_get_blocks(dep_geom* g, double* P)
 // ---------------------------------------------------------------------------
 
 // per-interface weights A_k, B_k of src/assembly_el.c:496-512
-static void iface_weights(const dep_sys* s, double freq, std::vector<double>& A, std::vector<double>& B)
+static void iface_weights(int nI, const double* mats, const uint32_t* ifaces, double freq, std::vector<double>& A,
+                          std::vector<double>& B)
 {
     const double W = freq * DEP_PI2 * DEP_EPS0;
-    A.assign(s->nI, 0.0);
-    B.assign(s->nI, 0.0);
-    for (int i = 0; i < s->nI; i++) {
-        const int m1 = (int)s->ifaces[2 * i] - 1, m2 = (int)s->ifaces[2 * i + 1] - 1;
-        const double SigTot = s->mats[2 * m1] + s->mats[2 * m2];
-        const double SigDif = s->mats[2 * m1] - s->mats[2 * m2];
-        const double EpsTot = W * (s->mats[2 * m1 + 1] + s->mats[2 * m2 + 1]);
-        const double EpsDif = W * (s->mats[2 * m1 + 1] - s->mats[2 * m2 + 1]);
+    A.assign(nI, 0.0);
+    B.assign(nI, 0.0);
+    for (int i = 0; i < nI; i++) {
+        const int m1 = (int)ifaces[2 * i] - 1, m2 = (int)ifaces[2 * i + 1] - 1;
+        const double SigTot = mats[2 * m1] + mats[2 * m2];
+        const double SigDif = mats[2 * m1] - mats[2 * m2];
+        const double EpsTot = W * (mats[2 * m1 + 1] + mats[2 * m2 + 1]);
+        const double EpsDif = W * (mats[2 * m1 + 1] - mats[2 * m2 + 1]);
         const double D = kC / (SigTot * SigTot + EpsTot * EpsTot);
         A[i] = D * (SigTot * SigDif + EpsTot * EpsDif);
         B[i] = D * (SigDif * EpsTot - SigTot * EpsDif);
     }
 }
 
+int dep_row_weights(dep_geom* g, int nI, const double* mats, const uint32_t* ifaces, double freq, double* wr,
+                    double* wi, double* sh)
+{
+    std::vector<double> A, B;
+    iface_weights(nI, mats, ifaces, freq, A, B);
+    for (int i = 0; i < g->N; i++) {
+        wr[i] = wi[i] = sh[i] = 0.0;
+        if (g->hkind[i] == 0) {
+            wr[i] = kC;
+        } else {
+            const int k = g->hiface[i];
+            if (k < 0 || k >= nI)
+                return dep_fail(g->ctx, DEP_ERR_ARG, "interface node %d has interface id %d outside [0,%d)",
+                                g->perm[i], k, nI);
+            // Z = F - iG with F = A_k P, G = B_k P (assembly_el.c:565-573)
+            wr[i] = A[k];
+            wi[i] = -B[k];
+            sh[i] = -0.5 / DEP_EPS0;
+        }
+    }
+    return DEP_OK;
+}
+
 static int set_weights(dep_sys* s, double freq)
 {
     dep_geom* g = s->g;
-    std::vector<double> A, B;
-    iface_weights(s, freq, A, B);
     s->hwr.assign(g->N, 0.0);
     s->hwi.assign(g->N, 0.0);
     s->hshift.assign(g->N, 0.0);
-    for (int i = 0; i < g->N; i++) {
-        if (g->hkind[i] == 0) {
-            s->hwr[i] = kC;
-        } else {
-            const int k = g->hiface[i];
-            if (k < 0 || k >= s->nI)
-                return dep_fail(g->ctx, DEP_ERR_ARG, "interface node %d has interface id %d outside [0,%d)",
-                                g->perm[i], k, s->nI);
-            // Z = F - iG with F = A_k P, G = B_k P (assembly_el.c:565-573)
-            s->hwr[i] = A[k];
-            s->hwi[i] = -B[k];
-            s->hshift[i] = -0.5 / DEP_EPS0;
-        }
-    }
+    DEP_CHECK(dep_row_weights(g, s->nI, s->mats.data(), s->ifaces.data(), freq, s->hwr.data(), s->hwi.data(),
+                              s->hshift.data()));
     s->freq = freq;
     DEP_CHECK(upload(g->ctx, s->wr, s->hwr.data(), g->N));
     DEP_CHECK(upload(g->ctx, s->wi, s->hwi.data(), g->N));
@@ -864,6 +874,7 @@ int dep_points_prepare(dep_geom* g, int m, const double* pts, dep_points* P)
     if (r == DEP_OK) r = dep_obj_alloc(&t, &P->pair_ek, pek.size());
     if (r == DEP_OK) r = dep_obj_alloc(&t, &P->partial, (size_t)P->nchunk * m * 6);
     if (r == DEP_OK) r = dep_obj_alloc(&t, &P->sing, ex.size() * 6);
+    if (r == DEP_OK) r = dep_obj_alloc(&t, &P->coef, ex.size() * (3 * (size_t)g->npe + 3));
     if (r == DEP_OK) r = dep_obj_alloc(&t, &P->out, (size_t)m * 6);
     if (r == DEP_OK) r = upload(c, P->pts, pts, 3 * (size_t)m);
     if (r == DEP_OK) r = upload(c, P->excl_ptr, ptr.data(), ptr.size());
@@ -955,25 +966,53 @@ static int mst_cache(dep_geom* g, dep_mst_cache** out)
     return DEP_OK;
 }
 
+int dep_mst_result_size(dep_geom* g, size_t* n)
+{
+    dep_mst_cache* mc = nullptr;
+    DEP_CHECK(mst_cache(g, &mc));
+    *n = 6 * (size_t)std::max(mc->nf, 1) + 3;
+    return DEP_OK;
+}
+
+int dep_mst_center(dep_geom* g, double* Xcm)
+{
+    dep_mst_cache* mc = nullptr;
+    DEP_CHECK(mst_cache(g, &mc));
+    for (int k = 0; k < 3; k++) Xcm[k] = mc->Xcm[k];
+    return DEP_OK;
+}
+
+int dep_mst_enqueue(dep_sys* s, double eps, double* dres)
+{
+    dep_geom* g = s->g;
+    dep_ctx* c = g->ctx;
+    dep_mst_cache* mc = nullptr;
+    DEP_CHECK(mst_cache(g, &mc));
+    const size_t nf1 = std::max(mc->nf, 1);
+    double* dF = dres ? dres : mc->res;
+    double* dX = dF + 3 * nf1;
+    double* dcm = dX + 3 * nf1;
+    hipError_t e = hipMemsetAsync(mc->Efn, 0, sizeof(double) * 3 * g->N, c->stream);
+    if (e == hipSuccess) e = launch_sigma_at_points(s);
+    if (e == hipSuccess) e = launch_points(s, 1, &mc->P, mc->P.out);
+    if (e == hipSuccess)
+        e = launch_mst(s, &mc->P, mc->P.out, mc->dnodes, mc->nf, mc->dfel, 0.5 * eps, mc->Efn, dF, dX, dcm);
+    if (e != hipSuccess) return dep_fail(c, DEP_ERR_HIP, "MST kernels: %s", hipGetErrorString(e));
+    return DEP_OK;
+}
+
 extern "C" int dep_post_mst(dep_sys* s, double eps, double* Fce, double* Xce, double* Fcm, double* Xcm)
 {
     if (!s || !Fce || !Xce || !Fcm || !Xcm) return DEP_ERR_ARG;
     dep_geom* g = s->g;
     dep_ctx* c = g->ctx;
-    const int N = g->N, E = g->E;
+    const int E = g->E;
     dep_mst_cache* mc = nullptr;
     DEP_CHECK(mst_cache(g, &mc));
     const int nf = mc->nf;
     const size_t nf1 = std::max(nf, 1);
     for (int k = 0; k < 3; k++) Xcm[k] = mc->Xcm[k];
-    double* dF = mc->res;
-    double* dX = dF + 3 * nf1;
-    double* dcm = dX + 3 * nf1;
-    hipError_t e = hipMemsetAsync(mc->Efn, 0, sizeof(double) * 3 * N, c->stream);
-    if (e == hipSuccess) e = launch_sigma_at_points(s);
-    if (e == hipSuccess) e = launch_points(s, 1, &mc->P, mc->P.out);
-    if (e == hipSuccess) e = launch_mst(s, &mc->P, mc->P.out, mc->dnodes, nf, mc->dfel, 0.5 * eps, mc->Efn, dF, dX, dcm);
-    if (e != hipSuccess) return dep_fail(c, DEP_ERR_HIP, "MST kernels: %s", hipGetErrorString(e));
+    DEP_CHECK(dep_mst_enqueue(s, eps, nullptr));
     std::vector<double> h(6 * nf1 + 3);
     DEP_CHECK(download(c, h.data(), mc->res, h.size()));
     memset(Fce, 0, sizeof(double) * 3 * E);
@@ -1103,7 +1142,7 @@ static void sys_release(dep_sys* s, void* p)
 }
 
 // FP32 copy of the geometric blocks (N x ldP), made on demand after each assembly
-static int ensure_p32(dep_geom* g)
+int ensure_p32(dep_geom* g)
 {
     if (g->P32 && g->p32_valid) return DEP_OK;
     DEP_CHECK(dep_geom_assemble(g));

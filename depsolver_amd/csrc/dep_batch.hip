@@ -1,0 +1,426 @@
+// dep_batch.hip -- kernels of the batched solves that share one set of
+// geometric blocks P (a frequency sweep on one geometry).
+//
+// The systems of a sweep differ only in their row weights and diagonal
+// shifts, Z_f = diag(w_f) P + diag(shift_f) with P real, so one Krylov step
+// of all of them is one real GEMM  P [x_1r x_1i ... x_Fr x_Fi]  (up to 128
+// columns) on the 16x16x4 matrix cores: P is read once per step for every
+// system instead of once per system.  The GEMM is split over K so that the
+// grid fills the chip; a second kernel adds the K partials in a fixed order
+// and applies each system's weights (the epilogue of the single-system
+// product in dep_solve.hip).  The Gram-Schmidt, normalisation and update
+// kernels below are those of dep_gmres.hip with one more grid dimension over
+// the active systems, each working on its own Krylov basis.
+//
+// A system list (dep_blist) names the systems a launch works on; system j's
+// vectors live at base + j * stride.
+#include "dep_common.h"
+#include "dep_internal.h"
+#include "dep_mfma.h"
+
+namespace {
+
+constexpr int BG_BM = 64;                  // rows per workgroup (16 per wave)
+constexpr int BG_BN = 2 * DEP_BATCH_MAX;   // columns: re and im of every system
+constexpr int BG_BK = 32;                  // K per LDS tile
+constexpr int BG_PAD = 4;
+
+// part[s][row][c] = sum over K chunk s of P[row][k] X_c[k] for c < nc = 2 L.n,
+// X_c = half (c & 1) (0 re, 1 im) of the vector X + L.id[c / 2] * xstride.
+// Staging: thread t loads element k = t % 32 of rows / columns t / 32 + 8e,
+// so each half-wave reads 32 consecutive values of one row or column.  The
+// fragment reads take 4 consecutive k per lane (one vector LDS read); MFMA e
+// of a 16-deep step then pairs slot g = lane / 16 with k = 4g + e in both
+// operands, which sums every k of the step once.
+template <class TP, class TA>
+__global__ __launch_bounds__(256) void k_bgemm(int N, const TP* __restrict__ P, long ldP, int kchunk,
+                                               dep_blist L, const double* __restrict__ X, long xstride,
+                                               TA* __restrict__ part)
+{
+    typedef typename mfma16<TA>::acc acc;
+    typedef TA v4 __attribute__((ext_vector_type(4)));
+    __shared__ __align__(16) TA As[BG_BM][BG_BK + BG_PAD];
+    __shared__ __align__(16) TA Bs[BG_BN][BG_BK + BG_PAD];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int m0 = blockIdx.x * BG_BM;
+    const int kb = blockIdx.y * kchunk, ke = min(N, kb + kchunk);
+    const int nc = 2 * L.n;
+    const int lk = t & 31, lr = t >> 5;
+    // Every load is unconditional: rows past N read row N - 1 and columns past
+    // nc read a vector of the list (L.id is zero past L.n); their products land
+    // in outputs that are never stored.  Only K past the chunk end must read as
+    // zero, which the last (partial) tile of a chunk masks.  (A guarded load
+    // compiles to a branch that waits for it and would serialise a tile's
+    // 24 loads.)
+    const double* src[16];
+#pragma unroll
+    for (int e = 0; e < 16; e++) {
+        const int c = 8 * e + lr;
+        src[e] = X + (long)L.id[c >> 1] * xstride + (long)(c & 1) * N;
+    }
+    const TP* arow[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) arow[e] = P + (long)min(m0 + 8 * e + lr, N - 1) * ldP;
+    // staged in the loaded types; converted to TA when written to LDS (a
+    // conversion right after each load would make every load wait)
+    TP ra[8];
+    double rb[16];
+    auto gload = [&](int k0) {
+        const int k = k0 + lk;
+        if (k0 + BG_BK <= ke) {
+#pragma unroll
+            for (int e = 0; e < 8; e++) ra[e] = arow[e][k];
+#pragma unroll
+            for (int e = 0; e < 16; e++) rb[e] = src[e][k];
+        } else {
+            const bool kin = k < ke;
+            const int kc = kin ? k : ke - 1;
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const TP v = arow[e][kc];
+                ra[e] = kin ? v : TP(0);
+            }
+#pragma unroll
+            for (int e = 0; e < 16; e++) {
+                const double v = src[e][kc];
+                rb[e] = kin ? v : 0.0;
+            }
+        }
+    };
+    acc C[8];
+#pragma unroll
+    for (int b = 0; b < 8; b++) C[b] = acc{0, 0, 0, 0};
+    if (kb < ke) gload(kb);
+    for (int k0 = kb; k0 < ke; k0 += BG_BK) {
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < 8; e++) As[8 * e + lr][lk] = (TA)ra[e];
+#pragma unroll
+        for (int e = 0; e < 16; e++) Bs[8 * e + lr][lk] = (TA)rb[e];
+        __syncthreads();
+        if (k0 + BG_BK < ke) gload(k0 + BG_BK);
+#pragma unroll
+        for (int kk = 0; kk < BG_BK; kk += 16) {
+            const v4 a = *(const v4*)&As[16 * w + (lane & 15)][kk + 4 * (lane >> 4)];
+#pragma unroll
+            for (int b = 0; b < 8; b++)
+                if (16 * b < nc) {
+                    const v4 bv = *(const v4*)&Bs[16 * b + (lane & 15)][kk + 4 * (lane >> 4)];
+#pragma unroll
+                    for (int e = 0; e < 4; e++) C[b] = mfma16<TA>::op(a[e], bv[e], C[b]);
+                }
+        }
+    }
+    TA* out = part + (long)blockIdx.y * N * nc;
+#pragma unroll
+    for (int b = 0; b < 8; b++)
+        if (16 * b < nc)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int row = m0 + 16 * w + mfma16<TA>::row(v, lane), col = 16 * b + (lane & 15);
+                if (row < N && col < nc) out[(long)row * nc + col] = C[b][v];
+            }
+}
+
+// y_j = diag(w_j) (P x_j) + diag(shift_j) x_j, or b - that (b != null), for
+// the systems of L: sums the S partials of columns 2a, 2a + 1 in order.
+// W holds wr, wi, shift (N each) per system, wstride apart.
+template <class TA>
+__global__ __launch_bounds__(256) void k_bgemm_epi(int N, int S, const TA* __restrict__ part, dep_blist L,
+                                                   const double* __restrict__ X, long xstride,
+                                                   const double* __restrict__ W, long wstride,
+                                                   const double* __restrict__ b, double* __restrict__ Y,
+                                                   long ystride)
+{
+    const int na = L.n, nc = 2 * na;
+    const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= (long)N * na) return;
+    const int row = (int)(idx / na), a = (int)(idx % na);
+    double zr = 0.0, zi = 0.0;
+    for (int s = 0; s < S; s++) {
+        const TA* p = part + ((long)s * N + row) * nc + 2 * a;
+        zr += (double)p[0];
+        zi += (double)p[1];
+    }
+    const long j = L.id[a];
+    const double* x = X + j * xstride;
+    const double* wj = W + j * wstride;
+    const double wr = wj[row], wi = wj[N + row], sh = wj[2 * N + row];
+    double yr = wr * zr - wi * zi + sh * x[row];
+    double yi = wr * zi + wi * zr + sh * x[N + row];
+    if (b) {
+        yr = b[row] - yr;
+        yi = b[N + row] - yi;
+    }
+    double* y = Y + j * ystride;
+    y[row] = yr;
+    y[N + row] = yi;
+}
+
+// Partial dots of system L.id[z]: part_j[v][c] = sum over chunk c of V_j[v] . V_j[wv]
+__global__ __launch_bounds__(256) void k_bdots(long n2, dep_blist L, const double* __restrict__ V, long vstride,
+                                               int wv, double* __restrict__ red, long rstride, long poff)
+{
+    __shared__ double sh[4];
+    const int c = blockIdx.x, v = blockIdx.y;
+    const long j = L.id[blockIdx.z];
+    const long len = (n2 + DEP_DOT_CHUNKS - 1) / DEP_DOT_CHUNKS;
+    const long e0 = c * len, e1 = min(n2, e0 + len);
+    const double* Vj = V + j * vstride;
+    const double* x = Vj + (long)v * n2;
+    const double* y = Vj + (long)wv * n2;
+    double s = 0.0;
+    for (long e = e0 + threadIdx.x; e < e1; e += 256) s += x[e] * y[e];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) red[j * rstride + poff + (long)v * DEP_DOT_CHUNKS + c] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+
+// One CGS pass of system L.id[blockIdx.y] (k_update_fused of dep_gmres.hip):
+// h[v] = sum of the chunk partials of v for v <= nv, V_j[wv] -= sum_{v < nv}
+// h[v] V_j[v]; block 0 stores h[0 .. nv] at red_j + hoff; with noff >= 0 each
+// block stores the sum of squares of its part of the result at red_j + noff.
+#define BUPD_MAXV 1024
+__global__ __launch_bounds__(256) void k_bupdate(int nv, long n2, dep_blist L, double* __restrict__ V, long vstride,
+                                                 int wv, double* __restrict__ red, long rstride, long poff, long hoff,
+                                                 long noff)
+{
+    __shared__ double h[BUPD_MAXV + 1];
+    __shared__ double sq4[4];
+    const long j = L.id[blockIdx.y];
+    double* rj = red + j * rstride;
+    for (int v = threadIdx.x; v <= nv; v += 256) {
+        double s = 0.0;
+        for (int c = 0; c < DEP_DOT_CHUNKS; c++) s += rj[poff + (long)v * DEP_DOT_CHUNKS + c];
+        h[v] = s;
+    }
+    __syncthreads();
+    if (blockIdx.x == 0)
+        for (int v = threadIdx.x; v <= nv; v += 256) rj[hoff + v] = h[v];
+    double* Vj = V + j * vstride;
+    double* wvec = Vj + (long)wv * n2;
+    double sq = 0.0;
+    for (long t = (long)blockIdx.x * 256 + threadIdx.x; t < n2; t += (long)gridDim.x * 256) {
+        double x = wvec[t];
+        for (int v = 0; v < nv; v++) x -= h[v] * Vj[(long)v * n2 + t];
+        wvec[t] = x;
+        sq += x * x;
+    }
+    if (noff < 0) return;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) sq += __shfl_xor(sq, o);
+    if ((threadIdx.x & 63) == 0) sq4[threadIdx.x >> 6] = sq;
+    __syncthreads();
+    if (threadIdx.x == 0) rj[noff + blockIdx.x] = (sq4[0] + sq4[1]) + (sq4[2] + sq4[3]);
+}
+
+// Block a of L: the Hessenberg column of step i of system j = L.id[a] into
+// out[a (i + 3) ..]: h1[v] + h2[v] (v <= i), |w|^2 before orthogonalisation,
+// |w|^2 after; then V_j[i + 1] /= |w| (unless zero).
+__global__ __launch_bounds__(256) void k_bcollect(int i, long n2, dep_blist L, double* __restrict__ V, long vstride,
+                                                  const double* __restrict__ red, long rstride, long h1off,
+                                                  long h2off, long noff, double* __restrict__ out)
+{
+    __shared__ double hn2;
+    const long j = L.id[blockIdx.x];
+    const double* rj = red + j * rstride;
+    double* o = out + (long)blockIdx.x * (i + 3);
+    for (int v = threadIdx.x; v <= i; v += 256) o[v] = rj[h1off + v] + rj[h2off + v];
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int k = 0; k < DEP_UPD_BLOCKS; k++) s += rj[noff + k];
+        hn2 = s;
+        o[i + 1] = rj[h1off + i + 1];
+        o[i + 2] = s;
+    }
+    __syncthreads();
+    if (!(hn2 > 0.0)) return;
+    const double sc = 1.0 / sqrt(hn2);
+    double* wv = V + j * vstride + (long)(i + 1) * n2;
+    for (long t = threadIdx.x; t < n2; t += 256) wv[t] *= sc;
+}
+
+// Block a of L: V_j[0] = D_j r_j / |D_j r_j| (both halves scaled by D_j, the
+// Jacobi factors), beta[a] = |D_j r_j|.  r_j = R + j * rstride.
+__global__ __launch_bounds__(256) void k_bstart(int N, dep_blist L, const double* __restrict__ R, long rstride,
+                                                const double* __restrict__ D, double* __restrict__ V, long vstride,
+                                                double* __restrict__ beta)
+{
+    __shared__ double sq4[4];
+    __shared__ double bsh;
+    const long j = L.id[blockIdx.x];
+    const double* r = R + j * rstride;
+    const double* d = D + j * (long)N;
+    double* v0 = V + j * vstride;
+    double sq = 0.0;
+    for (int i = threadIdx.x; i < N; i += 256) {
+        const double a = d[i] * r[i], b = d[i] * r[N + i];
+        v0[i] = a;
+        v0[N + i] = b;
+        sq += a * a + b * b;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) sq += __shfl_xor(sq, o);
+    if ((threadIdx.x & 63) == 0) sq4[threadIdx.x >> 6] = sq;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        bsh = sqrt((sq4[0] + sq4[1]) + (sq4[2] + sq4[3]));
+        beta[blockIdx.x] = bsh;
+    }
+    __syncthreads();
+    if (!(bsh > 0.0)) return;
+    const double sc = 1.0 / bsh;
+    for (int i = threadIdx.x; i < 2 * N; i += 256) v0[i] *= sc;
+}
+
+// Block a of L: out[a] = |r_j|^2
+__global__ __launch_bounds__(256) void k_bnorm2(long n2, dep_blist L, const double* __restrict__ R, long rstride,
+                                                double* __restrict__ out)
+{
+    __shared__ double sq4[4];
+    const double* r = R + (long)L.id[blockIdx.x] * rstride;
+    double sq = 0.0;
+    for (long i = threadIdx.x; i < n2; i += 256) sq += r[i] * r[i];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) sq += __shfl_xor(sq, o);
+    if ((threadIdx.x & 63) == 0) sq4[threadIdx.x >> 6] = sq;
+    __syncthreads();
+    if (threadIdx.x == 0) out[blockIdx.x] = (sq4[0] + sq4[1]) + (sq4[2] + sq4[3]);
+}
+
+// x_j += sum_{v < cnt[a]} y_a[v] V_j[v] for the systems of L (y_a = Y + a * ystride)
+__global__ __launch_bounds__(256) void k_bcombine(long n2, dep_blist L, const int* __restrict__ cnt,
+                                                  const double* __restrict__ Y, long ystride,
+                                                  const double* __restrict__ V, long vstride, double* __restrict__ X,
+                                                  long xstride)
+{
+    const int a = blockIdx.y;
+    const long j = L.id[a];
+    const long t = (long)blockIdx.x * 256 + threadIdx.x;
+    if (t >= n2) return;
+    const int n = cnt[a];
+    const double* y = Y + (long)a * ystride;
+    const double* Vj = V + j * vstride;
+    double s = X[j * xstride + t];
+    for (int v = 0; v < n; v++) s += y[v] * Vj[(long)v * n2 + t];
+    X[j * xstride + t] = s;
+}
+
+// W_j with the Jacobi factors folded in: D_j[i] = 1 / (wr_j[i] pd[i] + shift_j[i])
+// (real parts, as dep_gmres), GW_j = D_j (wr_j, wi_j, shift_j).
+__global__ void k_bjacobi(int N, int nf, const double* __restrict__ pd, const double* __restrict__ W,
+                          double* __restrict__ GW, double* __restrict__ D)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i >= N || j >= nf) return;
+    const double* w = W + (long)j * 3 * N;
+    const double d = 1.0 / (w[i] * pd[i] + w[2 * N + i]);
+    D[(long)j * N + i] = d;
+    double* g = GW + (long)j * 3 * N;
+    g[i] = d * w[i];
+    g[N + i] = d * w[N + i];
+    g[2 * N + i] = d * w[2 * N + i];
+}
+
+}  // namespace
+
+// K split of the batched product: chunks of a multiple of BG_BK columns so
+// that ceil(N / 64) x S workgroups cover the chip about four times.
+void bgemm_split(dep_ctx* c, int N, int* S, int* kchunk)
+{
+    const int rb = dep_cdiv(N, BG_BM);
+    const int want = 4 * (c->ncu > 0 ? c->ncu : 256);
+    int s = std::min(16, std::max(1, dep_cdiv(want, rb)));
+    int kc = dep_cdiv(dep_cdiv(N, s), BG_BK) * BG_BK;
+    *kchunk = kc;
+    *S = dep_cdiv(N, kc);
+}
+
+template <class TP, class TA>
+static hipError_t bgemm(dep_ctx* c, int N, const TP* P, long ldP, const dep_blist& L, const double* X, long xstride,
+                        TA* part, const double* W, long wstride, const double* b, double* Y, long ystride,
+                        const char* name)
+{
+    if (L.n < 1 || L.n > DEP_BATCH_MAX) return hipErrorInvalidValue;
+    int S, kc;
+    bgemm_split(c, N, &S, &kc);
+    hipEvent_t ev;
+    prof_begin(c, &ev);
+    hipLaunchKernelGGL((k_bgemm<TP, TA>), dim3(dep_cdiv(N, BG_BM), S), dim3(256), 0, c->stream, N, P, ldP, kc, L, X,
+                       xstride, part);
+    const double n = N, nc = 2.0 * L.n;
+    prof_end(c, ev, name, 2.0 * n * n * nc, sizeof(TP) * n * n + 8.0 * n * nc);
+    const long tot = (long)N * L.n;
+    hipLaunchKernelGGL((k_bgemm_epi<TA>), dim3((unsigned)dep_cdiv(tot, 256)), dim3(256), 0, c->stream, N, S, part,
+                       L, X, xstride, W, wstride, b, Y, ystride);
+    return hipGetLastError();
+}
+
+hipError_t launch_bmatvec32(dep_ctx* c, int N, const float* P32, long ldP, const dep_blist& L, const double* X,
+                            long xstride, void* part, const double* W, long wstride, double* Y, long ystride)
+{
+    return bgemm<float, float>(c, N, P32, ldP, L, X, xstride, (float*)part, W, wstride, nullptr, Y, ystride,
+                               "bgemm32");
+}
+
+hipError_t launch_bresidual64(dep_ctx* c, int N, const double* P, long ldP, const dep_blist& L, const double* X,
+                              long xstride, void* part, const double* W, long wstride, const double* b, double* Y,
+                              long ystride)
+{
+    return bgemm<double, double>(c, N, P, ldP, L, X, xstride, (double*)part, W, wstride, b, Y, ystride, "bgemm64");
+}
+
+size_t bgemm_part_bytes(dep_ctx* c, int N)
+{
+    int S, kc;
+    bgemm_split(c, N, &S, &kc);
+    return (size_t)S * N * BG_BN * sizeof(double);
+}
+
+hipError_t launch_bcgs_pass(dep_ctx* c, int nvec, long n2, const dep_blist& L, double* V, long vstride, int wv,
+                            double* red, long rstride, long poff, long hoff, long noff)
+{
+    if (nvec - 1 > BUPD_MAXV) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_bdots, dim3(DEP_DOT_CHUNKS, nvec, L.n), dim3(256), 0, c->stream, n2, L, V, vstride, wv, red,
+                       rstride, poff);
+    hipLaunchKernelGGL(k_bupdate, dim3(DEP_UPD_BLOCKS, L.n), dim3(256), 0, c->stream, nvec - 1, n2, L, V, vstride,
+                       wv, red, rstride, poff, hoff, noff);
+    return hipGetLastError();
+}
+
+hipError_t launch_bcollect(dep_ctx* c, int i, long n2, const dep_blist& L, double* V, long vstride, const double* red,
+                           long rstride, long h1off, long h2off, long noff, double* out)
+{
+    hipLaunchKernelGGL(k_bcollect, dim3(L.n), dim3(256), 0, c->stream, i, n2, L, V, vstride, red, rstride, h1off,
+                       h2off, noff, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_bstart(dep_ctx* c, int N, const dep_blist& L, const double* R, long rstride, const double* D,
+                         double* V, long vstride, double* beta)
+{
+    hipLaunchKernelGGL(k_bstart, dim3(L.n), dim3(256), 0, c->stream, N, L, R, rstride, D, V, vstride, beta);
+    return hipGetLastError();
+}
+
+hipError_t launch_bnorm2(dep_ctx* c, long n2, const dep_blist& L, const double* R, long rstride, double* out)
+{
+    hipLaunchKernelGGL(k_bnorm2, dim3(L.n), dim3(256), 0, c->stream, n2, L, R, rstride, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_bcombine(dep_ctx* c, long n2, const dep_blist& L, const int* cnt, const double* Y, long ystride,
+                           const double* V, long vstride, double* X, long xstride)
+{
+    hipLaunchKernelGGL(k_bcombine, dim3((unsigned)dep_cdiv(n2, 256), L.n), dim3(256), 0, c->stream, n2, L, cnt, Y,
+                       ystride, V, vstride, X, xstride);
+    return hipGetLastError();
+}
+
+hipError_t launch_bjacobi(dep_ctx* c, int N, int nf, const double* pd, const double* W, double* GW, double* D)
+{
+    hipLaunchKernelGGL(k_bjacobi, dim3(dep_cdiv(N, 256), nf), dim3(256), 0, c->stream, N, nf, pd, W, GW, D);
+    return hipGetLastError();
+}

@@ -201,3 +201,41 @@ hipError_t trsv_status(dep_sys* s, int* timed_out);
 hipError_t launch_axpy(dep_ctx* c, long n, const double* d, double* x);
 hipError_t launch_dense_zmatvec(dep_ctx* c, int N, const double* Zr, const double* Zi, long ldZ,
                                 const double* x, const double* b, double* y);
+
+// per-row weights of Z = diag(wr + i wi) P + diag(sh) at one frequency (host, internal order)
+int dep_row_weights(dep_geom* g, int nI, const double* mats, const uint32_t* ifaces, double freq, double* wr,
+                    double* wi, double* sh);
+int ensure_p32(dep_geom* g);
+// Maxwell-stress force of the solution in s->x, enqueued on the context's stream: element
+// forces (3 nf), element centres (3 nf), total (3) at dres (dep_mst_result_size doubles,
+// nf = force elements); dres == nullptr uses the geometry's own result buffer
+int dep_mst_result_size(dep_geom* g, size_t* n);
+int dep_mst_enqueue(dep_sys* s, double eps, double* dres);
+int dep_mst_center(dep_geom* g, double* Xcm);   // mean of the bc-6 nodes (force.c:493-507)
+
+// ---- batched systems sharing P (dep_batch.hip) ------------------------------
+#define DEP_BATCH_MAX 64
+// the systems a batched launch works on; system j's vectors are base + j * stride
+struct dep_blist {
+    int n = 0;
+    int id[DEP_BATCH_MAX] = {};
+};
+// Y_j = diag(GW_j) (P32 X_j) + shift: W holds wr, wi, shift (N each) per system
+hipError_t launch_bmatvec32(dep_ctx* c, int N, const float* P32, long ldP, const dep_blist& L, const double* X,
+                            long xstride, void* part, const double* W, long wstride, double* Y, long ystride);
+// Y_j = b - (diag(W_j) (P X_j) + shift) in FP64
+hipError_t launch_bresidual64(dep_ctx* c, int N, const double* P, long ldP, const dep_blist& L, const double* X,
+                              long xstride, void* part, const double* W, long wstride, const double* b, double* Y,
+                              long ystride);
+size_t bgemm_part_bytes(dep_ctx* c, int N);
+// one CGS pass on V_j[wv] against V_j[0 .. nvec-2] (red_j: partials at poff, h at hoff, norm partials at noff >= 0)
+hipError_t launch_bcgs_pass(dep_ctx* c, int nvec, long n2, const dep_blist& L, double* V, long vstride, int wv,
+                            double* red, long rstride, long poff, long hoff, long noff);
+hipError_t launch_bcollect(dep_ctx* c, int i, long n2, const dep_blist& L, double* V, long vstride, const double* red,
+                           long rstride, long h1off, long h2off, long noff, double* out);
+hipError_t launch_bstart(dep_ctx* c, int N, const dep_blist& L, const double* R, long rstride, const double* D,
+                         double* V, long vstride, double* beta);
+hipError_t launch_bnorm2(dep_ctx* c, long n2, const dep_blist& L, const double* R, long rstride, double* out);
+hipError_t launch_bcombine(dep_ctx* c, long n2, const dep_blist& L, const int* cnt, const double* Y, long ystride,
+                           const double* V, long vstride, double* X, long xstride);
+hipError_t launch_bjacobi(dep_ctx* c, int N, int nf, const double* pd, const double* W, double* GW, double* D);

@@ -24,6 +24,7 @@
 #include "dep_common.h"
 #include "dep_diag.h"
 #include "dep_internal.h"
+#include "dep_mfma.h"
 
 #define NBI DIAG_NB  // inner block (diagonal LU in LDS, inverse blocks)
 #define NBO DEP_LU_NBO   // outer panel width = trailing-update depth
@@ -326,30 +327,6 @@ __global__ __launch_bounds__(256) void k_gemm_f64mfma(int ra, int rb, int ca, in
 }
 
 // ---- complex GEMM on 16x16x4 MFMA (FP32 panels, FP64 factor) ----------------
-// Lane l supplies A[l%16][k = l/16] and B[k = l/16][l%16]; accumulator v of
-// lane l is row 4v + l/16 (FP64) or 4(l/16) + v (FP32) of the 16x16 block,
-// column l%16.
-template <class T>
-struct mfma16;
-template <>
-struct mfma16<double> {
-    typedef double acc __attribute__((ext_vector_type(4)));
-    static __device__ __forceinline__ acc op(double a, double b, acc c)
-    {
-        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-    }
-    static __device__ __forceinline__ int row(int v, int lane) { return 4 * v + (lane >> 4); }
-};
-template <>
-struct mfma16<float> {
-    typedef float acc __attribute__((ext_vector_type(4)));
-    static __device__ __forceinline__ acc op(float a, float b, acc c)
-    {
-        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-    }
-    static __device__ __forceinline__ int row(int v, int lane) { return 4 * (lane >> 4) + v; }
-};
-
 // W[ra:rb, ca:cb] -= W[ra:rb, kc:kc+nbk] W[kc:kc+nbk, ca:cb] (complex, split planes of
 // T): 64x64 complex tile per workgroup, 32x32 per wave = 2x2 blocks of 16x16 per plane,
 // Cr += Ar Br - Ai Bi and Ci += Ar Bi + Ai Br as four real MFMAs.  The next 16-deep
@@ -370,8 +347,21 @@ __device__ __forceinline__ void gemm_cmfma_tile(int bx, int by, int ra, int rb, 
     // staging: A rows t/4, k (t%4)*4 .. +3; B row k = t/16, columns (t%16)*4 .. +3
     const int a_r = t >> 2, a_k = (t & 3) * 4, b_k = t >> 4, b_c = (t & 15) * 4;
     T ga_r[4], ga_i[4], gb_r[4], gb_i[4];
+    // Tiles inside the update take unconditional loads (a guarded load compiles
+    // to a branch that waits for it); edge tiles mask per element.
+    const bool rows_in = m0 + BM <= rb, cols_in = n0 + BN <= cb;
     auto gload = [&](int kk) {
         const int gi = m0 + a_r;
+        if (rows_in && cols_in && kk + BK <= nbk) {
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                ga_r[e] = Zr[(long)gi * ldZ + kc + kk + a_k + e];
+                ga_i[e] = Zi[(long)gi * ldZ + kc + kk + a_k + e];
+                gb_r[e] = Zr[(long)(kc + kk + b_k) * ldZ + n0 + b_c + e];
+                gb_i[e] = Zi[(long)(kc + kk + b_k) * ldZ + n0 + b_c + e];
+            }
+            return;
+        }
 #pragma unroll
         for (int e = 0; e < 4; e++) {
             const bool ok = gi < rb && kk + a_k + e < nbk;
@@ -439,6 +429,30 @@ __device__ __forceinline__ void gemm_cmfma_tile(int bx, int by, int ra, int rb, 
         if (more) lstore(buf ^ 1);
         __syncthreads();
         buf ^= 1;
+    }
+    if (rows_in && cols_in) {   // all reads of the tile first, then the writes
+        T old_r[2][2][4], old_i[2][2][4];
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int b = 0; b < 2; b++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const long o = (long)(m0 + wm + 16 * a + mfma16<T>::row(v, lane)) * ldZ + n0 + wn + 16 * b + li;
+                    old_r[a][b][v] = Zr[o];
+                    old_i[a][b][v] = Zi[o];
+                }
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int b = 0; b < 2; b++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const long o = (long)(m0 + wm + 16 * a + mfma16<T>::row(v, lane)) * ldZ + n0 + wn + 16 * b + li;
+                    Zr[o] = old_r[a][b][v] - cr[a][b][v];
+                    Zi[o] = old_i[a][b][v] - ci[a][b][v];
+                }
+        return;
     }
 #pragma unroll
     for (int a = 0; a < 2; a++)
@@ -682,6 +696,25 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(int M, int NC, int NCp, int K
     const bool imag = n0 >= NCp;
     float* C = imag ? Zi : Zr;
     const int nb0 = imag ? n0 - NCp : n0;
+    if (m0 + 128 <= M && nb0 + 128 <= NC) {   // interior tile: all reads first, then the writes
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int b = 0; b < 2; b++) {
+                float old[16];
+#pragma unroll
+                for (int v = 0; v < 16; v++) {
+                    const int m = m0 + wm + 32 * a + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+                    old[v] = C[(long)(r0 + m) * ldZ + c0 + nb0 + wn + 32 * b + (lane & 31)];
+                }
+#pragma unroll
+                for (int v = 0; v < 16; v++) {
+                    const int m = m0 + wm + 32 * a + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+                    C[(long)(r0 + m) * ldZ + c0 + nb0 + wn + 32 * b + (lane & 31)] = old[v] - acc[a][b][v];
+                }
+            }
+        return;
+    }
 #pragma unroll
     for (int a = 0; a < 2; a++)
 #pragma unroll
@@ -718,14 +751,12 @@ __device__ __forceinline__ void gemm_mfma3_tile(int bx, int by, int M, int NC, i
             for (int v = 0; v < 16; v++) acc[a][b][v] = 0.0f;
     V8 ra[2], rb[2];
     const int r_lo = t >> 2, kc = (t & 3) * 8;    // chunks t and t + 256: rows r_lo and r_lo + 64
+    // rows past M read row M - 1 (their products are never stored): no guarded loads
     auto gload = [&](int k0) {
 #pragma unroll
         for (int h = 0; h < 2; h++) {
-            const int r = r_lo + 64 * h, gm = m0 + r;
-            if (gm < M) ra[h] = *(const V8*)(A + (long)gm * K2p + k0 + kc);
-            else
-#pragma unroll
-                for (int j = 0; j < 8; j++) ra[h][j] = to_h<H>(0.0f);
+            const int r = r_lo + 64 * h, gm = min(m0 + r, M - 1);
+            ra[h] = *(const V8*)(A + (long)gm * K2p + k0 + kc);
             rb[h] = *(const V8*)(BT + (long)(n0 + r) * K2p + k0 + kc);
         }
     };
@@ -768,6 +799,25 @@ __device__ __forceinline__ void gemm_mfma3_tile(int bx, int by, int M, int NC, i
     const bool imag = n0 >= NCp;
     float* C = imag ? Zi : Zr;
     const int nb0 = imag ? n0 - NCp : n0;
+    if (m0 + 128 <= M && nb0 + 128 <= NC) {   // interior tile: all reads first, then the writes
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int b = 0; b < 2; b++) {
+                float old[16];
+#pragma unroll
+                for (int v = 0; v < 16; v++) {
+                    const int m = m0 + wm + 32 * a + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+                    old[v] = C[(long)(r0 + m) * ldZ + c0 + nb0 + wn + 32 * b + (lane & 31)];
+                }
+#pragma unroll
+                for (int v = 0; v < 16; v++) {
+                    const int m = m0 + wm + 32 * a + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+                    C[(long)(r0 + m) * ldZ + c0 + nb0 + wn + 32 * b + (lane & 31)] = old[v] - acc[a][b][v];
+                }
+            }
+        return;
+    }
 #pragma unroll
     for (int a = 0; a < 2; a++)
 #pragma unroll

@@ -19,6 +19,8 @@ struct dep_points {
     int* pair_ek = nullptr;      // npairs
     double* partial = nullptr;   // nchunk x m x 6
     double* sing = nullptr;      // npairs x 6
+    double* coef = nullptr;      // npairs x (3 npe + 3): singular coefficients (geometry only)
+    mutable int coef_mode = -1;  // mode (0 potential, 1 field) the coefficients hold, -1 none
     double* out = nullptr;       // m x 6
     size_t bytes = 0;
     std::vector<std::pair<void*, size_t>> allocs;

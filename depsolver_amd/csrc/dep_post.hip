@@ -102,14 +102,22 @@ __global__ __launch_bounds__(256) void k_points_regular(
 }
 
 // one thread per coincident (point, element) pair
+// Singular part of a point set, in two steps.  The coefficients of each
+// (point, singular element) pair depend on the geometry only and are computed
+// once per point set: the NPE collocation sums of intSingularG (MODE 0) or the
+// 3 NPE of intSingularH with FLAG != 0 (MODE 1) and, for the field, the node
+// normal of the jump term (field.c:189-195).  Each evaluation then contracts
+// them with sigma.
 template <int NPE, int MODE>
-__global__ __launch_bounds__(64) void k_points_singular(int npairs, const int* __restrict__ pair_pt,
-                                  const int* __restrict__ pair_ek, const double* __restrict__ pts,
-                                  int N, const double* __restrict__ xyz, const int* __restrict__ conn,
-                                  const double* __restrict__ normal, const double* __restrict__ x,
-                                  double* __restrict__ out)
+__global__ __launch_bounds__(64) void k_points_singular_coef(int npairs, const int* __restrict__ pair_pt,
+                                                             const int* __restrict__ pair_ek,
+                                                             const double* __restrict__ pts,
+                                                             const double* __restrict__ xyz,
+                                                             const int* __restrict__ conn,
+                                                             const double* __restrict__ normal,
+                                                             double* __restrict__ coef)
 {
-    constexpr int NO = MODE ? 6 : 2;
+    constexpr int NC = 3 * NPE + 3;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= npairs) return;
     const int p = pair_pt[t], e = pair_ek[t] >> 3, k = pair_ek[t] & 7;
@@ -122,32 +130,55 @@ __global__ __launch_bounds__(64) void k_points_singular(int npairs, const int* _
         X[a][2] = xyz[3 * v + 2];
     }
     const double xe[3] = {pts[3 * p], pts[3 * p + 1], pts[3 * p + 2]};
-    double res[NO];
+    double* cf = coef + (long)t * NC;
     if constexpr (MODE == 0) {
         double sub[NPE];
         singular_G<NPE>(k + 1, X, xe, sub);
-        double sr = 0.0, si = 0.0;
 #pragma unroll
-        for (int a = 0; a < NPE; a++) {
-            const int v = conn[e * NPE + a];
-            sr += sub[a] * x[v];
-            si += sub[a] * x[N + v];
-        }
-        res[0] = sr;
-        res[1] = si;
+        for (int a = 0; a < NPE; a++) cf[a] = sub[a];
     } else {
         const int node = conn[e * NPE + k];
         const double n[3] = {normal[3 * node], normal[3 * node + 1], normal[3 * node + 2]};
         double sub[3 * NPE];
         singular_H<NPE, 1>(k + 1, X, xe, n, sub);
 #pragma unroll
+        for (int a = 0; a < 3 * NPE; a++) cf[a] = sub[a];
+#pragma unroll
+        for (int c = 0; c < 3; c++) cf[3 * NPE + c] = n[c];
+    }
+}
+
+template <int NPE, int MODE>
+__global__ __launch_bounds__(256) void k_points_singular(int npairs, const int* __restrict__ pair_ek, int N,
+                                                         const int* __restrict__ conn,
+                                                         const double* __restrict__ coef,
+                                                         const double* __restrict__ x, double* __restrict__ out)
+{
+    constexpr int NO = MODE ? 6 : 2, NC = 3 * NPE + 3;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= npairs) return;
+    const int e = pair_ek[t] >> 3;
+    const double* cf = coef + (long)t * NC;
+    double res[NO];
+    if constexpr (MODE == 0) {
+        double sr = 0.0, si = 0.0;
+#pragma unroll
+        for (int a = 0; a < NPE; a++) {
+            const int v = conn[e * NPE + a];
+            sr += cf[a] * x[v];
+            si += cf[a] * x[N + v];
+        }
+        res[0] = sr;
+        res[1] = si;
+    } else {
+#pragma unroll
         for (int c = 0; c < 3; c++) {
             double sr = 0.0, si = 0.0;
 #pragma unroll
             for (int a = 0; a < NPE; a++) {
                 const int v = conn[e * NPE + a];
-                sr += sub[3 * a + c] * x[v];
-                si += sub[3 * a + c] * x[N + v];
+                sr += cf[3 * a + c] * x[v];
+                si += cf[3 * a + c] * x[N + v];
             }
             res[2 * c] = sr;
             res[2 * c + 1] = si;
@@ -157,8 +188,8 @@ __global__ __launch_bounds__(64) void k_points_singular(int npairs, const int* _
         const int v0 = conn[e * NPE];
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            res[2 * c] += DEP_PI2 * x[v0] * n[c];
-            res[2 * c + 1] += DEP_PI2 * x[N + v0] * n[c];
+            res[2 * c] += DEP_PI2 * x[v0] * cf[3 * NPE + c];
+            res[2 * c + 1] += DEP_PI2 * x[N + v0] * cf[3 * NPE + c];
         }
     }
 #pragma unroll
@@ -406,11 +437,23 @@ hipError_t launch_points(dep_sys* s, int mode, const dep_points* P, double* out)
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (P->npairs > 0) {
-        dim3 gs(dep_cdiv(P->npairs, 64));
+        if (P->coef_mode != mode) {
+            dim3 gs(dep_cdiv(P->npairs, 64));
+#define COEF(NPE_, MODE_)                                                                          \
+    hipLaunchKernelGGL((k_points_singular_coef<NPE_, MODE_>), gs, dim3(64), 0, st, P->npairs,     \
+                       P->pair_pt, P->pair_ek, P->pts, g->xyz, g->conn, g->normal, P->coef)
+            if (g->npe == 3) {
+                if (mode == 0) COEF(3, 0); else COEF(3, 1);
+            } else {
+                if (mode == 0) COEF(6, 0); else COEF(6, 1);
+            }
+#undef COEF
+            P->coef_mode = mode;
+        }
+        dim3 gs(dep_cdiv(P->npairs, 256));
 #define SING(NPE_, MODE_)                                                                         \
-    hipLaunchKernelGGL((k_points_singular<NPE_, MODE_>), gs, dim3(64), 0, st, P->npairs,         \
-                       P->pair_pt, P->pair_ek, P->pts, g->N, g->xyz, g->conn, g->normal, s->x, \
-                       P->sing)
+    hipLaunchKernelGGL((k_points_singular<NPE_, MODE_>), gs, dim3(256), 0, st, P->npairs, P->pair_ek, \
+                       g->N, g->conn, P->coef, s->x, P->sing)
         if (g->npe == 3) {
             if (mode == 0) SING(3, 0); else SING(3, 1);
         } else {

@@ -1,0 +1,387 @@
+// dep_sweep.cpp -- frequency sweeps on one geometry (refined FP32 GMRES for
+// up to DEP_BATCH_MAX frequencies at once).
+//
+// The reference solves each frequency as a separate run of depSolver
+// (src/assembly_el.c re-integrates the whole matrix for every vProbParam[0];
+// the frequency enters only through the interface weights A_k, B_k of
+// :496-512).  Here the geometric blocks P are integrated once and every
+// frequency f is the system Z_f = diag(w_f) P + diag(shift_f).  All systems
+// run the refined FP32 GMRES of dep_gmres_ir in lockstep: one batched FP64
+// residual product per correction, and per Krylov step one batched FP32
+// product on the matrix cores plus batched CGS2 kernels, with one transfer of
+// all the systems' Hessenberg columns to the host, where each system's Givens
+// recurrence runs as in src/gmres_el.c:196-241.  A system leaves the batch as
+// soon as it converges.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "dep_common.h"
+#include "dep_internal.h"
+
+struct dep_sweep {
+    dep_geom* g = nullptr;
+    dep_sys* sys = nullptr;       // the first frequency's system: RHS, post-processing
+    int nf = 0, cap = 0;
+    std::vector<double> freqs;
+    double bnorm = 0.0;
+    double* W = nullptr;          // nf x 3N: wr, wi, shift
+    double* GW = nullptr;         // nf x 3N: the same scaled by the Jacobi factors
+    double* D = nullptr;          // nf x N Jacobi factors
+    double* X = nullptr;          // nf x 2N solutions
+    double* R = nullptr;          // nf x 2N FP64 residuals
+    double* V = nullptr;          // nf x (cap + 1) x 2N Krylov bases
+    double* red = nullptr;        // nf x rstride reduction scratch
+    void* part = nullptr;         // split-K partials of the batched products
+    double* cbuf = nullptr;       // DEP_BATCH_MAX x (cap + 3) Hessenberg columns
+    double* Yd = nullptr;         // DEP_BATCH_MAX x (cap + 1) combination coefficients
+    int* cntd = nullptr;          // DEP_BATCH_MAX coefficient counts
+    double* scal = nullptr;       // DEP_BATCH_MAX per-system scalars
+    double* mres = nullptr;       // nf MST result slots
+    size_t mres_n = 0;
+    double* hpin = nullptr;       // pinned host staging
+    size_t hpin_count = 0;
+    std::vector<int> iters, outer;   // per system, last solve
+    std::vector<double> res;
+    double solve_ms = 0.0;
+    std::vector<std::pair<void*, size_t>> allocs;
+    dep_ctx* ctx_of() { return g->ctx; }
+};
+
+namespace {
+
+long rstride_of(int cap) { return 2L * (cap + 2) + DEP_UPD_BLOCKS + (long)(cap + 2) * DEP_DOT_CHUNKS; }
+
+void release(dep_sweep* w, void* p)
+{
+    for (auto it = w->allocs.begin(); it != w->allocs.end(); ++it)
+        if (it->first == p) {
+            dep_free(w->g->ctx, p, it->second);
+            w->allocs.erase(it);
+            return;
+        }
+}
+
+// Krylov storage for max_inner steps
+int ensure_storage(dep_sweep* w, int max_inner)
+{
+    if (max_inner > 1000) return dep_fail(w->g->ctx, DEP_ERR_ARG, "sweep: at most 1000 GMRES steps per correction");
+    if (w->V && w->cap >= max_inner) return DEP_OK;
+    dep_ctx* c = w->g->ctx;
+    DEP_HIP(c, hipStreamSynchronize(c->stream));
+    for (void* p : {(void*)w->V, (void*)w->red, (void*)w->cbuf, (void*)w->Yd})
+        if (p) release(w, p);
+    if (w->hpin) (void)hipHostFree(w->hpin);
+    w->hpin = nullptr;
+    const long n2 = 2L * w->g->N;
+    DEP_CHECK(dep_obj_alloc(w, &w->V, (size_t)w->nf * (max_inner + 1) * n2));
+    DEP_CHECK(dep_obj_alloc(w, &w->red, (size_t)w->nf * rstride_of(max_inner)));
+    DEP_CHECK(dep_obj_alloc(w, &w->cbuf, (size_t)DEP_BATCH_MAX * (max_inner + 3)));
+    DEP_CHECK(dep_obj_alloc(w, &w->Yd, (size_t)DEP_BATCH_MAX * (max_inner + 1)));
+    w->hpin_count = (size_t)DEP_BATCH_MAX * (max_inner + 3) + (size_t)DEP_BATCH_MAX * (max_inner + 1) + DEP_BATCH_MAX;
+    if (hipHostMalloc((void**)&w->hpin, sizeof(double) * w->hpin_count, 0) != hipSuccess) {
+        w->hpin = nullptr;
+        return dep_fail(c, DEP_ERR_NOMEM, "sweep: pinned host staging of %zu bytes", sizeof(double) * w->hpin_count);
+    }
+    w->cap = max_inner;
+    return DEP_OK;
+}
+
+template <class T>
+int down(dep_ctx* c, T* h, const T* d, size_t n)
+{
+    DEP_HIP(c, hipMemcpyAsync(h, d, sizeof(T) * n, hipMemcpyDeviceToHost, c->stream));
+    DEP_HIP(c, hipStreamSynchronize(c->stream));
+    return DEP_OK;
+}
+
+// Per-system state of one GMRES run (the recurrence of gmres_run in dep_api.cpp)
+struct Run {
+    std::vector<double> H, cs, sn, g;
+    double beta = 0.0, tol = 0.0, err0 = 0.0, err1 = 0.0;
+    int iters = 0, rc = 1;   // rc: 0 converged, 1 iteration limit, 2 stagnation
+};
+
+}  // namespace
+
+extern "C" int dep_sweep_create(dep_geom* g, int nM, const double* mats, int nI, const uint32_t* ifaces, int nf,
+                                const double* freqs, const double* rhs, dep_sweep** out)
+{
+    if (!g || !out || !freqs || nf < 1) return dep_fail(g ? g->ctx : nullptr, DEP_ERR_ARG, "dep_sweep_create: bad arguments");
+    dep_ctx* c = g->ctx;
+    if (nf > DEP_BATCH_MAX) return dep_fail(c, DEP_ERR_ARG, "dep_sweep_create: at most %d frequencies", DEP_BATCH_MAX);
+    for (int f = 0; f < nf; f++)
+        if (!(freqs[f] > 0.0) || !std::isfinite(freqs[f]))
+            return dep_fail(c, DEP_ERR_ARG, "dep_sweep_create: frequency %d is %g", f, freqs[f]);
+    dep_sweep* w = new dep_sweep();
+    w->g = g;
+    w->nf = nf;
+    w->freqs.assign(freqs, freqs + nf);
+    auto fail = [&](int r) {
+        dep_sweep_destroy(w);
+        return r;
+    };
+    int r = dep_sys_create(g, nM, mats, nI, ifaces, freqs[0], rhs, &w->sys);
+    if (r != DEP_OK) {
+        w->sys = nullptr;
+        return fail(r);
+    }
+    const int N = g->N;
+    const long n2 = 2L * N;
+    if ((r = ensure_p32(g)) != DEP_OK) return fail(r);
+    if ((r = dep_obj_alloc(w, &w->W, (size_t)nf * 3 * N)) != DEP_OK) return fail(r);
+    if ((r = dep_obj_alloc(w, &w->GW, (size_t)nf * 3 * N)) != DEP_OK) return fail(r);
+    if ((r = dep_obj_alloc(w, &w->D, (size_t)nf * N)) != DEP_OK) return fail(r);
+    if ((r = dep_obj_alloc(w, &w->X, (size_t)nf * n2)) != DEP_OK) return fail(r);
+    if ((r = dep_obj_alloc(w, &w->R, (size_t)nf * n2)) != DEP_OK) return fail(r);
+    if ((r = dep_obj_alloc(w, &w->cntd, (size_t)DEP_BATCH_MAX)) != DEP_OK) return fail(r);
+    if ((r = dep_obj_alloc(w, &w->scal, (size_t)DEP_BATCH_MAX)) != DEP_OK) return fail(r);
+    if ((r = dep_obj_alloc(w, (char**)&w->part, bgemm_part_bytes(c, N))) != DEP_OK) return fail(r);
+    std::vector<double> hw((size_t)nf * 3 * N);
+    for (int f = 0; f < nf; f++) {
+        double* p = hw.data() + (size_t)f * 3 * N;
+        if ((r = dep_row_weights(g, nI, mats, ifaces, freqs[f], p, p + N, p + 2 * N)) != DEP_OK) return fail(r);
+    }
+    if (hipMemcpy(w->W, hw.data(), sizeof(double) * hw.size(), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(dep_fail(c, DEP_ERR_HIP, "sweep: weight upload"));
+    // Jacobi factors from the diagonal of P (as dep_gmres)
+    double* pd = w->R;   // scratch until the first solve
+    if (launch_diag_gather(c, N, g->P, g->ldP, pd) != hipSuccess || launch_bjacobi(c, N, nf, pd, w->W, w->GW, w->D) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+        return fail(dep_fail(c, DEP_ERR_HIP, "sweep: Jacobi factors"));
+    std::vector<double> hb(n2);
+    for (int i = 0; i < N; i++) {
+        hb[i] = rhs[i];
+        hb[N + i] = rhs[N + i];
+    }
+    double bn = 0.0;
+    for (double v : hb) bn += v * v;
+    w->bnorm = bn > 0.0 ? sqrt(bn) : 1.0;
+    *out = w;
+    return DEP_OK;
+}
+
+extern "C" int dep_sweep_destroy(dep_sweep* w)
+{
+    if (!w) return DEP_OK;
+    dep_ctx* c = w->g->ctx;
+    (void)hipStreamSynchronize(c->stream);
+    if (w->hpin) (void)hipHostFree(w->hpin);
+    for (auto& a : w->allocs) dep_free(c, a.first, a.second);
+    if (w->sys) dep_sys_destroy(w->sys);
+    delete w;
+    return DEP_OK;
+}
+
+extern "C" int dep_sweep_solve(dep_sweep* w, double tol, double inner_tol, int max_inner, int max_outer,
+                               double* sigma, dep_solve_info* info)
+{
+    if (!w || max_inner <= 0 || max_outer <= 0 || !(tol > 0.0) || !(inner_tol > 0.0)) return DEP_ERR_ARG;
+    dep_geom* g = w->g;
+    dep_ctx* c = g->ctx;
+    DEP_CHECK(ensure_storage(w, max_inner));
+    DEP_CHECK(ensure_p32(g));
+    const int N = g->N, nf = w->nf, M = max_inner;
+    const long n2 = 2L * N, vstride = (long)(w->cap + 1) * n2, rstride = rstride_of(w->cap);
+    const long h1off = 0, h2off = w->cap + 2, noff = 2L * (w->cap + 2), poff = noff + DEP_UPD_BLOCKS;
+    double* hdown = w->hpin;
+    double* hY = hdown + (size_t)DEP_BATCH_MAX * (w->cap + 3);
+    int* hcnt = (int*)(hY + (size_t)DEP_BATCH_MAX * (w->cap + 1));
+    const double* b = w->sys->b;
+
+    DEP_HIP(c, hipEventRecord(c->ev0, c->stream));
+    DEP_HIP(c, hipMemsetAsync(w->X, 0, sizeof(double) * nf * n2, c->stream));
+    w->iters.assign(nf, 0);
+    w->outer.assign(nf, 0);
+    w->res.assign(nf, 1.0);
+    std::vector<char> done(nf, 0), stalled(nf, 0);
+    bool first = true;
+    std::vector<Run> run(nf);
+    for (;;) {
+        dep_blist L;
+        for (int j = 0; j < nf; j++)
+            if (!done[j]) L.id[L.n++] = j;
+        if (L.n == 0) break;
+        if (!first) {   // FP64 residuals of the systems still open
+            DEP_HIP(c, launch_bresidual64(c, N, g->P, g->ldP, L, w->X, n2, w->part, w->W, 3L * N, b, w->R, n2));
+            DEP_HIP(c, launch_bnorm2(c, n2, L, w->R, n2, w->scal));
+            DEP_CHECK(down(c, hdown, w->scal, L.n));
+            for (int a = 0; a < L.n; a++) w->res[L.id[a]] = sqrt(hdown[a]) / w->bnorm;
+        }
+        dep_blist A;
+        for (int a = 0; a < L.n; a++) {
+            const int j = L.id[a];
+            if (!(w->res[j] > tol) || w->outer[j] >= max_outer || stalled[j] || !std::isfinite(w->res[j]))
+                done[j] = 1;
+            else
+                A.id[A.n++] = j;
+        }
+        if (A.n == 0) break;
+        // inner right-hand sides D_j r_j, normalised into V_j[0]
+        DEP_HIP(c, launch_bstart(c, N, A, first ? b : w->R, first ? 0 : n2, w->D, w->V, vstride, w->scal));
+        DEP_CHECK(down(c, hdown, w->scal, A.n));
+        dep_blist I;
+        std::vector<int> slot(nf, -1);   // index of a system in A
+        for (int a = 0; a < A.n; a++) {
+            const int j = A.id[a];
+            slot[j] = a;
+            Run& q = run[j];
+            q.H.assign((size_t)(M + 1) * (M + 1), 0.0);
+            q.cs.assign(M + 1, 0.0);
+            q.sn.assign(M + 1, 0.0);
+            q.g.assign(M + 2, 0.0);
+            q.beta = hdown[a];
+            q.g[0] = q.beta;
+            q.tol = inner_tol * q.beta;
+            q.err0 = q.err1 = q.beta;
+            q.iters = 0;
+            q.rc = q.beta < q.tol || !(q.beta > 0.0) ? 0 : 1;
+            if (q.rc == 1) I.id[I.n++] = j;
+        }
+        for (int i = 0; I.n > 0 && i < M; i++) {
+            DEP_HIP(c, launch_bmatvec32(c, N, g->P32, g->ldP, I, w->V + (long)i * n2, vstride, w->part, w->GW, 3L * N,
+                                        w->V + (long)(i + 1) * n2, vstride));
+            DEP_HIP(c, launch_bcgs_pass(c, i + 2, n2, I, w->V, vstride, i + 1, w->red, rstride, poff, h1off, -1));
+            DEP_HIP(c, launch_bcgs_pass(c, i + 2, n2, I, w->V, vstride, i + 1, w->red, rstride, poff, h2off, noff));
+            DEP_HIP(c, launch_bcollect(c, i, n2, I, w->V, vstride, w->red, rstride, h1off, h2off, noff, w->cbuf));
+            DEP_CHECK(down(c, hdown, w->cbuf, (size_t)I.n * (i + 3)));
+            dep_blist next;
+            for (int a = 0; a < I.n; a++) {
+                const int j = I.id[a];
+                Run& q = run[j];
+                const double* hc = hdown + (size_t)a * (i + 3);
+#define HH(r, col) q.H[(size_t)(r) * (M + 1) + (col)]
+                q.iters++;
+                const double wn = sqrt(hc[i + 1]);
+                for (int k = 0; k <= i; k++) HH(k, i) = hc[k];
+                HH(i + 1, i) = sqrt(hc[i + 2]);
+                for (int k = 0; k <= i - 1; k++) {
+                    const double g1 = q.cs[k] * HH(k, i) + q.sn[k] * HH(k + 1, i);
+                    const double g2 = -q.sn[k] * HH(k, i) + q.cs[k] * HH(k + 1, i);
+                    HH(k, i) = g1;
+                    HH(k + 1, i) = g2;
+                }
+                const double htmp = HH(i + 1, i) / HH(i, i);
+                if (wn != 0.0) {
+                    q.cs[i] = 1.0 / sqrt(1.0 + htmp * htmp);
+                    q.sn[i] = q.cs[i] * htmp;
+                    HH(i, i) = q.cs[i] * HH(i, i) + q.sn[i] * HH(i + 1, i);
+                    HH(i + 1, i) = 0.0;
+                    q.g[i + 1] = -q.sn[i] * q.g[i];
+                    q.g[i] = q.cs[i] * q.g[i];
+                }
+                q.err1 = fabs(q.g[i + 1]);
+                if (q.err1 < q.tol) q.rc = 0;
+                else if (fabs((q.err1 - q.err0) / q.err0) < 1.0e-6 * q.tol) q.rc = 2;
+                else q.err0 = q.err1;
+#undef HH
+                if (q.rc == 1) next.id[next.n++] = j;
+            }
+            I = next;
+        }
+        // x_j += V_j y_j with y_j from the triangular Hessenberg system
+        for (int a = 0; a < A.n; a++) {
+            const int j = A.id[a];
+            Run& q = run[j];
+            const int n = (q.rc != 2) ? q.iters : 0;
+            double* y = hY + (size_t)a * (w->cap + 1);
+            for (int k = 0; k < n; k++) y[k] = q.g[k];
+            for (int k = n - 1; k >= 0; k--) {
+                for (int l = k + 1; l < n; l++) y[k] -= q.H[(size_t)k * (M + 1) + l] * y[l];
+                y[k] /= q.H[(size_t)k * (M + 1) + k];
+            }
+            hcnt[a] = n;
+            w->iters[j] += q.iters;
+            w->outer[j]++;
+            if (q.iters == 0) stalled[j] = 1;
+        }
+        DEP_HIP(c, hipMemcpyAsync(w->Yd, hY, sizeof(double) * (size_t)A.n * (w->cap + 1), hipMemcpyHostToDevice,
+                                  c->stream));
+        DEP_HIP(c, hipMemcpyAsync(w->cntd, hcnt, sizeof(int) * A.n, hipMemcpyHostToDevice, c->stream));
+        DEP_HIP(c, launch_bcombine(c, n2, A, w->cntd, w->Yd, w->cap + 1, w->V, vstride, w->X, n2));
+        DEP_HIP(c, hipStreamSynchronize(c->stream));   // the staging is rewritten by the next pass
+        first = false;
+    }
+    DEP_HIP(c, hipEventRecord(c->ev1, c->stream));
+    DEP_HIP(c, hipEventSynchronize(c->ev1));
+    float ms = 0.0f;
+    DEP_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    w->solve_ms = ms;
+    int worst = DEP_OK;
+    for (int j = 0; j < nf; j++) {
+        if (info) {
+            dep_solve_info& in = info[j];
+            memset(&in, 0, sizeof(in));
+            in.iterations = w->outer[j];
+            in.inner_iterations = w->iters[j];
+            in.residual = w->res[j];
+            in.converged = !(w->res[j] > tol);
+            in.solve_ms = ms;
+        }
+        if (!std::isfinite(w->res[j])) worst = DEP_ERR_SINGULAR;
+        else if (w->res[j] > tol && worst == DEP_OK) worst = DEP_ERR_NOCONV;
+    }
+    if (sigma)
+        for (int j = 0; j < nf; j++) DEP_CHECK(dep_sweep_get_solution(w, j, sigma + (size_t)j * n2));
+    if (worst == DEP_ERR_SINGULAR) return dep_fail(c, worst, "dep_sweep_solve: non-finite residual");
+    if (worst == DEP_ERR_NOCONV) return dep_fail(c, worst, "dep_sweep_solve: a frequency did not reach %.3e", tol);
+    return DEP_OK;
+}
+
+extern "C" int dep_sweep_get_solution(dep_sweep* w, int f, double* sigma)
+{
+    if (!w || !sigma || f < 0 || f >= w->nf) return DEP_ERR_ARG;
+    dep_geom* g = w->g;
+    const int N = g->N;
+    std::vector<double> hx(2 * (size_t)N);
+    DEP_CHECK(down(g->ctx, hx.data(), w->X + (size_t)f * 2 * N, hx.size()));
+    for (int i = 0; i < N; i++) {
+        sigma[g->perm[i]] = hx[i];
+        sigma[N + g->perm[i]] = hx[N + i];
+    }
+    return DEP_OK;
+}
+
+extern "C" int dep_sweep_system(dep_sweep* w, int f, dep_sys** sys)
+{
+    if (!w || !sys || f < 0 || f >= w->nf) return DEP_ERR_ARG;
+    dep_ctx* c = w->g->ctx;
+    DEP_CHECK(dep_sys_set_frequency(w->sys, w->freqs[f]));
+    DEP_HIP(c, hipMemcpyAsync(w->sys->x, w->X + (size_t)f * 2 * w->g->N, sizeof(double) * 2 * w->g->N,
+                              hipMemcpyDeviceToDevice, c->stream));
+    *sys = w->sys;
+    return DEP_OK;
+}
+
+// Per frequency: sigma_f into the sweep's system and the MST kernels into that
+// frequency's slot of one device buffer, all enqueued; one download at the end.
+extern "C" int dep_sweep_mst(dep_sweep* w, double eps_medium, double* Fcm, double* Xcm)
+{
+    if (!w || !Fcm || !Xcm) return DEP_ERR_ARG;
+    dep_geom* g = w->g;
+    dep_ctx* c = g->ctx;
+    size_t nres = 0;
+    DEP_CHECK(dep_mst_result_size(g, &nres));
+    if (!w->mres || w->mres_n < nres * w->nf) {
+        if (w->mres) {
+            DEP_HIP(c, hipStreamSynchronize(c->stream));
+            release(w, w->mres);
+            w->mres = nullptr;
+        }
+        DEP_CHECK(dep_obj_alloc(w, &w->mres, nres * w->nf));
+        w->mres_n = nres * w->nf;
+    }
+    for (int f = 0; f < w->nf; f++) {
+        DEP_HIP(c, hipMemcpyAsync(w->sys->x, w->X + (size_t)f * 2 * g->N, sizeof(double) * 2 * g->N,
+                                  hipMemcpyDeviceToDevice, c->stream));
+        DEP_CHECK(dep_mst_enqueue(w->sys, eps_medium, w->mres + (size_t)f * nres));
+    }
+    std::vector<double> h(nres * w->nf);
+    DEP_CHECK(down(c, h.data(), w->mres, h.size()));
+    for (int f = 0; f < w->nf; f++)
+        for (int k = 0; k < 3; k++) Fcm[3 * (size_t)f + k] = h[(size_t)f * nres + nres - 3 + k];
+    return dep_mst_center(g, Xcm);
+}

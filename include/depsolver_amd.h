@@ -54,6 +54,7 @@ extern "C" {
 typedef struct dep_ctx dep_ctx;
 typedef struct dep_geom dep_geom;
 typedef struct dep_sys dep_sys;
+typedef struct dep_sweep dep_sweep;
 
 typedef struct dep_solve_info {
     int iterations;        /* GMRES iterations, or refinement steps */
@@ -148,6 +149,27 @@ int dep_post_mst(dep_sys *s, double eps_medium, double *Fce, double *Xce, double
                  double *Xcm);
 int dep_post_multipole(dep_sys *s, int order, double radius, int m, const double *pts,
                        double *F /* m x 3 */);
+
+/* ---- frequency sweeps on one geometry ----------------------------------
+ * No single counterpart in the reference, which runs depSolver once per
+ * frequency (vProbParam[0], read at src/depSolver.c:126) and re-assembles
+ * everything each time (src/assembly_el.c:470-589).  Here the geometric
+ * blocks are integrated once and the nf <= 64 systems Z_f = diag(w_f) P +
+ * diag(shift_f) (weights of src/assembly_el.c:496-512 at freqs[f]) are solved
+ * together by refined FP32 GMRES (as dep_gmres_ir) with batched matrix-core
+ * products.  mats/ifaces/rhs as dep_sys_create; all frequencies share rhs. */
+int dep_sweep_create(dep_geom *g, int n_mats, const double *mats, int n_ifaces, const uint32_t *ifaces,
+                     int nf, const double *freqs, const double *rhs, dep_sweep **out);
+int dep_sweep_destroy(dep_sweep *w);
+/* sigma: nf x 2N (may be NULL); info: nf entries (may be NULL) */
+int dep_sweep_solve(dep_sweep *w, double tol, double inner_tol, int max_inner, int max_outer,
+                    double *sigma, dep_solve_info *info);
+int dep_sweep_get_solution(dep_sweep *w, int f, double *sigma);
+/* the sweep's system set to frequency f with solution sigma_f, for the
+ * dep_post_* calls (valid until the next dep_sweep_system call) */
+int dep_sweep_system(dep_sweep *w, int f, dep_sys **sys);
+/* total Maxwell-stress force per frequency (Fcm: nf x 3) and the force centre Xcm (3) */
+int dep_sweep_mst(dep_sweep *w, double eps_medium, double *Fcm, double *Xcm);
 
 /* ---- reference-signature shims (src/depSolver.c:284-344) ----------------- */
 int electricFormA_tria3(double **mNodes, unsigned int **mElems, unsigned int *vBCType,

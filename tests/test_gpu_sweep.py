@@ -1,0 +1,105 @@
+"""GPU: frequency sweeps on one geometry (dep_sweep: the systems of up to 64
+frequencies solved together by refined FP32 GMRES with batched matrix-core
+products over the shared geometric blocks).
+
+* tiny meshes: every frequency's sigma matches the oracle's FP64 LU on the
+  matrix the reference assembles at that frequency (src/assembly_el.c with
+  vProbParam[0] = f) to 1e-10 relative;
+* the sweep at the config-1 and config-3 sizes matches the GPU FP64 LU
+  (itself pinned to the reference) per frequency to 1e-10 (sigma) and 1e-8
+  (Maxwell-stress element forces);
+* batching does not change a result: a frequency solved alone and inside a
+  batch gives the same bits (each system's columns of the batched product,
+  its reductions and its Givens recurrence are independent of the others).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from depsolver_amd import mesh
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def load_case(name):
+    z = np.load(os.path.join(GOLD, f"{name}.npz"))
+    return mesh.Problem(z["xyz"], z["conn"].astype(np.int64), z["bctype"], z["bc"], z["mats"],
+                        z["ifaces"], float(z["freq"]), None, name)
+
+
+def at_freq(p, f):
+    return mesh.Problem(p.xyz, p.conn, p.bctype, p.bc, p.mats, p.ifaces, float(f), p.body, p.name)
+
+
+def rel(a, b):
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+@pytest.mark.parametrize("name", ["tiny_tria6", "tiny_tria3"])
+def test_sweep_matches_oracle_lu_per_frequency(ctx, orc, name):
+    import depsolver_amd as dep
+    print(f"[gpu] sweep {name}", flush=True)
+    p = load_case(name)
+    freqs = mesh.sweep_frequencies(6)
+    g = dep.Geometry(ctx, p)
+    w = dep.Sweep(g, freqs)
+    try:
+        sig, info = w.solve(tol=1e-13)
+        for k, f in enumerate(freqs):
+            assert info[k]["converged"] == 1 and info[k]["residual"] <= 1e-13, (k, info[k])
+            A, b = orc.assemble(at_freq(p, f), True)
+            want = orc.gauss_bksb(A, b)
+            assert rel(sig[k], want) < 1e-10, (k, f, rel(sig[k], want))
+    finally:
+        w.close()
+        g.close()
+
+
+@pytest.mark.parametrize("cfg,nf,check", [("config1", 8, (0, 3, 7)), ("config3", 64, (0, 21, 42, 63))])
+def test_sweep_matches_fp64_lu(ctx, cfg, nf, check):
+    import depsolver_amd as dep
+    print(f"[gpu] sweep {cfg} x{nf}", flush=True)
+    p = mesh.config1() if cfg == "config1" else mesh.config3_geometry()
+    freqs = mesh.sweep_frequencies(nf)
+    g = dep.Geometry(ctx, p)
+    w = dep.Sweep(g, freqs)
+    s = dep.System(g)
+    try:
+        sig, info = w.solve(tol=1e-12)
+        assert all(x["converged"] == 1 for x in info), [x["residual"] for x in info]
+        F, Xcm = w.mst()
+        for k in check:
+            s.set_frequency(freqs[k])
+            s.factor("fp64")
+            want, _ = s.solve(tol=1e-13, max_steps=3)
+            assert rel(sig[k], want) < 1e-10, (k, rel(sig[k], want))
+            Fce, Xce, Fcm, Xc = s.mst()
+            # the sweep's total force from its own sigma; the LU's element forces set the scale
+            assert np.abs(F[k] - Fcm).max() <= 1e-8 * np.abs(Fce).sum(), (k, F[k], Fcm)
+            assert np.abs(Xcm - Xc).max() == 0.0
+    finally:
+        s.close()
+        w.close()
+        g.close()
+
+
+def test_sweep_result_does_not_depend_on_the_batch(ctx):
+    import depsolver_amd as dep
+    p = mesh.config1()
+    freqs = mesh.sweep_frequencies(8)
+    g = dep.Geometry(ctx, p)
+    w8 = dep.Sweep(g, freqs)
+    w1 = dep.Sweep(g, freqs[5:6])
+    try:
+        s8, i8 = w8.solve(tol=1e-12)
+        s1, i1 = w1.solve(tol=1e-12)
+        assert i8[5]["inner_iterations"] == i1[0]["inner_iterations"]
+        assert np.array_equal(s8[5], s1[0])
+        again, _ = w8.solve(tol=1e-12)
+        assert np.array_equal(again, s8)
+    finally:
+        w1.close()
+        w8.close()
+        g.close()
