@@ -210,6 +210,70 @@ def sweep_bench(dep, ctx, steps, nf=64):
             "roofline": roofline(stats)}
 
 
+def serve_bench(dep, n_req, slots, solvers=("gmres-ir-fp32", "lu-fp16")):
+    """Serving stream (Poisson arrivals at 200 req/s of 1-3 shelled particles in the quadrupole trap,
+    N 5.6k-30k, f 10 kHz-50 MHz): each request is assembled, solved to an FP64 residual of 1e-12 and
+    its MST force computed; `slots` requests run concurrently.  Latency = arrival to completion."""
+    from depsolver_amd import mesh
+    st = mesh.config5_stream(n_req)
+
+    def run(nslots, solver):
+        sv = dep.Server(0, slots=nslots)
+        sv.wait(sv.submit(st[0][1], solver, want_sigma=False))   # warm the slot's arena and kernels
+        t0 = time.perf_counter()
+        tickets = []
+        for ta, p in st:
+            now = time.perf_counter() - t0
+            if ta > now:
+                time.sleep(ta - now)
+            tickets.append(sv.submit(p, solver, want_sigma=False))
+        res = [sv.wait(t)[1] for t in tickets]
+        span = time.perf_counter() - t0 - st[0][0]
+        stats = sv.stats()
+        sv.close()
+        lat = np.array([r["queue_ms"] + r["service_ms"] for r in res])
+        svc = np.array([r["service_ms"] for r in res])
+        return {"slots": nslots, "solves_per_sec": n_req / span, "p50_ms": float(np.percentile(lat, 50)),
+                "p99_ms": float(np.percentile(lat, 99)), "service_p50_ms": float(np.percentile(svc, 50)),
+                "max_residual": max(r["info"]["residual"] for r in res),
+                "peak_reserved_gb": stats["peak_reserved"] / 1e9, "budget_gb": stats["budget"] / 1e9}
+
+    out = {"workload": f"config5: {n_req} requests, Poisson 200 req/s, 1-3 shelled particles in the quadrupole "
+                       f"trap, N {min(p.n_nodes for _, p in st)}-{max(p.n_nodes for _, p in st)}, "
+                       "f log-uniform 10 kHz-50 MHz, assemble + solve (FP64 residual 1e-12) + MST per request",
+           "offered_rate": n_req / st[-1][0], "solvers": {}}
+    for sol in solvers:
+        r = {"concurrent": run(slots, sol), "one_slot": run(1, sol)}
+        log(f"bench: serving {sol}: {slots} slots {r['concurrent']}; 1 slot {r['one_slot']}")
+        out["solvers"][sol] = r
+    best = max(out["solvers"], key=lambda k: out["solvers"][k]["concurrent"]["solves_per_sec"])
+    out["best"] = dict(out["solvers"][best]["concurrent"], solver=best)
+    return out
+
+
+def batch_bench(dep, n_cases, slots, solver="lu-bf16"):
+    """Uneven particle-position batch: n_cases spheres (radius 0.5-1.5, mesh level 2-4, N 5.0k-14.6k)
+    at seeded positions in the quadrupole trap, all submitted at once to the serving loop; each is
+    assembled, factored (BF16 MFMA LU, FP64 refinement to 1e-12) and its MST force computed."""
+    from depsolver_amd import mesh
+    ps = mesh.config4_problems(n_cases)
+    sv = dep.Server(0, slots=slots)
+    sv.wait(sv.submit(ps[0], solver, want_sigma=False))
+    t0 = time.perf_counter()
+    tickets = [sv.submit(p, solver, want_sigma=False) for p in ps]
+    res = [sv.wait(t)[1] for t in tickets]
+    span = time.perf_counter() - t0
+    sv.close()
+    svc = np.array([r["service_ms"] for r in res])
+    return {"workload": f"config4: {n_cases} sphere positions in the quadrupole trap, N "
+                        f"{min(p.n_nodes for p in ps)}-{max(p.n_nodes for p in ps)}, all at once, "
+                        "assemble + solve (FP64 residual 1e-12) + MST each",
+            "solver": solver, "slots": slots, "solves_per_sec": n_cases / span, "seconds": span,
+            "service_p50_ms": float(np.percentile(svc, 50)), "service_p99_ms": float(np.percentile(svc, 99)),
+            "max_residual": max(r["info"]["residual"] for r in res),
+            "fallbacks": int(sum(r["info"]["fallback"] != 0 for r in res))}
+
+
 SOLVERS = ("gmres-ir-fp32", "lu-fp16", "lu-bf16", "gmres-fp64", "lu-fp64")
 DTYPE = {"gmres-ir-fp32": "fp32/f64", "lu-fp16": "fp16/fp32+f64", "lu-bf16": "bf16/fp32+f64",
          "gmres-fp64": "f64", "lu-fp64": "f64"}
@@ -228,6 +292,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--sweep-steps", type=int, default=2,
                     help="timed 64-frequency sweeps on the config-3 geometry (0: skip)")
+    ap.add_argument("--serve-requests", type=int, default=48,
+                    help="requests of the config-5 serving stream (0: skip)")
+    ap.add_argument("--batch-cases", type=int, default=256,
+                    help="sphere positions of the config-4 uneven batch (0: skip)")
+    ap.add_argument("--slots", type=int, default=4, help="concurrent requests of the serving loop")
     ap.add_argument("--pmc-calibration-mib", type=int, default=0,
                     help="first copy this many MiB device-to-device (counter calibration runs)")
     args = ap.parse_args()
@@ -320,6 +389,15 @@ def main():
     if args.sweep_steps > 0:
         log("bench: frequency sweep (config 3)")
         sweep = sweep_bench(dep, ctx, args.sweep_steps)
+    serving = None
+    if args.serve_requests > 0:
+        log("bench: serving stream (config 5)")
+        serving = serve_bench(dep, args.serve_requests, args.slots)
+    batch = None
+    if args.batch_cases > 0:
+        log("bench: uneven position batch (config 4)")
+        batch = batch_bench(dep, args.batch_cases, args.slots)
+        log(f"bench: batch {batch}")
     cpu = None
     if not args.no_cpu_baseline:
         log("bench: cpu baseline (bounded sample)")
@@ -343,6 +421,8 @@ def main():
         "roofline": roofline(stats),
         "cpu_baseline": cpu,
         "sweep": sweep,
+        "serving": serving,
+        "position_batch": batch,
     }
     if cpu:
         out["speedup_vs_cpu"] = value / cpu["value"]

@@ -40,8 +40,25 @@ class SolveInfo(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
-_lib = None
 _p = ctypes.c_void_p
+
+
+class Request(ctypes.Structure):
+    """dep_request: one depSolver run for the serving loop."""
+    _fields_ = [("n_nodes", ctypes.c_int), ("xyz", _p), ("n_elems", ctypes.c_int), ("npe", ctypes.c_int),
+                ("conn", _p), ("bctype", _p), ("iface", _p), ("n_mats", ctypes.c_int), ("mats", _p),
+                ("n_ifaces", ctypes.c_int), ("ifaces", _p), ("freq", ctypes.c_double), ("rhs", _p),
+                ("solver", ctypes.c_int), ("tol", ctypes.c_double), ("want_mst", ctypes.c_int),
+                ("eps_medium", ctypes.c_double)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("Fcm", ctypes.c_double * 3), ("Xcm", ctypes.c_double * 3),
+                ("info", SolveInfo), ("queue_ms", ctypes.c_double), ("service_ms", ctypes.c_double),
+                ("slot", ctypes.c_int), ("device_bytes", ctypes.c_size_t), ("error", ctypes.c_char * 256)]
+
+
+_lib = None
 _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
@@ -87,6 +104,13 @@ def lib():
         "dep_sweep_get_solution": ([_p, i, _f64p], i),
         "dep_sweep_system": ([_p, i, ctypes.POINTER(_p)], i),
         "dep_sweep_mst": ([_p, d, _f64p, _f64p], i),
+        "dep_server_create": ([i, i, ctypes.c_size_t, ctypes.POINTER(_p)], i),
+        "dep_server_submit": ([_p, ctypes.POINTER(Request), _p, ctypes.POINTER(ctypes.c_uint64)], i),
+        "dep_server_wait": ([_p, ctypes.c_uint64, ctypes.POINTER(Result)], i),
+        "dep_request_footprint": ([ctypes.POINTER(Request)], ctypes.c_size_t),
+        "dep_server_stats": ([_p, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_size_t),
+                              ctypes.POINTER(ctypes.c_size_t)], i),
+        "dep_server_destroy": ([_p], i),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -292,4 +316,72 @@ class Sweep:
     def close(self):
         if self._h:
             lib().dep_sweep_destroy(self._h)
+            self._h = _p()
+
+
+SOLVER = {"gmres-ir-fp32": 0, "lu-fp16": 1, "lu-bf16": 2, "gmres-fp64": 3, "lu-fp64": 4}
+
+
+class Server:
+    """The serving loop (dep_server): `slots` concurrent depSolver runs on one
+    device, admitted in arrival order as slots and memory free up."""
+
+    def __init__(self, device: int = 0, slots: int = 4, mem_budget: int = 0):
+        self._h = _p()
+        rc = lib().dep_server_create(device, slots, mem_budget, ctypes.byref(self._h))
+        if rc != DEP_OK:
+            raise DepError(rc, "dep_server_create failed")
+        self._keep = {}
+
+    @staticmethod
+    def _request(p, solver, tol, want_mst):
+        n = p.n_nodes
+        bct = np.ascontiguousarray(p.bctype[:n], dtype=np.uint32)
+        iface = np.ascontiguousarray(p.bc[:n, 1].astype(np.int64) - 1, dtype=np.int32)
+        iface[(bct != 0) & (bct != 6)] = -1
+        arrs = dict(xyz=_c64(p.xyz), conn=np.ascontiguousarray(p.conn, dtype=np.int32), bctype=bct,
+                    iface=iface, mats=_c64(p.mats),
+                    ifaces=np.ascontiguousarray(p.ifaces.reshape(-1), dtype=np.uint32), rhs=_c64(p.rhs()))
+        rq = Request(n, arrs["xyz"].ctypes.data, p.n_elems, p.npe, arrs["conn"].ctypes.data,
+                     arrs["bctype"].ctypes.data, arrs["iface"].ctypes.data, len(p.mats), arrs["mats"].ctypes.data,
+                     len(p.ifaces), arrs["ifaces"].ctypes.data, float(p.freq), arrs["rhs"].ctypes.data,
+                     SOLVER[solver], float(tol), int(want_mst), float(p.mats[0, 1] * 8.854187817e-12))
+        return rq, arrs
+
+    @staticmethod
+    def footprint(p, solver="gmres-ir-fp32", want_mst=True) -> int:
+        rq, arrs = Server._request(p, solver, 1e-12, want_mst)
+        return int(lib().dep_request_footprint(ctypes.byref(rq)))
+
+    def submit(self, p, solver="gmres-ir-fp32", tol=1e-12, want_mst=True, want_sigma=True) -> int:
+        rq, arrs = self._request(p, solver, tol, want_mst)
+        sigma = np.zeros(2 * p.n_nodes) if want_sigma else None
+        t = ctypes.c_uint64()
+        rc = lib().dep_server_submit(self._h, ctypes.byref(rq), None if sigma is None else sigma.ctypes.data,
+                                     ctypes.byref(t))
+        if rc != DEP_OK:
+            raise DepError(rc, "dep_server_submit failed")
+        self._keep[t.value] = sigma
+        return t.value
+
+    def wait(self, ticket):
+        """(sigma, result dict); raises DepError if the request failed."""
+        r = Result()
+        rc = lib().dep_server_wait(self._h, ticket, ctypes.byref(r))
+        sigma = self._keep.pop(ticket, None)
+        out = {"status": r.status, "Fcm": np.array(r.Fcm[:]), "Xcm": np.array(r.Xcm[:]),
+               "info": r.info.as_dict(), "queue_ms": r.queue_ms, "service_ms": r.service_ms,
+               "slot": r.slot, "device_bytes": int(r.device_bytes), "error": r.error.decode()}
+        if rc != DEP_OK:
+            raise DepError(rc, f"request {ticket}: {out['error']}")
+        return sigma, out
+
+    def stats(self):
+        n, peak, bud = ctypes.c_long(), ctypes.c_size_t(), ctypes.c_size_t()
+        lib().dep_server_stats(self._h, ctypes.byref(n), ctypes.byref(peak), ctypes.byref(bud))
+        return {"completed": n.value, "peak_reserved": peak.value, "budget": bud.value}
+
+    def close(self):
+        if self._h:
+            lib().dep_server_destroy(self._h)
             self._h = _p()

@@ -45,6 +45,16 @@ int dep_fail(dep_ctx* c, int code, const char* fmt, ...)
 
 int dep_malloc(dep_ctx* c, void** p, size_t bytes)
 {
+    if (c->arena) {   // carve from the arena; past its end, an ordinary allocation
+        const size_t off = (c->arena_used + 255) & ~(size_t)255;
+        if (off + bytes <= c->arena_size) {
+            *p = c->arena + off;
+            c->arena_used = off + bytes;
+            c->arena_peak = std::max(c->arena_peak, c->arena_used);
+            c->bytes += bytes;
+            return DEP_OK;
+        }
+    }
     hipError_t e = hipMalloc(p, bytes);
     if (e != hipSuccess) {
         (void)hipGetLastError();
@@ -59,8 +69,9 @@ int dep_malloc(dep_ctx* c, void** p, size_t bytes)
 void dep_free(dep_ctx* c, void* p, size_t bytes)
 {
     if (!p) return;
-    (void)hipFree(p);
     c->bytes -= bytes;
+    if (c->arena && (char*)p >= c->arena && (char*)p < c->arena + c->arena_size) return;   // freed with the arena
+    (void)hipFree(p);
 }
 
 template <class Obj>

@@ -51,6 +51,11 @@ struct dep_ctx {
     };
     std::vector<Pending> pending;
     std::vector<hipEvent_t> pool;
+    // optional device arena (the serving loop's slots): allocations are carved
+    // from one buffer and released together, so a slot never calls hipFree
+    // (which synchronises the whole device) between requests
+    char* arena = nullptr;
+    size_t arena_size = 0, arena_used = 0, arena_peak = 0;
 };
 
 // time one launch: prof_begin before, prof_end after (no-ops unless profiling)

@@ -1,0 +1,346 @@
+// dep_server.cpp -- the serving loop: depSolver runs admitted as device slots
+// and memory free up, executed concurrently on one GPU.
+//
+// Each slot owns a context (its own non-blocking HIP streams) and a host
+// thread, and allocates everything a request needs from its own device
+// arena, which is kept between requests: a slot never frees device memory in
+// steady state (hipFree synchronises the whole device and would stall the
+// other slots' work).  The reserved memory is the sum of the slots' arenas;
+// a request is admitted, in arrival order, to a free slot when the arena that
+// slot needs keeps the sum within the budget.  Under memory pressure, idle
+// slots are asked to release their arenas.  Requests on different slots
+// overlap on the device (the GMRES host round trips of one leave the device
+// to the others); a request's arithmetic does not depend on the others, since
+// it runs the same kernels on its own buffers and stream.
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "dep_internal.h"
+
+namespace {
+
+using clk = std::chrono::steady_clock;
+
+double ms_between(clk::time_point a, clk::time_point b)
+{
+    return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+struct Job {
+    uint64_t ticket = 0;
+    int N = 0, E = 0, npe = 0, nM = 0, nI = 0, solver = 0, want_mst = 0;
+    double freq = 0.0, tol = 0.0, eps = 0.0;
+    std::vector<double> xyz, mats, rhs;
+    std::vector<int32_t> conn, iface;
+    std::vector<uint32_t> bctype, ifaces;
+    double* sigma = nullptr;
+    size_t need = 0;
+    clk::time_point t_submit, t_start, t_end;
+    bool done = false;
+    dep_result res{};
+};
+
+struct Slot {
+    std::thread th;
+    dep_ctx* ctx = nullptr;
+    size_t arena = 0;        // bytes of the slot's arena
+    bool busy = false;
+    bool release = false;    // asked to free its arena while idle
+};
+
+}  // namespace
+
+struct dep_server {
+    int device = 0;
+    size_t budget = 0;
+    std::mutex mu;
+    std::condition_variable cv;   // queue, memory and completion changes
+    std::deque<Job*> queue;
+    std::unordered_map<uint64_t, Job*> jobs;
+    uint64_t next = 1;
+    bool stop = false;
+    std::vector<Slot> slots;
+    size_t reserved = 0, peak = 0;
+    long completed = 0;
+};
+
+extern "C" size_t dep_request_footprint(const dep_request* rq)
+{
+    if (!rq || rq->n_nodes <= 0 || rq->n_elems <= 0) return 0;
+    const size_t N = rq->n_nodes, E = rq->n_elems, npe = rq->npe;
+    const size_t ldP = (N + 63) / 64 * 64, nb = (N + 63) / 64;
+    size_t b = 8 * N * ldP;                                   // geometric blocks P
+    b += 8 * npe * E * npe;                                   // singular blocks
+    b += 128 * N + E * (7 * 32 + 8 * npe + 8 * npe * npe);    // geometry, incidence, patches, quadrature
+    b += 256 * N + 65536 * 8 + 16 * 7 * E;                    // system vectors, work, reductions, sigma at points
+    const size_t cap = 601;
+    switch (rq->solver) {
+    case DEP_SOLVER_GMRES_IR:
+        b += 4 * N * ldP + cap * 16 * N + 72 * N + 8 * (34 * (cap + 2) + 128);
+        break;
+    case DEP_SOLVER_GMRES_FP64:
+        b += 601 * 16 * N + 72 * N + 8 * (34 * 602 + 128);
+        break;
+    case DEP_SOLVER_LU_FP64:
+        b += 16 * N * ldP + 8 * nb * 4 * 4096;
+        break;
+    default: {   // FP16 / BF16 LU
+        const size_t ld32 = (N + 127) / 128 * 128;
+        b += 8 * N * ld32 + 4 * nb * 4 * 4096 + 2 * (N + 2 * ld32 + 2 * DEP_LU_NBO) * 2 * DEP_LU_NBO + 16 * N;
+        break;
+    }
+    }
+    if (rq->want_mst)   // point set of the bc-6 nodes (at most N): partials, pairs, coefficients
+        b += ((size_t)13 << 20) + 2048 * N + 56 * E;
+    b += b / 20 + ((size_t)4 << 20);                          // alignment and slack
+    return b;
+}
+
+namespace {
+
+int run_job(dep_ctx* c, Job* j)
+{
+    dep_geom* g = nullptr;
+    dep_sys* s = nullptr;
+    dep_result& R = j->res;
+    int r = dep_geom_create(c, j->N, j->xyz.data(), j->E, j->npe, j->conn.data(), j->bctype.data(),
+                            j->iface.empty() ? nullptr : j->iface.data(), &g);
+    if (r == DEP_OK) r = dep_geom_assemble(g);
+    if (r == DEP_OK)
+        r = dep_sys_create(g, j->nM, j->mats.data(), j->nI, j->ifaces.data(), j->freq, j->rhs.data(), &s);
+    if (r == DEP_OK) {
+        switch (j->solver) {
+        // corrections of up to 600 steps: the two close surfaces of a shelled particle slow
+        // GMRES down (200-step corrections needed up to 19 refinements there, 600-step ones 3)
+        case DEP_SOLVER_GMRES_IR: r = dep_gmres_ir(s, j->tol, 1e-6, 600, 20, nullptr, &R.info); break;
+        case DEP_SOLVER_GMRES_FP64: r = dep_gmres(s, 1, j->tol, 600, nullptr, nullptr, &R.info, nullptr); break;
+        default: {
+            const int prec = j->solver == DEP_SOLVER_LU_FP64 ? DEP_PREC_FP64
+                             : j->solver == DEP_SOLVER_LU_BF16 ? DEP_PREC_BF16
+                                                               : DEP_PREC_FP16;
+            r = dep_factor(s, prec);
+            if (r == DEP_OK) r = dep_solve(s, j->tol, 30, nullptr, &R.info);
+        }
+        }
+    }
+    if (r == DEP_OK && j->want_mst) {
+        std::vector<double> Fce(3 * (size_t)j->E), Xce(3 * (size_t)j->E);
+        r = dep_post_mst(s, j->eps, Fce.data(), Xce.data(), R.Fcm, R.Xcm);
+    }
+    if (r == DEP_OK && j->sigma) r = dep_get_solution(s, j->sigma);
+    if (r != DEP_OK) snprintf(R.error, sizeof(R.error), "%s", c->err.c_str());
+    dep_sys_destroy(s);
+    dep_geom_destroy(g);
+    return r;
+}
+
+void worker(dep_server* sv, int k)
+{
+    Slot& sl = sv->slots[k];
+    (void)hipSetDevice(sv->device);
+    std::unique_lock<std::mutex> lk(sv->mu);
+    for (;;) {
+        if (sl.release && !sl.busy) {   // give the arena back (idle, asked under memory pressure)
+            char* a = sl.ctx->arena;
+            const size_t n = sl.arena;
+            sl.ctx->arena = nullptr;
+            sl.ctx->arena_size = sl.ctx->arena_used = 0;
+            sl.arena = 0;
+            sl.release = false;
+            lk.unlock();
+            if (a) (void)hipFree(a);
+            lk.lock();
+            sv->reserved -= n;
+            sv->cv.notify_all();
+            continue;
+        }
+        if (sv->queue.empty()) {
+            if (sv->stop) break;
+            sv->cv.wait(lk);
+            continue;
+        }
+        Job* j = sv->queue.front();
+        if (j->need > sv->budget) {   // can never fit
+            sv->queue.pop_front();
+            j->res.status = DEP_ERR_ADMISSION;
+            snprintf(j->res.error, sizeof(j->res.error), "request needs %zu bytes, budget %zu", j->need,
+                     sv->budget);
+            j->t_start = j->t_end = clk::now();
+            j->done = true;
+            sv->completed++;
+            sv->cv.notify_all();
+            continue;
+        }
+        const size_t grow = j->need > sl.arena ? j->need - sl.arena : 0;
+        if (sv->reserved + grow > sv->budget) {   // wait for memory: ask idle slots to release theirs
+            for (auto& o : sv->slots)
+                if (&o != &sl && !o.busy && o.arena > 0) o.release = true;
+            sv->cv.notify_all();
+            sv->cv.wait(lk);
+            continue;
+        }
+        sv->queue.pop_front();
+        sl.busy = true;
+        sl.release = false;
+        sv->reserved += grow;
+        sv->peak = std::max(sv->peak, sv->reserved);
+        const size_t target = sl.arena + grow;
+        j->t_start = clk::now();
+        lk.unlock();
+
+        int r = DEP_OK;
+        if (grow > 0) {   // grow the arena (the only device allocation of a slot in steady state)
+            if (sl.ctx->arena) (void)hipFree(sl.ctx->arena);
+            sl.ctx->arena = nullptr;
+            void* p = nullptr;
+            if (hipMalloc(&p, target) != hipSuccess) {
+                (void)hipGetLastError();
+                r = DEP_ERR_NOMEM;
+                snprintf(j->res.error, sizeof(j->res.error), "hipMalloc of a %zu-byte arena failed", target);
+            } else {
+                sl.ctx->arena = (char*)p;
+                sl.ctx->arena_size = target;
+            }
+        }
+        sl.ctx->arena_used = sl.ctx->arena_peak = 0;
+        if (r == DEP_OK) r = run_job(sl.ctx, j);
+        j->res.status = r;
+        j->res.slot = k;
+        j->res.device_bytes = sl.ctx->arena_peak;
+
+        lk.lock();
+        if (grow > 0 && !sl.ctx->arena) {   // the arena could not be allocated
+            sv->reserved -= target;
+            sl.arena = 0;
+        } else {
+            sl.arena = target;
+        }
+        j->t_end = clk::now();
+        j->done = true;
+        sl.busy = false;
+        sv->completed++;
+        sv->cv.notify_all();
+    }
+}
+
+}  // namespace
+
+extern "C" int dep_server_destroy(dep_server* sv);
+
+extern "C" int dep_server_create(int device, int nslots, size_t budget, dep_server** out)
+{
+    if (!out || nslots < 1 || nslots > 16) return DEP_ERR_ARG;
+    *out = nullptr;
+    if (hipSetDevice(device) != hipSuccess) return DEP_ERR_HIP;
+    if (budget == 0) {
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) != hipSuccess) return DEP_ERR_HIP;
+        budget = fr / 10 * 9;
+    }
+    dep_server* sv = new dep_server();
+    sv->device = device;
+    sv->budget = budget;
+    sv->slots.resize(nslots);
+    for (auto& sl : sv->slots)
+        if (dep_ctx_create(device, &sl.ctx) != DEP_OK) {
+            dep_server_destroy(sv);
+            return DEP_ERR_HIP;
+        }
+    for (int k = 0; k < nslots; k++) sv->slots[k].th = std::thread(worker, sv, k);
+    *out = sv;
+    return DEP_OK;
+}
+
+extern "C" int dep_server_submit(dep_server* sv, const dep_request* rq, double* sigma, uint64_t* ticket)
+{
+    if (!sv || !rq || !ticket || rq->n_nodes <= 0 || rq->n_elems <= 0 || (rq->npe != 3 && rq->npe != 6) ||
+        !rq->xyz || !rq->conn || !rq->bctype || !rq->rhs || (rq->n_mats > 0 && !rq->mats) ||
+        (rq->n_ifaces > 0 && !rq->ifaces) || rq->solver < 0 || rq->solver > DEP_SOLVER_LU_FP64 || !(rq->tol > 0.0))
+        return DEP_ERR_ARG;
+    Job* j = new Job();
+    const size_t N = rq->n_nodes, E = rq->n_elems;
+    j->N = rq->n_nodes;
+    j->E = rq->n_elems;
+    j->npe = rq->npe;
+    j->nM = rq->n_mats;
+    j->nI = rq->n_ifaces;
+    j->solver = rq->solver;
+    j->want_mst = rq->want_mst;
+    j->freq = rq->freq;
+    j->tol = rq->tol;
+    j->eps = rq->eps_medium;
+    j->xyz.assign(rq->xyz, rq->xyz + 3 * N);
+    j->conn.assign(rq->conn, rq->conn + E * rq->npe);
+    j->bctype.assign(rq->bctype, rq->bctype + N);
+    if (rq->iface) j->iface.assign(rq->iface, rq->iface + N);
+    j->mats.assign(rq->mats, rq->mats + 2 * (size_t)rq->n_mats);
+    j->ifaces.assign(rq->ifaces, rq->ifaces + 2 * (size_t)rq->n_ifaces);
+    j->rhs.assign(rq->rhs, rq->rhs + 2 * N);
+    j->sigma = sigma;
+    j->need = dep_request_footprint(rq);
+    j->t_submit = clk::now();
+    std::lock_guard<std::mutex> lk(sv->mu);
+    j->ticket = sv->next++;
+    sv->jobs[j->ticket] = j;
+    sv->queue.push_back(j);
+    *ticket = j->ticket;
+    sv->cv.notify_all();
+    return DEP_OK;
+}
+
+extern "C" int dep_server_wait(dep_server* sv, uint64_t ticket, dep_result* out)
+{
+    if (!sv || !out) return DEP_ERR_ARG;
+    std::unique_lock<std::mutex> lk(sv->mu);
+    auto it = sv->jobs.find(ticket);
+    if (it == sv->jobs.end()) return DEP_ERR_ARG;
+    Job* j = it->second;
+    sv->cv.wait(lk, [&] { return j->done; });
+    *out = j->res;
+    out->queue_ms = ms_between(j->t_submit, j->t_start);
+    out->service_ms = ms_between(j->t_start, j->t_end);
+    sv->jobs.erase(it);
+    delete j;
+    return out->status;
+}
+
+extern "C" int dep_server_stats(dep_server* sv, long* completed, size_t* peak_reserved, size_t* budget)
+{
+    if (!sv) return DEP_ERR_ARG;
+    std::lock_guard<std::mutex> lk(sv->mu);
+    if (completed) *completed = sv->completed;
+    if (peak_reserved) *peak_reserved = sv->peak;
+    if (budget) *budget = sv->budget;
+    return DEP_OK;
+}
+
+extern "C" int dep_server_destroy(dep_server* sv)
+{
+    if (!sv) return DEP_OK;
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        sv->stop = true;
+        sv->cv.notify_all();
+    }
+    for (auto& sl : sv->slots)
+        if (sl.th.joinable()) sl.th.join();
+    for (auto& sl : sv->slots)
+        if (sl.ctx) {
+            if (sl.ctx->arena) (void)hipFree(sl.ctx->arena);
+            sl.ctx->arena = nullptr;
+            dep_ctx_destroy(sl.ctx);
+        }
+    for (auto& kv : sv->jobs) delete kv.second;
+    delete sv;
+    return DEP_OK;
+}

@@ -268,6 +268,44 @@ def config4_cases(n: int = 256, seed: int = 20261203, levels=(2, 3, 4)):
     return out
 
 
+def config4_problems(n: int = 256, seed: int = 20261203):
+    """The uneven particle-position batch: one sphere per case in the quadrupole trap."""
+    return [quadrupole_trap(lvl, r, c, name=f"quadrupole-L{lvl}-{k}")
+            for k, (lvl, r, c) in enumerate(config4_cases(n, seed))]
+
+
+def config5_stream(n: int = 64, seed: int = 20261203, rate: float = 200.0, n_max: int = 30000):
+    """The serving stream: (arrival time s, Problem) with Poisson arrivals at
+    `rate` per second; each request holds 1-3 shelled particles (surface
+    levels from {2, 3, 4}, radius 0.5-1.0, centres in |x|,|y| <= 2,
+    1 <= z <= 4, no overlap, clear of the electrodes) in the quadrupole trap,
+    at a log-uniform frequency in 10 kHz - 50 MHz, with at most n_max nodes."""
+    rng = np.random.default_rng(seed)
+    t, out = 0.0, []
+    plates = 4 * (2 * 16 + 1) ** 2
+    nodes = {lvl: 2 * (40 * 4 ** lvl + 2) for lvl in (2, 3, 4)}   # two quadratic icospheres per particle
+    for _ in range(n):
+        t += float(rng.exponential(1.0 / rate))
+        while True:
+            k = int(rng.integers(1, 4))
+            parts = []
+            for _ in range(200):
+                if len(parts) == k:
+                    break
+                r = float(rng.uniform(0.5, 1.0))
+                c = np.array([rng.uniform(-2, 2), rng.uniform(-2, 2), rng.uniform(1, 4)])
+                if c[2] - 1.05 * r < 0.25:
+                    continue
+                if any(np.linalg.norm(c - q[2]) < 1.05 * (r + q[1]) + 0.25 for q in parts):
+                    continue
+                parts.append((int(rng.choice((2, 3, 4))), r, c))
+            if len(parts) == k and plates + sum(nodes[q[0]] for q in parts) <= n_max:
+                break
+        f = float(10 ** rng.uniform(4.0, np.log10(5e7)))
+        out.append((t, shelled_trap([(q[0], q[1], tuple(q[2])) for q in parts], f)))
+    return out
+
+
 def shelled_trap(particles, freq: float, n_grid: int = 16, quadratic: bool = True) -> Problem:
     """Quadrupole trap with shelled particles.  ``particles`` is a list of
     (level, radius, centre).  Materials: 1 medium, 2 shell, 3 core; the outer

@@ -171,6 +171,68 @@ int dep_sweep_system(dep_sweep *w, int f, dep_sys **sys);
 /* total Maxwell-stress force per frequency (Fcm: nf x 3) and the force centre Xcm (3) */
 int dep_sweep_mst(dep_sweep *w, double eps_medium, double *Fcm, double *Xcm);
 
+/* ---- serving: a request loop with memory admission -----------------------
+ * No counterpart in the reference (one depSolver process per problem).  A
+ * request is a whole depSolver run on a new geometry: assembly, solve to an
+ * FP64 residual, optionally the Maxwell-stress force of the bc-6 surface.
+ * The server runs `slots` requests at once, each on its own context (own HIP
+ * streams, own host thread), so their kernels overlap on the device.
+ * Requests are admitted in arrival order whenever a slot is free and the
+ * reserved device memory (the slots' arenas) stays within mem_budget bytes;
+ * a request larger than the budget fails with DEP_ERR_ADMISSION.  Results do
+ * not depend on what else runs: each request executes the same kernels on
+ * its own buffers. */
+#define DEP_SOLVER_GMRES_IR 0     /* refined FP32 GMRES (dep_gmres_ir) */
+#define DEP_SOLVER_LU_FP16 1      /* FP16 MFMA LU + FP64 refinement (dep_solve) */
+#define DEP_SOLVER_LU_BF16 2
+#define DEP_SOLVER_GMRES_FP64 3   /* plain FP64 GMRES, Jacobi-scaled (dep_gmres) */
+#define DEP_SOLVER_LU_FP64 4      /* plain FP64 LU (dep_factor + dep_solve) */
+
+typedef struct dep_server dep_server;
+typedef struct dep_request {
+    /* geometry as dep_geom_create */
+    int n_nodes;
+    const double *xyz;
+    int n_elems, npe;
+    const int32_t *conn;
+    const uint32_t *bctype;
+    const int32_t *iface;
+    /* materials and right-hand side as dep_sys_create */
+    int n_mats;
+    const double *mats;
+    int n_ifaces;
+    const uint32_t *ifaces;
+    double freq;
+    const double *rhs;
+    int solver;          /* DEP_SOLVER_* */
+    double tol;          /* relative FP64 residual */
+    int want_mst;        /* 1: total Maxwell-stress force of the bc-6 surface */
+    double eps_medium;   /* permittivity of the medium (F/m) for the force */
+} dep_request;
+
+typedef struct dep_result {
+    int status;                 /* DEP_OK or a DEP_ERR_* code */
+    double Fcm[3], Xcm[3];      /* force and its centre (want_mst) */
+    dep_solve_info info;
+    double queue_ms;            /* submission to admission */
+    double service_ms;          /* admission to completion */
+    int slot;                   /* slot that ran it */
+    size_t device_bytes;        /* arena bytes the request used */
+    char error[256];
+} dep_result;
+
+/* mem_budget 0: 90% of the device memory free at creation */
+int dep_server_create(int device, int slots, size_t mem_budget, dep_server **out);
+/* Inputs are copied; sigma (2N, reference node order) may be NULL and must
+ * stay valid until dep_server_wait returns for the ticket. */
+int dep_server_submit(dep_server *sv, const dep_request *rq, double *sigma, uint64_t *ticket);
+int dep_server_wait(dep_server *sv, uint64_t ticket, dep_result *out);
+/* device bytes a request is charged at admission */
+size_t dep_request_footprint(const dep_request *rq);
+/* completed requests and peak reserved bytes since creation */
+int dep_server_stats(dep_server *sv, long *completed, size_t *peak_reserved, size_t *budget);
+int dep_server_destroy(dep_server *sv);   /* finishes the queued requests first */
+
 /* ---- reference-signature shims (src/depSolver.c:284-344) ----------------- */
 int electricFormA_tria3(double **mNodes, unsigned int **mElems, unsigned int *vBCType,
                         unsigned int **vInterfaces, double **vMatParam, double *vProbParam,

@@ -1,0 +1,93 @@
+"""GPU: the serving loop (dep_server) -- continuous admission of depSolver
+runs on one device, each slot on its own context and streams.
+
+* a request's sigma and force are the same bits alone, in a mix of other
+  requests running concurrently, and through the single-context API;
+* served solutions reach the FP64 LU of the same problem (1e-10 sigma,
+  1e-8 force relative to the element-force scale);
+* memory admission: with a budget that holds one large request at a time
+  every request still completes and the reserved peak stays in the budget;
+  a request larger than the budget is refused with DEP_ERR_ADMISSION.
+"""
+import numpy as np
+import pytest
+
+from depsolver_amd import mesh
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def stream():
+    return [p for _, p in mesh.config5_stream(8, seed=11, n_max=12000)]
+
+
+@pytest.mark.parametrize("solver", ["gmres-ir-fp32", "lu-fp16"])
+def test_request_is_the_same_alone_and_in_the_mix(ctx, stream, solver):
+    import depsolver_amd as dep
+    print(f"[gpu] serving mix {solver}", flush=True)
+    target, others = stream[0], stream[1:]
+    sv = dep.Server(0, slots=4)
+    try:
+        s_alone, r_alone = sv.wait(sv.submit(target, solver))
+        tickets = [sv.submit(p, solver) for p in others[:3]]
+        tickets.append(sv.submit(target, solver))
+        tickets += [sv.submit(p, solver) for p in others[3:]]
+        res = [sv.wait(t) for t in tickets]
+    finally:
+        sv.close()
+    s_mix, r_mix = res[3]
+    assert r_mix["info"]["converged"] == 1, r_mix
+    assert np.array_equal(s_alone, s_mix)
+    assert np.array_equal(r_alone["Fcm"], r_mix["Fcm"])
+    assert len({r["slot"] for _, r in res}) > 1   # the mix ran on several slots
+    if solver == "gmres-ir-fp32":   # the single-context API runs the same kernels
+        g = dep.Geometry(ctx, target)
+        s = dep.System(g)
+        try:
+            sig, info = s.gmres_ir(tol=1e-12, inner_tol=1e-6)
+            assert np.array_equal(sig, s_alone)
+        finally:
+            s.close()
+            g.close()
+
+
+def test_served_solution_matches_fp64_lu(ctx, stream):
+    import depsolver_amd as dep
+    p = stream[0]
+    sv = dep.Server(0, slots=2)
+    try:
+        sig, r = sv.wait(sv.submit(p, "gmres-ir-fp32"))
+    finally:
+        sv.close()
+    g = dep.Geometry(ctx, p)
+    s = dep.System(g)
+    try:
+        s.factor("fp64")
+        want, _ = s.solve(tol=1e-13, max_steps=3)
+        Fce, Xce, Fcm, Xcm = s.mst()
+    finally:
+        s.close()
+        g.close()
+    assert np.abs(sig - want).max() <= 1e-10 * np.abs(want).max()
+    assert np.abs(r["Fcm"] - Fcm).max() <= 1e-8 * np.abs(Fce).sum()
+    assert np.array_equal(r["Xcm"], Xcm)
+
+
+def test_memory_admission(stream):
+    import depsolver_amd as dep
+    big = max(dep.Server.footprint(p) for p in stream)
+    budget = int(big * 1.2)
+    sv = dep.Server(0, slots=4, mem_budget=budget)
+    try:
+        tickets = [sv.submit(p) for p in stream]
+        res = [sv.wait(t) for t in tickets]
+        st = sv.stats()
+        assert st["completed"] == len(stream)
+        assert st["peak_reserved"] <= budget
+        assert all(r["info"]["converged"] == 1 for _, r in res)
+        with pytest.raises(dep.DepError) as e:
+            sv.wait(sv.submit(mesh.config2()))
+        assert e.value.code == -7   # DEP_ERR_ADMISSION
+    finally:
+        sv.close()
