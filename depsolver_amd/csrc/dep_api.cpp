@@ -441,6 +441,7 @@ extern "C" int dep_geom_assemble(dep_geom* g)
     DEP_HIP(c, launch_assemble(g));
     DEP_HIP(c, hipStreamSynchronize(c->stream));
     g->assembled = true;
+    g->asm_count++;
     return DEP_OK;
 }
 
@@ -1394,9 +1395,11 @@ extern "C" int dep_gmres(dep_sys* s, int precond, double tol_rel, int max_iters,
 }
 
 // Refined FP32 GMRES: FP64 residuals r = b - Z x from the FP64 blocks P; each
-// correction solves D Z32 d = D r by GMRES on the FP32 copy of P (half the bytes
-// per product) with the Jacobi scaling D of dep_gmres applied to both halves of
-// r, to the relative tolerance inner_tol; x += d until |r| <= tol |b|.
+// correction solves Z32 d = r by GMRES on the FP32 copy of P, right-
+// preconditioned by the block-Jacobi inverse M^-1 of Z's 64x64 diagonal blocks
+// (Morton node order), to the relative tolerance inner_tol; x += d until
+// |r| <= tol |b|.  With DEPSOLVER_GMRES_PRECOND=jacobi the corrections use the
+// row scaling of dep_gmres instead (D Z32 d = D r).
 extern "C" int dep_gmres_ir(dep_sys* s, double tol_rel, double inner_tol, int max_inner, int max_outer,
                             double* sigma, dep_solve_info* info)
 {
@@ -1408,21 +1411,40 @@ extern "C" int dep_gmres_ir(dep_sys* s, double tol_rel, double inner_tol, int ma
     const long n2 = 2L * N;
     DEP_CHECK(ensure_gmres_storage(s, max_inner));
     DEP_CHECK(ensure_p32(g));
+    const char* pc = getenv("DEPSOLVER_GMRES_PRECOND");
+    const bool block = !(pc && strcmp(pc, "jacobi") == 0);
     double* gw = s->gm_vec;
     double* gwi = gw + N;
     double* gsh = gwi + N;
-    double* rs = gsh + N;      // scaled residual (inner right-hand side)
+    double* rs = gsh + N;      // scaled residual (Jacobi) / preconditioned correction (block)
     double* d = rs + n2;       // inner solution
-    double* w = d + n2;
+    double* w = d + n2;        // M^-1 v inside the product
     double* r = s->work;       // FP64 residual
     double* dsc = s->work + n2;   // Jacobi factors on the device (N)
-    std::vector<double> hd;
-    DEP_CHECK(jacobi_weights(s, 1, hd, gw, gwi, gsh));
-    DEP_CHECK(upload(c, dsc, hd.data(), N));
-    auto matvec32 = [&](const double* in, const double* rhs, double* out) -> hipError_t {
-        return launch_zmatvec32_w(s, gw, gwi, gsh, in, rhs, out);
-    };
     DEP_HIP(c, hipEventRecord(c->ev0, c->stream));
+    if (block) {
+        if (!s->bjB) {
+            DEP_CHECK(dep_obj_alloc(s, &s->bjB, bj_floats(N, 1)));
+            DEP_CHECK(dep_obj_alloc(s, &s->bj_scr, bj_scratch_doubles(N, 1)));
+        }
+        if (s->bj_asm != g->asm_count || s->bj_freq != s->freq) {
+            DEP_HIP(c, launch_bj_build(c, N, g->P, g->ldP, s->wr, s->wi, s->shift, 0, 1, s->bj_scr, s->bjB));
+            s->bj_asm = g->asm_count;
+            s->bj_freq = s->freq;
+        }
+    } else {
+        std::vector<double> hd;
+        DEP_CHECK(jacobi_weights(s, 1, hd, gw, gwi, gsh));
+        DEP_CHECK(upload(c, dsc, hd.data(), N));
+    }
+    dep_blist one;
+    one.n = 1;
+    auto matvec32 = [&](const double* in, const double* rhs, double* out) -> hipError_t {
+        if (!block) return launch_zmatvec32_w(s, gw, gwi, gsh, in, rhs, out);
+        hipError_t e = launch_bj_apply(c, N, s->bjB, one, in, 0, w, 0);
+        if (e != hipSuccess) return e;
+        return launch_zmatvec32_w(s, s->wr, s->wi, s->shift, w, rhs, out);
+    };
     double bn = 0.0, res = 0.0;
     DEP_CHECK(dev_norm(s, s->b, &bn));
     if (bn == 0.0) bn = 1.0;
@@ -1434,14 +1456,22 @@ extern "C" int dep_gmres_ir(dep_sys* s, double tol_rel, double inner_tol, int ma
         DEP_CHECK(dev_norm(s, r, &res));
         res /= bn;
         if (!(res > tol_rel) || outer >= max_outer || stalled) break;
-        DEP_HIP(c, launch_rowscale(c, N, dsc, r, rs));
+        const double* rhs = r;
+        if (!block) {
+            DEP_HIP(c, launch_rowscale(c, N, dsc, r, rs));
+            rhs = rs;
+        }
         double rsn = 0.0;
-        DEP_CHECK(dev_norm(s, rs, &rsn));
+        DEP_CHECK(dev_norm(s, rhs, &rsn));
         GmresRun run;
-        DEP_CHECK(gmres_run(s, matvec32, rs, true, d, w, inner_tol * rsn, max_inner, nullptr, &run));
+        DEP_CHECK(gmres_run(s, matvec32, rhs, true, d, w, inner_tol * rsn, max_inner, nullptr, &run));
         inner_total += run.iters;
         if (run.iters == 0) stalled = true;
-        DEP_HIP(c, launch_axpy(c, n2, d, s->x));
+        if (block) {
+            DEP_HIP(c, launch_bj_apply(c, N, s->bjB, one, d, 0, s->x, 0, 1));
+        } else {
+            DEP_HIP(c, launch_axpy(c, n2, d, s->x));
+        }
         outer++;
     }
     DEP_HIP(c, hipEventRecord(c->ev1, c->stream));

@@ -243,7 +243,7 @@ __global__ __launch_bounds__(256) void k_bcollect(int i, long n2, dep_blist L, d
 }
 
 // Block a of L: V_j[0] = D_j r_j / |D_j r_j| (both halves scaled by D_j, the
-// Jacobi factors), beta[a] = |D_j r_j|.  r_j = R + j * rstride.
+// Jacobi factors; D null: no scaling), beta[a] = |D_j r_j|.  r_j = R + j * rstride.
 __global__ __launch_bounds__(256) void k_bstart(int N, dep_blist L, const double* __restrict__ R, long rstride,
                                                 const double* __restrict__ D, double* __restrict__ V, long vstride,
                                                 double* __restrict__ beta)
@@ -252,11 +252,12 @@ __global__ __launch_bounds__(256) void k_bstart(int N, dep_blist L, const double
     __shared__ double bsh;
     const long j = L.id[blockIdx.x];
     const double* r = R + j * rstride;
-    const double* d = D + j * (long)N;
+    const double* d = D ? D + j * (long)N : nullptr;
     double* v0 = V + j * vstride;
     double sq = 0.0;
     for (int i = threadIdx.x; i < N; i += 256) {
-        const double a = d[i] * r[i], b = d[i] * r[N + i];
+        const double di = d ? d[i] : 1.0;
+        const double a = di * r[i], b = di * r[N + i];
         v0[i] = a;
         v0[N + i] = b;
         sq += a * a + b * b;

@@ -89,6 +89,7 @@ struct dep_geom {
     double* P = nullptr;
     long ldP = 0;
     bool assembled = false;
+    long asm_count = 0;                     // assemblies so far (invalidates derived data)
     int tc = DEP_TC;                        // columns per assembly patch (16, 32 or 64)
     struct dep_mst_cache* mst = nullptr;   // Maxwell-stress point set, built on first use
     float* P32 = nullptr;                   // FP32 copy of P for refined FP32 GMRES
@@ -132,6 +133,11 @@ struct dep_sys {
     // reductions and dot-product partials
     int gm_cap = 0;
     double *gm_vec = nullptr, *gm_V = nullptr, *gm_red = nullptr;
+    // block-Jacobi preconditioner of the refined FP32 GMRES (valid for bj_asm / bj_freq)
+    float* bjB = nullptr;
+    double* bj_scr = nullptr;
+    long bj_asm = -1;
+    double bj_freq = -1.0;
     size_t bytes = 0;                 // device bytes owned by the system
     std::vector<std::pair<void*, size_t>> allocs;
     dep_ctx* ctx_of() { return g->ctx; }
@@ -244,3 +250,14 @@ hipError_t launch_bnorm2(dep_ctx* c, long n2, const dep_blist& L, const double* 
 hipError_t launch_bcombine(dep_ctx* c, long n2, const dep_blist& L, const int* cnt, const double* Y, long ystride,
                            const double* V, long vstride, double* X, long xstride);
 hipError_t launch_bjacobi(dep_ctx* c, int N, int nf, const double* pd, const double* W, double* GW, double* D);
+
+// ---- block-Jacobi preconditioner (dep_precond.hip) --------------------------
+size_t bj_blocks(int N);
+size_t bj_scratch_doubles(int N, int nsys);
+size_t bj_floats(int N, int nsys);
+// inverse diagonal 64-blocks of nsys systems (weights wr/wi/sh + sys * wstride) into B
+hipError_t launch_bj_build(dep_ctx* c, int N, const double* P, long ldP, const double* wr, const double* wi,
+                           const double* sh, long wstride, int nsys, double* scratch, float* B);
+// Y_j = M_j^-1 X_j (accumulate: Y_j += M_j^-1 X_j) for the systems of L
+hipError_t launch_bj_apply(dep_ctx* c, int N, const float* B, const dep_blist& L, const double* X, long xstride,
+                           double* Y, long ystride, int accumulate = 0);

@@ -1,0 +1,156 @@
+// dep_precond.hip -- block-Jacobi preconditioner of the refined FP32 GMRES.
+//
+// M is the block diagonal of Z = diag(w) P + diag(shift) over 64-node blocks
+// of the internal (Morton) node order, so a block holds a compact patch of
+// the surface and its strongest near-field couplings (on a shelled particle
+// the two close surfaces land in the same or neighbouring blocks).  Each
+// block is factored without pivoting by the register-resident 64x64 LU of
+// dep_diag.h, which also forms L^-1 and U^-1; the preconditioner keeps
+// M_b^-1 = U^-1 L^-1 in FP32 split planes and applies it as one dense 64x64
+// complex product per block (right preconditioning: GMRES solves Z M^-1 y = r
+// and the correction is M^-1 y).  Systems of a sweep have their own blocks:
+// every kernel takes a system index in blockIdx.y and strides.
+#include "dep_common.h"
+#include "dep_diag.h"
+#include "dep_internal.h"
+
+namespace {
+
+constexpr int BJ = DIAG_NB;   // 64
+
+// Zb[sys][b] (split planes 64 x 64) = the diagonal block b of system sys,
+// identity where the last block runs past N.  System sys has the weights
+// wr, wi, sh (N each) at sys * wstride from the pointers given.
+__global__ __launch_bounds__(256) void k_bj_gather(int N, const double* __restrict__ P, long ldP,
+                                                   const double* __restrict__ wr, const double* __restrict__ wi,
+                                                   const double* __restrict__ sh, long wstride, int nblk,
+                                                   double* __restrict__ Zb)
+{
+    const int b = blockIdx.x, sys = blockIdx.y;
+    const long wo = (long)sys * wstride;
+    double* zr = Zb + ((long)sys * nblk + b) * 2 * BJ * BJ;
+    double* zi = zr + BJ * BJ;
+    for (int idx = threadIdx.x; idx < BJ * BJ; idx += 256) {
+        const int r = idx / BJ, c = idx % BJ, i = b * BJ + r, j = b * BJ + c;
+        double vr = r == c ? 1.0 : 0.0, vi = 0.0;
+        if (i < N && j < N) {
+            const double p = P[(long)i * ldP + j];
+            vr = wr[wo + i] * p + (i == j ? sh[wo + i] : 0.0);
+            vi = wi[wo + i] * p;
+        }
+        zr[idx] = vr;
+        zi[idx] = vi;
+    }
+}
+
+// LU and triangular inverses of every block (one 1024-thread workgroup each)
+__global__ __launch_bounds__(DIAG_THREADS) void k_bj_factor(int nblk, double* __restrict__ Zb,
+                                                            double* __restrict__ inv, double* __restrict__ piv)
+{
+    const long q = (long)blockIdx.y * nblk + blockIdx.x;
+    double* zr = Zb + q * 2 * BJ * BJ;
+    diag_factor_invert<double>(BJ, zr, zr + BJ * BJ, BJ, 0, inv + q * 4 * BJ * BJ, piv + q);
+}
+
+// B = U^-1 L^-1 of each block (complex, FP64 arithmetic), stored in FP32 split planes
+__global__ __launch_bounds__(256) void k_bj_combine(int nblk, const double* __restrict__ inv, float* __restrict__ B)
+{
+    __shared__ double Lr[BJ][BJ + 1], Li[BJ][BJ + 1];
+    const long q = (long)blockIdx.y * nblk + blockIdx.x;
+    const double* L = inv + q * 4 * BJ * BJ;   // interleaved complex, row-major
+    const double* U = L + 2 * BJ * BJ;
+    for (int idx = threadIdx.x; idx < BJ * BJ; idx += 256) {
+        Lr[idx / BJ][idx % BJ] = L[2 * idx];
+        Li[idx / BJ][idx % BJ] = L[2 * idx + 1];
+    }
+    __syncthreads();
+    float* br = B + q * 2 * BJ * BJ;
+    float* bi = br + BJ * BJ;
+    for (int idx = threadIdx.x; idx < BJ * BJ; idx += 256) {
+        const int r = idx / BJ, c = idx % BJ;
+        double sr = 0.0, si = 0.0;
+        for (int k = r; k < BJ; k++) {   // U^-1 is upper triangular
+            const double ur = U[2 * (r * BJ + k)], ui = U[2 * (r * BJ + k) + 1];
+            sr += ur * Lr[k][c] - ui * Li[k][c];
+            si += ur * Li[k][c] + ui * Lr[k][c];
+        }
+        br[idx] = (float)sr;
+        bi[idx] = (float)si;
+    }
+}
+
+// y_sys = M_sys^-1 x_sys (accumulate: y_sys += M_sys^-1 x_sys) for the systems
+// of L (x, y split: re block, im block).
+// Thread t: row t / 4 of the block, columns 16 (t % 4) .. + 15; the four
+// partial sums of a row are combined by lane shuffles.
+__global__ __launch_bounds__(256) void k_bj_apply(int N, int nblk, const float* __restrict__ B, dep_blist L,
+                                                  const double* __restrict__ X, long xstride, double* __restrict__ Y,
+                                                  long ystride, int accumulate)
+{
+    __shared__ double xr[BJ], xi[BJ];
+    const int b = blockIdx.x;
+    const long sys = L.id[blockIdx.y];
+    const double* x = X + sys * xstride;
+    double* y = Y + sys * ystride;
+    const int t = threadIdx.x, r = t >> 2, q = t & 3;
+    if (t < BJ) {
+        const int j = b * BJ + t;
+        xr[t] = j < N ? x[j] : 0.0;
+        xi[t] = j < N ? x[N + j] : 0.0;
+    }
+    __syncthreads();
+    const float* br = B + (sys * nblk + b) * 2 * BJ * BJ + r * BJ + 16 * q;
+    const float* bi = br + BJ * BJ;
+    double sr = 0.0, si = 0.0;
+#pragma unroll
+    for (int c = 0; c < 16; c++) {
+        const double ar = br[c], ai = bi[c], vr = xr[16 * q + c], vi = xi[16 * q + c];
+        sr += ar * vr - ai * vi;
+        si += ar * vi + ai * vr;
+    }
+    sr += __shfl_xor(sr, 1);
+    si += __shfl_xor(si, 1);
+    sr += __shfl_xor(sr, 2);
+    si += __shfl_xor(si, 2);
+    const int i = b * BJ + r;
+    if (q == 0 && i < N) {
+        if (accumulate) {
+            y[i] += sr;
+            y[N + i] += si;
+        } else {
+            y[i] = sr;
+            y[N + i] = si;
+        }
+    }
+}
+
+}  // namespace
+
+size_t bj_blocks(int N) { return (size_t)dep_cdiv(N, BJ); }
+
+// scratch: nsys * nblk * (2 + 4) * 64^2 + nsys * nblk doubles; B: nsys * nblk * 2 * 64^2 floats
+size_t bj_scratch_doubles(int N, int nsys) { return (size_t)nsys * bj_blocks(N) * (6 * BJ * BJ + 1); }
+size_t bj_floats(int N, int nsys) { return (size_t)nsys * bj_blocks(N) * 2 * BJ * BJ; }
+
+hipError_t launch_bj_build(dep_ctx* c, int N, const double* P, long ldP, const double* wr, const double* wi,
+                           const double* sh, long wstride, int nsys, double* scratch, float* B)
+{
+    const int nblk = (int)bj_blocks(N);
+    double* Zb = scratch;
+    double* inv = Zb + (size_t)nsys * nblk * 2 * BJ * BJ;
+    double* piv = inv + (size_t)nsys * nblk * 4 * BJ * BJ;
+    hipLaunchKernelGGL(k_bj_gather, dim3(nblk, nsys), dim3(256), 0, c->stream, N, P, ldP, wr, wi, sh, wstride, nblk,
+                       Zb);
+    hipLaunchKernelGGL(k_bj_factor, dim3(nblk, nsys), dim3(DIAG_THREADS), 0, c->stream, nblk, Zb, inv, piv);
+    hipLaunchKernelGGL(k_bj_combine, dim3(nblk, nsys), dim3(256), 0, c->stream, nblk, inv, B);
+    return hipGetLastError();
+}
+
+hipError_t launch_bj_apply(dep_ctx* c, int N, const float* B, const dep_blist& L, const double* X, long xstride,
+                           double* Y, long ystride, int accumulate)
+{
+    const int nblk = (int)bj_blocks(N);
+    hipLaunchKernelGGL(k_bj_apply, dim3(nblk, L.n), dim3(256), 0, c->stream, N, nblk, B, L, X, xstride, Y, ystride,
+                       accumulate);
+    return hipGetLastError();
+}

@@ -85,8 +85,8 @@ extern "C" size_t dep_request_footprint(const dep_request* rq)
     b += 256 * N + 65536 * 8 + 16 * 7 * E;                    // system vectors, work, reductions, sigma at points
     const size_t cap = 601;
     switch (rq->solver) {
-    case DEP_SOLVER_GMRES_IR:
-        b += 4 * N * ldP + cap * 16 * N + 72 * N + 8 * (34 * (cap + 2) + 128);
+    case DEP_SOLVER_GMRES_IR:   // + block-Jacobi inverses (FP32) and their build scratch
+        b += 4 * N * ldP + cap * 16 * N + 72 * N + 8 * (34 * (cap + 2) + 128) + (nb + 1) * (32768 + 196616);
         break;
     case DEP_SOLVER_GMRES_FP64:
         b += 601 * 16 * N + 72 * N + 8 * (34 * 602 + 128);

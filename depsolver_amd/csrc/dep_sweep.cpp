@@ -6,7 +6,8 @@
 // the frequency enters only through the interface weights A_k, B_k of
 // :496-512).  Here the geometric blocks P are integrated once and every
 // frequency f is the system Z_f = diag(w_f) P + diag(shift_f).  All systems
-// run the refined FP32 GMRES of dep_gmres_ir in lockstep: one batched FP64
+// run the refined FP32 GMRES of dep_gmres_ir (block-Jacobi right
+// preconditioning with each frequency's own blocks) in lockstep: one batched FP64
 // residual product per correction, and per Krylov step one batched FP32
 // product on the matrix cores plus batched CGS2 kernels, with one transfer of
 // all the systems' Hessenberg columns to the host, where each system's Givens
@@ -14,6 +15,7 @@
 // soon as it converges.
 #include <hip/hip_runtime.h>
 #include <math.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -40,6 +42,8 @@ struct dep_sweep {
     double* Yd = nullptr;         // DEP_BATCH_MAX x (cap + 1) combination coefficients
     int* cntd = nullptr;          // DEP_BATCH_MAX coefficient counts
     double* scal = nullptr;       // DEP_BATCH_MAX per-system scalars
+    bool block = true;            // block-Jacobi right preconditioning (else row scaling)
+    float* bjB = nullptr;         // nf x (N / 64) inverse diagonal blocks
     double* mres = nullptr;       // nf MST result slots
     size_t mres_n = 0;
     double* hpin = nullptr;       // pinned host staging
@@ -152,6 +156,19 @@ extern "C" int dep_sweep_create(dep_geom* g, int nM, const double* mats, int nI,
     if (launch_diag_gather(c, N, g->P, g->ldP, pd) != hipSuccess || launch_bjacobi(c, N, nf, pd, w->W, w->GW, w->D) != hipSuccess ||
         hipStreamSynchronize(c->stream) != hipSuccess)
         return fail(dep_fail(c, DEP_ERR_HIP, "sweep: Jacobi factors"));
+    // block-Jacobi inverses of every frequency's diagonal blocks (the build scratch is freed here)
+    const char* pc = getenv("DEPSOLVER_GMRES_PRECOND");
+    w->block = !(pc && strcmp(pc, "jacobi") == 0);
+    if (w->block) {
+        if ((r = dep_obj_alloc(w, &w->bjB, bj_floats(N, nf))) != DEP_OK) return fail(r);
+        double* scr = nullptr;
+        const size_t nscr = bj_scratch_doubles(N, nf) * sizeof(double);
+        if ((r = dep_malloc(c, (void**)&scr, nscr)) != DEP_OK) return fail(r);
+        const hipError_t e = launch_bj_build(c, N, g->P, g->ldP, w->W, w->W + N, w->W + 2 * N, 3L * N, nf, scr, w->bjB);
+        const hipError_t e2 = hipStreamSynchronize(c->stream);
+        dep_free(c, scr, nscr);
+        if (e != hipSuccess || e2 != hipSuccess) return fail(dep_fail(c, DEP_ERR_HIP, "sweep: block-Jacobi build"));
+    }
     std::vector<double> hb(n2);
     for (int i = 0; i < N; i++) {
         hb[i] = rhs[i];
@@ -221,7 +238,8 @@ extern "C" int dep_sweep_solve(dep_sweep* w, double tol, double inner_tol, int m
         }
         if (A.n == 0) break;
         // inner right-hand sides D_j r_j, normalised into V_j[0]
-        DEP_HIP(c, launch_bstart(c, N, A, first ? b : w->R, first ? 0 : n2, w->D, w->V, vstride, w->scal));
+        DEP_HIP(c, launch_bstart(c, N, A, first ? b : w->R, first ? 0 : n2, w->block ? nullptr : w->D, w->V, vstride,
+                                 w->scal));
         DEP_CHECK(down(c, hdown, w->scal, A.n));
         dep_blist I;
         std::vector<int> slot(nf, -1);   // index of a system in A
@@ -242,8 +260,14 @@ extern "C" int dep_sweep_solve(dep_sweep* w, double tol, double inner_tol, int m
             if (q.rc == 1) I.id[I.n++] = j;
         }
         for (int i = 0; I.n > 0 && i < M; i++) {
-            DEP_HIP(c, launch_bmatvec32(c, N, g->P32, g->ldP, I, w->V + (long)i * n2, vstride, w->part, w->GW, 3L * N,
-                                        w->V + (long)(i + 1) * n2, vstride));
+            if (w->block) {   // V_j[i+1] = Z_j M_j^-1 V_j[i], M_j^-1 V_j[i] staged in R (free inside a pass)
+                DEP_HIP(c, launch_bj_apply(c, N, w->bjB, I, w->V + (long)i * n2, vstride, w->R, n2));
+                DEP_HIP(c, launch_bmatvec32(c, N, g->P32, g->ldP, I, w->R, n2, w->part, w->W, 3L * N,
+                                            w->V + (long)(i + 1) * n2, vstride));
+            } else {
+                DEP_HIP(c, launch_bmatvec32(c, N, g->P32, g->ldP, I, w->V + (long)i * n2, vstride, w->part, w->GW,
+                                            3L * N, w->V + (long)(i + 1) * n2, vstride));
+            }
             DEP_HIP(c, launch_bcgs_pass(c, i + 2, n2, I, w->V, vstride, i + 1, w->red, rstride, poff, h1off, -1));
             DEP_HIP(c, launch_bcgs_pass(c, i + 2, n2, I, w->V, vstride, i + 1, w->red, rstride, poff, h2off, noff));
             DEP_HIP(c, launch_bcollect(c, i, n2, I, w->V, vstride, w->red, rstride, h1off, h2off, noff, w->cbuf));
@@ -301,7 +325,13 @@ extern "C" int dep_sweep_solve(dep_sweep* w, double tol, double inner_tol, int m
         DEP_HIP(c, hipMemcpyAsync(w->Yd, hY, sizeof(double) * (size_t)A.n * (w->cap + 1), hipMemcpyHostToDevice,
                                   c->stream));
         DEP_HIP(c, hipMemcpyAsync(w->cntd, hcnt, sizeof(int) * A.n, hipMemcpyHostToDevice, c->stream));
-        DEP_HIP(c, launch_bcombine(c, n2, A, w->cntd, w->Yd, w->cap + 1, w->V, vstride, w->X, n2));
+        if (w->block) {   // x_j += M_j^-1 (V_j y_j), V_j y_j gathered in R
+            DEP_HIP(c, hipMemsetAsync(w->R, 0, sizeof(double) * nf * n2, c->stream));
+            DEP_HIP(c, launch_bcombine(c, n2, A, w->cntd, w->Yd, w->cap + 1, w->V, vstride, w->R, n2));
+            DEP_HIP(c, launch_bj_apply(c, N, w->bjB, A, w->R, n2, w->X, n2, 1));
+        } else {
+            DEP_HIP(c, launch_bcombine(c, n2, A, w->cntd, w->Yd, w->cap + 1, w->V, vstride, w->X, n2));
+        }
         DEP_HIP(c, hipStreamSynchronize(c->stream));   // the staging is rewritten by the next pass
         first = false;
     }
