@@ -438,9 +438,10 @@ extern "C" int dep_geom_assemble(dep_geom* g)
         DEP_CHECK(dep_obj_alloc(g, &g->S, (size_t)g->npe * g->E * g->npe));
         DEP_HIP(c, hipMemsetAsync(g->P, 0, sizeof(double) * g->N * g->ldP, c->stream));
     }
-    DEP_HIP(c, launch_assemble(g));
+    DEP_HIP(c, launch_assemble(g));   // also writes the FP32 copy once one is kept
     DEP_HIP(c, hipStreamSynchronize(c->stream));
     g->assembled = true;
+    g->p32_valid = g->P32 != nullptr;
     g->asm_count++;
     return DEP_OK;
 }
@@ -1153,7 +1154,8 @@ static void sys_release(dep_sys* s, void* p)
         }
 }
 
-// FP32 copy of the geometric blocks (N x ldP), made on demand after each assembly
+// FP32 copy of the geometric blocks (N x ldP): converted from P on first use;
+// from then on every assembly writes it beside P
 int ensure_p32(dep_geom* g)
 {
     if (g->P32 && g->p32_valid) return DEP_OK;
@@ -1173,8 +1175,11 @@ static int ensure_gmres_storage(dep_sys* s, int max_iters)
         DEP_HIP(s->g->ctx, hipStreamSynchronize(s->g->ctx->stream));
         sys_release(s, s->gm_V);
         sys_release(s, s->gm_red);
-        s->gm_V = s->gm_red = nullptr;
+        sys_release(s, s->gm_dev);
+        s->gm_V = s->gm_red = s->gm_dev = nullptr;
     }
+    if (!s->gm_flags) DEP_CHECK(dep_obj_alloc(s, &s->gm_flags, 4));
+    DEP_CHECK(dep_obj_alloc(s, &s->gm_dev, (size_t)(max_iters + 1) * (max_iters + 2) + 4 * (size_t)(max_iters + 2) + 4));
     if (!s->gm_vec) DEP_CHECK(dep_obj_alloc(s, &s->gm_vec, 3 * N + 3 * n2));
     DEP_CHECK(dep_obj_alloc(s, &s->gm_V, (size_t)(max_iters + 1) * n2));
     // reductions: h1 (cap + 2), h2 (cap + 2), DEP_UPD_BLOCKS norm partials, dot partials
@@ -1311,6 +1316,75 @@ static int gmres_run(dep_sys* s, MV matvec, const double* b, bool x_zero, double
     return DEP_OK;
 }
 
+// The same iteration with the Givens recurrence on the device (k_gmres_givens):
+// solves Op d = b from d = 0 until the estimate drops below tol_rel |b| or
+// after M steps.  The host enqueues the steps in batches and reads the stop
+// flag once per batch; steps after the stop are skipped by the kernels.  The
+// triangular solve for the coefficients stays on the host (one transfer).
+template <class MV>
+static int gmres_run_dev(dep_sys* s, MV matvec, const double* b, double* d, double tol_rel, int M, GmresRun* out)
+{
+    dep_ctx* c = s->g->ctx;
+    const long n2 = 2L * s->g->N;
+    double* V = s->gm_V;
+    const int cap = s->gm_cap;
+    double* h1 = s->gm_red;
+    double* h2 = h1 + (cap + 2);
+    double* np = h2 + (cap + 2);
+    double* part = np + DEP_UPD_BLOCKS;
+    double* H = s->gm_dev;   // column i at H + i (cap + 2)
+    double* cs = H + (size_t)(cap + 1) * (cap + 2);
+    double* sn = cs + (cap + 2);
+    double* g = sn + (cap + 2);
+    double* err = g + (cap + 2);
+    double* sc = err + (cap + 2);
+    int* flags = s->gm_flags;
+    if (M > cap) return dep_fail(c, DEP_ERR_ARG, "gmres_run_dev: %d iterations exceed the workspace", M);
+    DEP_HIP(c, hipMemsetAsync(d, 0, sizeof(double) * n2, c->stream));
+    DEP_HIP(c, hipMemsetAsync(g, 0, sizeof(double) * (cap + 2), c->stream));
+    DEP_HIP(c, hipMemcpyAsync(V, b, sizeof(double) * n2, hipMemcpyDeviceToDevice, c->stream));
+    DEP_HIP(c, launch_dots(c, 1, n2, V, V, h1, part));
+    DEP_HIP(c, launch_gmres_init(c, h1, tol_rel, g, sc, flags));
+    DEP_HIP(c, launch_scale_dev(c, n2, V, sc, flags));
+    constexpr int BATCH = 8;
+    int hf[3] = {0, 0, 1};
+    for (int i0 = 0; i0 < M; i0 += BATCH) {
+        for (int i = i0; i < std::min(M, i0 + BATCH); i++) {
+            double* vi = V + (size_t)i * n2;
+            double* wv = V + (size_t)(i + 1) * n2;
+            DEP_HIP(c, matvec(vi, wv, flags));
+            DEP_HIP(c, launch_dot_partials(c, i + 2, n2, V, wv, part, flags));
+            DEP_HIP(c, launch_update_fused(c, i + 1, n2, V, part, wv, h1, nullptr, flags));
+            DEP_HIP(c, launch_dot_partials(c, i + 2, n2, V, wv, part, flags));
+            DEP_HIP(c, launch_update_fused(c, i + 1, n2, V, part, wv, h2, np, flags));
+            DEP_HIP(c, launch_gmres_givens(c, i, cap, h1, h2, np, H, cs, sn, g, err, sc, flags));
+            DEP_HIP(c, launch_scale_dev(c, n2, wv, sc, flags));
+        }
+        DEP_CHECK(download(c, hf, flags, 3));
+        if (hf[0]) break;
+    }
+    const int iters = hf[1], rc = hf[0] ? hf[2] : 1;
+    if (rc != 2 && iters > 0) {
+        std::vector<double> Hh((size_t)iters * (cap + 2)), y(iters + 1);
+        DEP_CHECK(download(c, Hh.data(), H, Hh.size()));
+        DEP_CHECK(download(c, y.data(), g, iters + 1));
+#define HD(r, col) Hh[(size_t)(col) * (cap + 2) + (r)]
+        for (int i = iters - 1; i >= 0; i--) {
+            for (int j = i + 1; j < iters; j++) y[i] -= HD(i, j) * y[j];
+            y[i] = y[i] / HD(i, i);
+        }
+#undef HD
+        DEP_CHECK(upload(c, h1, y.data(), iters));
+        DEP_HIP(c, launch_combine(c, iters, n2, V, h1, d));
+    }
+    double e1 = 0.0;
+    if (iters > 0) DEP_CHECK(download(c, &e1, err + iters - 1, 1));
+    out->iters = iters;
+    out->rc = rc;
+    out->err = e1;
+    return DEP_OK;
+}
+
 extern "C" int dep_gmres(dep_sys* s, int precond, double tol_rel, int max_iters, const double* x0,
                          double* sigma, dep_solve_info* info, const char* log_path)
 {
@@ -1439,11 +1513,11 @@ extern "C" int dep_gmres_ir(dep_sys* s, double tol_rel, double inner_tol, int ma
     }
     dep_blist one;
     one.n = 1;
-    auto matvec32 = [&](const double* in, const double* rhs, double* out) -> hipError_t {
-        if (!block) return launch_zmatvec32_w(s, gw, gwi, gsh, in, rhs, out);
-        hipError_t e = launch_bj_apply(c, N, s->bjB, one, in, 0, w, 0);
+    auto matvec32 = [&](const double* in, double* out, const int* stop) -> hipError_t {
+        if (!block) return launch_zmatvec32_w(s, gw, gwi, gsh, in, nullptr, out, stop);
+        hipError_t e = launch_bj_apply(c, N, s->bjB, one, in, 0, w, 0, 0, stop);
         if (e != hipSuccess) return e;
-        return launch_zmatvec32_w(s, s->wr, s->wi, s->shift, w, rhs, out);
+        return launch_zmatvec32_w(s, s->wr, s->wi, s->shift, w, nullptr, out, stop);
     };
     double bn = 0.0, res = 0.0;
     DEP_CHECK(dev_norm(s, s->b, &bn));
@@ -1461,10 +1535,8 @@ extern "C" int dep_gmres_ir(dep_sys* s, double tol_rel, double inner_tol, int ma
             DEP_HIP(c, launch_rowscale(c, N, dsc, r, rs));
             rhs = rs;
         }
-        double rsn = 0.0;
-        DEP_CHECK(dev_norm(s, rhs, &rsn));
         GmresRun run;
-        DEP_CHECK(gmres_run(s, matvec32, rhs, true, d, w, inner_tol * rsn, max_inner, nullptr, &run));
+        DEP_CHECK(gmres_run_dev(s, matvec32, rhs, d, inner_tol, max_inner, &run));
         inner_total += run.iters;
         if (run.iters == 0) stalled = true;
         if (block) {

@@ -95,7 +95,7 @@ __global__ __launch_bounds__(TR) void k_assemble_regular(
     const int8_t* __restrict__ kind, const double* __restrict__ normal,
     const double4* __restrict__ q, const int* __restrict__ patch_ptr,
     const int* __restrict__ patch_elem, const int8_t* __restrict__ patch_slot,
-    double* __restrict__ P, long ldP)
+    double* __restrict__ P, long ldP, float* __restrict__ P32)
 {
     constexpr ShapeTable<NPE> SF{};
     __shared__ double acc[TC][TR + 1];
@@ -154,13 +154,17 @@ __global__ __launch_bounds__(TR) void k_assemble_regular(
         }
     }
     __syncthreads();
-    // coalesced write-out of the TR x TC tile
+    // coalesced write-out of the TR x TC tile (and of its FP32 copy when P32 is kept)
     const int c0 = patch * TC;
     constexpr int RPP = TR / TC;   // rows per pass
     const int cc = t % TC, rr = t / TC;
     for (int r = rr; r < TR; r += RPP) {
         const int gi = blockIdx.y * TR + r, gc = c0 + cc;
-        if (gi < N && gc < N) P[(long)gi * ldP + gc] = acc[cc][r];
+        if (gi < N && gc < N) {
+            const double v = acc[cc][r];
+            P[(long)gi * ldP + gc] = v;
+            if (P32) P32[(long)gi * ldP + gc] = (float)v;
+        }
     }
 }
 
@@ -194,11 +198,13 @@ __global__ __launch_bounds__(64) void k_assemble_singular(int E, const double* _
     for (int m = 0; m < NPE; m++) S[(long)t * NPE + m] = out[m];
 }
 
-// one thread per row: add the singular blocks of the incident elements
+// one thread per row: add the singular blocks of the incident elements; with
+// P32 the touched entries of the FP32 copy are then rewritten from the final P
 template <int NPE>
 __global__ void k_scatter_singular(int N, int E, const int* __restrict__ conn,
                                    const int* __restrict__ inc_ptr, const int* __restrict__ inc_ek,
-                                   const double* __restrict__ S, double* __restrict__ P, long ldP)
+                                   const double* __restrict__ S, double* __restrict__ P, long ldP,
+                                   float* __restrict__ P32)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
@@ -208,6 +214,13 @@ __global__ void k_scatter_singular(int N, int E, const int* __restrict__ conn,
         const double* s = S + ((long)k * E + e) * NPE;
 #pragma unroll
         for (int m = 0; m < NPE; m++) Pi[conn[e * NPE + m]] += s[m];
+    }
+    if (!P32) return;
+    float* Fi = P32 + (long)i * ldP;
+    for (int t = inc_ptr[i]; t < inc_ptr[i + 1]; t++) {
+        const int e = inc_ek[t] >> 3;
+#pragma unroll
+        for (int m = 0; m < NPE; m++) Fi[conn[e * NPE + m]] = (float)Pi[conn[e * NPE + m]];
     }
 }
 
@@ -222,30 +235,30 @@ hipError_t assemble_impl(dep_geom* g)
     if (g->tc == 4)
         hipLaunchKernelGGL((k_assemble_regular<NPE, TR, 4>), grid, dim3(TR), 0, st, g->N, g->xyz, g->conn,
                            g->kind, g->normal, (const double4*)g->q, g->patch_ptr, g->patch_elem,
-                           g->patch_slot, g->P, g->ldP);
+                           g->patch_slot, g->P, g->ldP, g->P32);
     else if (g->tc == 8)
         hipLaunchKernelGGL((k_assemble_regular<NPE, TR, 8>), grid, dim3(TR), 0, st, g->N, g->xyz, g->conn,
                            g->kind, g->normal, (const double4*)g->q, g->patch_ptr, g->patch_elem,
-                           g->patch_slot, g->P, g->ldP);
+                           g->patch_slot, g->P, g->ldP, g->P32);
     else if (g->tc == 16)
         hipLaunchKernelGGL((k_assemble_regular<NPE, TR, 16>), grid, dim3(TR), 0, st, g->N, g->xyz, g->conn,
                            g->kind, g->normal, (const double4*)g->q, g->patch_ptr, g->patch_elem,
-                           g->patch_slot, g->P, g->ldP);
+                           g->patch_slot, g->P, g->ldP, g->P32);
     else if (g->tc == 32)
         hipLaunchKernelGGL((k_assemble_regular<NPE, TR, 32>), grid, dim3(TR), 0, st, g->N, g->xyz, g->conn,
                            g->kind, g->normal, (const double4*)g->q, g->patch_ptr, g->patch_elem,
-                           g->patch_slot, g->P, g->ldP);
+                           g->patch_slot, g->P, g->ldP, g->P32);
     else
         hipLaunchKernelGGL((k_assemble_regular<NPE, TR, 64>), grid, dim3(TR), 0, st, g->N, g->xyz, g->conn,
                            g->kind, g->normal, (const double4*)g->q, g->patch_ptr, g->patch_elem,
-                           g->patch_slot, g->P, g->ldP);
+                           g->patch_slot, g->P, g->ldP, g->P32);
     // algorithmic flops per (row, element) pair of the 7-point rule, sqrt and
     // division counted as one flop each: 7 (10 + 2 npe) for G rows, 7 (18 + 2 npe)
-    // for dG/dn rows; bytes: the N x N FP64 blocks written once
+    // for dG/dn rows; bytes: the N x N FP64 blocks (and their FP32 copy) written once
     long nI = 0;
     for (int8_t k : g->hkind) nI += (k != 0);
     const double flops = (double)g->E * (7.0 * (10 + 2 * NPE) * (g->N - nI) + 7.0 * (18 + 2 * NPE) * nI);
-    prof_end(g->ctx, ev, "assemble_regular", flops, 8.0 * g->N * (double)g->N);
+    prof_end(g->ctx, ev, "assemble_regular", flops, (g->P32 ? 12.0 : 8.0) * g->N * (double)g->N);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const int np = NPE * g->E;
@@ -254,7 +267,7 @@ hipError_t assemble_impl(dep_geom* g)
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_scatter_singular<NPE>), dim3(dep_cdiv(g->N, 128)), dim3(128), 0, st, g->N,
-                       g->E, g->conn, g->inc_ptr, g->inc_ek, g->S, g->P, g->ldP);
+                       g->E, g->conn, g->inc_ptr, g->inc_ek, g->S, g->P, g->ldP, g->P32);
     return hipGetLastError();
 }
 

@@ -19,8 +19,9 @@ namespace {
 #define DOT_CHUNKS DEP_DOT_CHUNKS
 __global__ __launch_bounds__(256) void k_dots_partial(long n2, const double* __restrict__ V,
                                                       const double* __restrict__ w,
-                                                      double* __restrict__ part)
+                                                      double* __restrict__ part, const int* __restrict__ stop)
 {
+    if (stop && *stop) return;
     __shared__ double sh[4];
     const int c = blockIdx.x, j = blockIdx.y;
     const long len = (n2 + DOT_CHUNKS - 1) / DOT_CHUNKS;
@@ -51,8 +52,10 @@ __global__ void k_dots_reduce(int nvec, const double* __restrict__ part, double*
 #define UPD_MAXV 1024
 __global__ __launch_bounds__(256) void k_update_fused(int nv, long n2, const double* __restrict__ V,
                                                       const double* __restrict__ part, double* __restrict__ w,
-                                                      double* __restrict__ hout, double* __restrict__ normp)
+                                                      double* __restrict__ hout, double* __restrict__ normp,
+                                                      const int* __restrict__ stop)
 {
+    if (stop && *stop) return;
     __shared__ double h[UPD_MAXV + 1];
     __shared__ double red[4];
     for (int j = threadIdx.x; j <= nv; j += 256) {
@@ -129,6 +132,86 @@ __global__ void k_combine(int nvec, long n2, const double* __restrict__ V, const
     x[t] = s;
 }
 
+// ---- GMRES with the Givens recurrence on the device ------------------------
+// State of one run (GmresDev): H column i at H + i (cap + 2), rotations cs/sn,
+// residual vector g, per-step residual estimates err, scalars sc[0] = 1/|w|
+// (0: no scaling), sc[1] = the previous estimate, sc[2] = the tolerance; flags
+// [0] stop, [1] steps done, [2] rc (0 converged, 1 limit, 2 stagnation).
+
+// beta = sqrt(|V0|^2) (h[0]); g = (beta, 0, ...), V0 scale 1/beta, tolerance tol_rel * beta
+__global__ void k_gmres_init(const double* __restrict__ h, double tol_rel, double* __restrict__ g,
+                             double* __restrict__ sc, int* __restrict__ flags)
+{
+    if (threadIdx.x != 0) return;
+    const double beta = sqrt(h[0]);
+    g[0] = beta;
+    sc[0] = beta > 0.0 ? 1.0 / beta : 0.0;
+    sc[1] = beta;
+    sc[2] = tol_rel * beta;
+    flags[0] = beta < tol_rel * beta || !(beta > 0.0) ? 1 : 0;
+    flags[1] = 0;
+    flags[2] = flags[0] ? 0 : 1;
+}
+
+// Step i: the Hessenberg column from the two CGS passes (h1 + h2, |w| from the
+// norm partials), the previous rotations, the new rotation and the residual
+// estimate; the arithmetic and its order are those of gmres_el.c:196-241 as
+// restated on the host in gmres_run.
+__global__ void k_gmres_givens(int i, int cap, const double* __restrict__ h1, const double* __restrict__ h2,
+                               const double* __restrict__ np, double* __restrict__ H, double* __restrict__ cs,
+                               double* __restrict__ sn, double* __restrict__ g, double* __restrict__ err,
+                               double* __restrict__ sc, int* __restrict__ flags)
+{
+    if (flags[0]) return;
+    double* col = H + (long)i * (cap + 2);
+    for (int j = threadIdx.x; j <= i; j += blockDim.x) col[j] = h1[j] + h2[j];
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const double wn = sqrt(h1[i + 1]);
+    double hn2 = 0.0;
+    for (int k = 0; k < DEP_UPD_BLOCKS; k++) hn2 += np[k];
+    const double hn = sqrt(hn2);
+    col[i + 1] = hn;
+    sc[0] = hn != 0.0 ? 1.0 / hn : 0.0;
+    for (int j = 0; j <= i - 1; j++) {
+        const double g1 = cs[j] * col[j] + sn[j] * col[j + 1];
+        const double g2 = -sn[j] * col[j] + cs[j] * col[j + 1];
+        col[j] = g1;
+        col[j + 1] = g2;
+    }
+    const double htmp = col[i + 1] / col[i];
+    if (wn != 0.0) {
+        cs[i] = 1.0 / sqrt(1.0 + htmp * htmp);
+        sn[i] = cs[i] * htmp;
+        col[i] = cs[i] * col[i] + sn[i] * col[i + 1];
+        col[i + 1] = 0.0;
+        g[i + 1] = -sn[i] * g[i];
+        g[i] = cs[i] * g[i];
+    }
+    const double err1 = fabs(g[i + 1]), err0 = sc[1], tol = sc[2];
+    err[i] = err1;
+    flags[1] = i + 1;
+    if (err1 < tol) {
+        flags[0] = 1;
+        flags[2] = 0;
+    } else if (fabs((err1 - err0) / err0) < 1.0e-6 * tol) {
+        flags[0] = 1;
+        flags[2] = 2;
+    } else {
+        sc[1] = err1;
+    }
+}
+
+// w *= sc[0] unless stopped or sc[0] == 0
+__global__ void k_scale_dev(long n2, double* __restrict__ w, const double* __restrict__ sc,
+                            const int* __restrict__ stop)
+{
+    if (stop && *stop) return;
+    const double a = sc[0];
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n2 && a != 0.0) w[t] *= a;
+}
+
 // y = Z x for a dense complex Z (split planes), one wave per row; with b: y = b - Z x
 __global__ __launch_bounds__(256) void k_dense_zmatvec(int N, const double* __restrict__ Zr,
                                                        const double* __restrict__ Zi, long ldZ,
@@ -167,7 +250,8 @@ __global__ __launch_bounds__(256) void k_dense_zmatvec(int N, const double* __re
 hipError_t launch_dots(dep_ctx* c, int nvec, long n2, const double* V, const double* w, double* h,
                        double* part)
 {
-    hipLaunchKernelGGL(k_dots_partial, dim3(DOT_CHUNKS, nvec), dim3(256), 0, c->stream, n2, V, w, part);
+    hipLaunchKernelGGL(k_dots_partial, dim3(DOT_CHUNKS, nvec), dim3(256), 0, c->stream, n2, V, w, part,
+                       (const int*)nullptr);
     hipLaunchKernelGGL(k_dots_reduce, dim3(dep_cdiv(nvec, 256)), dim3(256), 0, c->stream, nvec, part, h);
     return hipGetLastError();
 }
@@ -190,18 +274,19 @@ hipError_t launch_rowscale(dep_ctx* c, int N, const double* d, const double* in,
     return hipGetLastError();
 }
 
-hipError_t launch_dot_partials(dep_ctx* c, int nvec, long n2, const double* V, const double* w, double* part)
+hipError_t launch_dot_partials(dep_ctx* c, int nvec, long n2, const double* V, const double* w, double* part,
+                               const int* stop)
 {
-    hipLaunchKernelGGL(k_dots_partial, dim3(DOT_CHUNKS, nvec), dim3(256), 0, c->stream, n2, V, w, part);
+    hipLaunchKernelGGL(k_dots_partial, dim3(DOT_CHUNKS, nvec), dim3(256), 0, c->stream, n2, V, w, part, stop);
     return hipGetLastError();
 }
 
 hipError_t launch_update_fused(dep_ctx* c, int nv, long n2, const double* V, const double* part, double* w,
-                               double* hout, double* normp)
+                               double* hout, double* normp, const int* stop)
 {
     if (nv > UPD_MAXV) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_update_fused, dim3(DEP_UPD_BLOCKS), dim3(256), 0, c->stream, nv, n2, V, part, w, hout,
-                       normp);
+                       normp, stop);
     return hipGetLastError();
 }
 
@@ -228,5 +313,25 @@ hipError_t launch_dense_zmatvec(dep_ctx* c, int N, const double* Zr, const doubl
 {
     hipLaunchKernelGGL(k_dense_zmatvec, dim3(dep_cdiv(N, 4)), dim3(256), 0, c->stream, N, Zr, Zi, ldZ, x,
                        b, y);
+    return hipGetLastError();
+}
+
+hipError_t launch_gmres_init(dep_ctx* c, const double* h, double tol_rel, double* g, double* sc, int* flags)
+{
+    hipLaunchKernelGGL(k_gmres_init, dim3(1), dim3(64), 0, c->stream, h, tol_rel, g, sc, flags);
+    return hipGetLastError();
+}
+
+hipError_t launch_gmres_givens(dep_ctx* c, int i, int cap, const double* h1, const double* h2, const double* np,
+                               double* H, double* cs, double* sn, double* g, double* err, double* sc, int* flags)
+{
+    hipLaunchKernelGGL(k_gmres_givens, dim3(1), dim3(64), 0, c->stream, i, cap, h1, h2, np, H, cs, sn, g, err, sc,
+                       flags);
+    return hipGetLastError();
+}
+
+hipError_t launch_scale_dev(dep_ctx* c, long n2, double* w, const double* sc, const int* stop)
+{
+    hipLaunchKernelGGL(k_scale_dev, dim3((unsigned)dep_cdiv(n2, 256)), dim3(256), 0, c->stream, n2, w, sc, stop);
     return hipGetLastError();
 }

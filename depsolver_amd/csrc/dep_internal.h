@@ -133,6 +133,8 @@ struct dep_sys {
     // reductions and dot-product partials
     int gm_cap = 0;
     double *gm_vec = nullptr, *gm_V = nullptr, *gm_red = nullptr;
+    double* gm_dev = nullptr;         // device Givens state: H, cs, sn, g, err, scalars
+    int* gm_flags = nullptr;          // stop, steps, rc
     // block-Jacobi preconditioner of the refined FP32 GMRES (valid for bj_asm / bj_freq)
     float* bjB = nullptr;
     double* bj_scr = nullptr;
@@ -194,14 +196,21 @@ hipError_t launch_to_f32(dep_ctx* c, long n, const double* in, float* out);
 hipError_t launch_rowscale(dep_ctx* c, int N, const double* d, const double* in, double* out);
 // y = Z x (b == nullptr) or b - Z x with the FP32 blocks P32 and the given weights
 hipError_t launch_zmatvec32_w(dep_sys* s, const double* wr, const double* wi, const double* shift,
-                              const double* x, const double* b, double* y);
+                              const double* x, const double* b, double* y, const int* stop = nullptr);
 hipError_t launch_update(dep_ctx* c, int nvec, long n2, const double* V, const double* h, double* w);
 // partials only (DEP_DOT_CHUNKS per vector), consumed by launch_update_fused
-hipError_t launch_dot_partials(dep_ctx* c, int nvec, long n2, const double* V, const double* w, double* part);
+hipError_t launch_dot_partials(dep_ctx* c, int nvec, long n2, const double* V, const double* w, double* part,
+                               const int* stop = nullptr);
 // CGS pass from dot partials (see dep_gmres.hip); DEP_UPD_BLOCKS workgroups, nv <= 1024
 #define DEP_UPD_BLOCKS 128
 hipError_t launch_update_fused(dep_ctx* c, int nv, long n2, const double* V, const double* part, double* w,
-                               double* hout, double* normp);
+                               double* hout, double* normp, const int* stop = nullptr);
+// GMRES with the Givens recurrence on the device (dep_gmres.hip): the kernels of a
+// step are skipped once flags[0] (stop) is set
+hipError_t launch_gmres_init(dep_ctx* c, const double* h, double tol_rel, double* g, double* sc, int* flags);
+hipError_t launch_gmres_givens(dep_ctx* c, int i, int cap, const double* h1, const double* h2, const double* np,
+                               double* H, double* cs, double* sn, double* g, double* err, double* sc, int* flags);
+hipError_t launch_scale_dev(dep_ctx* c, long n2, double* w, const double* sc, const int* stop);
 hipError_t launch_scale(dep_ctx* c, long n2, const double* w, double alpha, double* out);
 hipError_t launch_combine(dep_ctx* c, int nvec, long n2, const double* V, const double* y, double* x);
 size_t mixed_bytes(int N);
@@ -260,4 +269,4 @@ hipError_t launch_bj_build(dep_ctx* c, int N, const double* P, long ldP, const d
                            const double* sh, long wstride, int nsys, double* scratch, float* B);
 // Y_j = M_j^-1 X_j (accumulate: Y_j += M_j^-1 X_j) for the systems of L
 hipError_t launch_bj_apply(dep_ctx* c, int N, const float* B, const dep_blist& L, const double* X, long xstride,
-                           double* Y, long ystride, int accumulate = 0);
+                           double* Y, long ystride, int accumulate = 0, const int* stop = nullptr);

@@ -85,8 +85,9 @@ __global__ __launch_bounds__(256) void k_bj_combine(int nblk, const double* __re
 // partial sums of a row are combined by lane shuffles.
 __global__ __launch_bounds__(256) void k_bj_apply(int N, int nblk, const float* __restrict__ B, dep_blist L,
                                                   const double* __restrict__ X, long xstride, double* __restrict__ Y,
-                                                  long ystride, int accumulate)
+                                                  long ystride, int accumulate, const int* __restrict__ stop)
 {
+    if (stop && *stop) return;
     __shared__ double xr[BJ], xi[BJ];
     const int b = blockIdx.x;
     const long sys = L.id[blockIdx.y];
@@ -147,10 +148,10 @@ hipError_t launch_bj_build(dep_ctx* c, int N, const double* P, long ldP, const d
 }
 
 hipError_t launch_bj_apply(dep_ctx* c, int N, const float* B, const dep_blist& L, const double* X, long xstride,
-                           double* Y, long ystride, int accumulate)
+                           double* Y, long ystride, int accumulate, const int* stop)
 {
     const int nblk = (int)bj_blocks(N);
     hipLaunchKernelGGL(k_bj_apply, dim3(nblk, L.n), dim3(256), 0, c->stream, N, nblk, B, L, X, xstride, Y, ystride,
-                       accumulate);
+                       accumulate, stop);
     return hipGetLastError();
 }

@@ -25,46 +25,60 @@ __global__ void k_weight(int N, const double* __restrict__ P, long ldP, const do
 }
 
 // ---- products with P -------------------------------------------------------
-// y = Z x with Z = w .* P + diag(shift); x, y split (re block, im block) of length N.
-// One wave per row, FP64.
 // y = Z x (or b - Z x) with Z = diag(w) P + diag(shift), P real (FP64 or FP32
-// storage, FP64 arithmetic): each wave takes ZROWS rows and walks their columns
-// in chunks of 256 (4 consecutive columns per lane, vector loads), so every
-// chunk of x is read once per ZROWS rows; a scalar loop finishes the columns
-// past the last full chunk.  ZROWS = 4 (8 rows per wave measured slower for the
-// FP32 blocks: fewer waves in flight).
-template <class TP, int ZROWS>
-__global__ __launch_bounds__(256) void k_zmatvec(int N, const TP* __restrict__ P, long ldP,
-                                                 const double* __restrict__ wr,
-                                                 const double* __restrict__ wi,
-                                                 const double* __restrict__ shift,
-                                                 const double* __restrict__ x,
-                                                 const double* __restrict__ b,
-                                                 double* __restrict__ y)
+// storage, FP64 arithmetic).  Each wave (WPB per workgroup) owns ZROWS rows
+// and walks their columns in chunks of 256 (4 consecutive columns per lane,
+// vector loads), UNR chunks in flight, so every chunk of x is read once per
+// ZROWS rows; with NT, P is streamed with non-temporal loads (it is read once
+// per product and would only evict x); a scalar loop finishes the columns past
+// the last full group of chunks.  Shapes measured at N = 4,740 / 9,480 /
+// 15,044: FP32 blocks 6 rows, 2 chunks, one wave, NT (23 / 67 / 148 us, 6.1
+// TB/s at the largest; 4 rows, 1 chunk, 4 waves: 28 / 87 / 192 us); FP64
+// blocks 4 rows, 1 chunk, 4 waves, cached loads (0.40 ms, 4.6 TB/s; one-wave
+// shapes 0.44-0.77 ms).
+template <class TP, int ZROWS, int UNR, int WPB, bool NT>
+__global__ __launch_bounds__(64 * WPB) void k_zmatvec(int N, const TP* __restrict__ P, long ldP,
+                                                const double* __restrict__ wr,
+                                                const double* __restrict__ wi,
+                                                const double* __restrict__ shift,
+                                                const double* __restrict__ x,
+                                                const double* __restrict__ b,
+                                                double* __restrict__ y, const int* __restrict__ stop)
 {
+    if (stop && *stop) return;   // a step past the end of a device-driven GMRES run
     typedef TP p4 __attribute__((ext_vector_type(4)));
     typedef double d4 __attribute__((ext_vector_type(4)));
     const int lane = threadIdx.x & 63;
-    const int row0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * ZROWS;
+    const int row0 = (blockIdx.x * WPB + (threadIdx.x >> 6)) * ZROWS;
     if (row0 >= N) return;
     const int nr = min(ZROWS, N - row0);
     double sr[ZROWS], si[ZROWS];
 #pragma unroll
     for (int k = 0; k < ZROWS; k++) sr[k] = si[k] = 0.0;
-    const int full = N / 256 * 256;
-    for (int c = 4 * lane; c < full; c += 256) {
-        const d4 xr = *(const d4*)(x + c), xi = *(const d4*)(x + N + c);
+    const int full = N / (256 * UNR) * (256 * UNR);
+    for (int c0 = 4 * lane; c0 < full; c0 += 256 * UNR) {
+        p4 v[UNR][ZROWS];
+        d4 xr[UNR], xi[UNR];
 #pragma unroll
-        for (int k = 0; k < ZROWS; k++) {
-            if (k < nr) {
-                const p4 v = *(const p4*)(P + (long)(row0 + k) * ldP + c);
+        for (int u = 0; u < UNR; u++) {
+            const int c = c0 + 256 * u;
+            xr[u] = *(const d4*)(x + c);
+            xi[u] = *(const d4*)(x + N + c);
 #pragma unroll
-                for (int e = 0; e < 4; e++) {
-                    sr[k] += (double)v[e] * xr[e];
-                    si[k] += (double)v[e] * xi[e];
-                }
+            for (int k = 0; k < ZROWS; k++) {   // rows past N re-read the last row (never stored)
+                const p4* src = (const p4*)(P + (long)(row0 + min(k, nr - 1)) * ldP + c);
+                v[u][k] = NT ? __builtin_nontemporal_load(src) : *src;
             }
         }
+#pragma unroll
+        for (int u = 0; u < UNR; u++)
+#pragma unroll
+            for (int k = 0; k < ZROWS; k++)
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    sr[k] += (double)v[u][k][e] * xr[u][e];
+                    si[k] += (double)v[u][k][e] * xi[u][e];
+                }
     }
     for (int c = full + lane; c < N; c += 64) {
         const double xr = x[c], xi = x[N + c];
@@ -126,21 +140,21 @@ hipError_t launch_zmatvec_w(dep_sys* s, const double* wr, const double* wi, cons
     dep_geom* g = s->g;
     hipEvent_t ev;
     prof_begin(g->ctx, &ev);
-    hipLaunchKernelGGL((k_zmatvec<double, 4>), dim3(dep_cdiv(g->N, 16)), dim3(256), 0, g->ctx->stream, g->N,
-                       g->P, g->ldP, wr, wi, shift, x, b, y);
+    hipLaunchKernelGGL((k_zmatvec<double, 4, 1, 4, false>), dim3(dep_cdiv(g->N, 16)), dim3(256), 0, g->ctx->stream,
+                       g->N, g->P, g->ldP, wr, wi, shift, x, b, y, (const int*)nullptr);
     const double n = g->N;
     prof_end(g->ctx, ev, "zmatvec", 4.0 * n * n, 8.0 * n * n + 48.0 * n);
     return hipGetLastError();
 }
 
 hipError_t launch_zmatvec32_w(dep_sys* s, const double* wr, const double* wi, const double* shift,
-                              const double* x, const double* b, double* y)
+                              const double* x, const double* b, double* y, const int* stop)
 {
     dep_geom* g = s->g;
     hipEvent_t ev;
     prof_begin(g->ctx, &ev);
-    hipLaunchKernelGGL((k_zmatvec<float, 4>), dim3(dep_cdiv(g->N, 16)), dim3(256), 0, g->ctx->stream, g->N,
-                       g->P32, g->ldP, wr, wi, shift, x, b, y);
+    hipLaunchKernelGGL((k_zmatvec<float, 6, 2, 1, true>), dim3(dep_cdiv(g->N, 6)), dim3(64), 0, g->ctx->stream, g->N,
+                       g->P32, g->ldP, wr, wi, shift, x, b, y, stop);
     const double n = g->N;
     prof_end(g->ctx, ev, "zmatvec32", 4.0 * n * n, 4.0 * n * n + 48.0 * n);
     return hipGetLastError();

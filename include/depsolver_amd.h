@@ -59,7 +59,7 @@ typedef struct dep_sweep dep_sweep;
 typedef struct dep_solve_info {
     int iterations;        /* GMRES iterations, or refinement steps */
     int inner_iterations;  /* GMRES iterations inside GMRES-IR */
-    int fallback;          /* 0 none, 1 GMRES-IR used, 2 FP64 LU used */
+    int fallback;          /* 0 none, 2 the FP64 LU answered (mixed refinement stalled) */
     int converged;         /* 1 if the FP64 residual reached the tolerance */
     double residual;       /* ||b - Z x|| / ||b|| in FP64 */
     double min_pivot;      /* smallest |U_ii| seen by the factorization */
@@ -116,8 +116,9 @@ int dep_sys_get_matrix(dep_sys *s, int layout, double *mA);
 /* ---- dense solve: replaces gaussBksb (src/gaussBksb.c:39-108) ----------- */
 int dep_factor(dep_sys *s, int precision);
 /* FP64 iterative refinement on the factor to ||b - Zx|| <= tol ||b||
- * (a single triangular solve for DEP_PREC_FP64), GMRES-IR and then the FP64
- * LU as fallbacks.  sigma (2N) may be NULL. */
+ * (a single triangular solve for DEP_PREC_FP64); if a mixed factor's
+ * refinement stalls, the FP64 LU answers (info.fallback = 2).  sigma (2N)
+ * may be NULL. */
 int dep_solve(dep_sys *s, double tol, int max_steps, double *sigma, dep_solve_info *info);
 
 /* ---- GMRES: replaces solverGMRES_el (src/gmres_el.c:66-128) ------------- */
