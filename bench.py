@@ -102,6 +102,19 @@ def roofline(stats):
             "algorithmic_per_launch": (st["bytes"] if bound == "hbm" else st["flops"]) / st["launches"]}
 
 
+def kernel_rooflines(stats, path, out):
+    """One entry per profiled kernel: achieved rate of its algorithmic work against the MI355X peak
+    (FLOP-bound kernels also report their algorithmic bytes as GB/s)."""
+    for name, st in stats.items():
+        kind = KERNEL_PEAK[name]
+        peak, unit, bound = PEAKS[kind]
+        sec = st["ms"] / 1e3
+        rate = (st["bytes"] / sec / 1e9) if bound == "hbm" else (st["flops"] / sec / 1e12)
+        out[f"{name} ({path})"] = {"kernel": name, "path": path, "bound": bound, "achieved": rate, "peak": peak, "unit": unit,
+                     "frac": rate / peak, "launches": st["launches"], "avg_launch_ms": st["ms"] / st["launches"],
+                     "algorithmic_GBps": st["bytes"] / sec / 1e9}
+
+
 def cpu_baseline(p, s, rows=48, pivots=6, points=24):
     """Time the oracle (single core) on a bounded sample of the same solve and
     extrapolate to one full solve: assembly rows, the first LU pivots of the
@@ -177,6 +190,8 @@ def sweep_bench(dep, ctx, steps, nf=64):
     ctx.synchronize()
     stats = kernel_stats(dep, ctx, ["bgemm32", "bgemm64"])
     dep.lib().dep_ctx_set_profiling(ctx._h, 0)
+    rl = {}
+    kernel_rooflines(stats, "sweep", rl)
     s = dep.System(g)
     s.set_frequency(freqs[0])
     s.gmres_ir(tol=1e-12)
@@ -207,7 +222,7 @@ def sweep_bench(dep, ctx, steps, nf=64):
                                  int(max(x["inner_iterations"] for x in info))],
             "errors_vs_fp64_lu": {"sigma_rel": err_s, "force_total_rel": err_f},
             "kernels": {k: {"launches": v["launches"], "avg_ms": v["ms"] / v["launches"]} for k, v in stats.items()},
-            "roofline": roofline(stats)}
+            "roofline": roofline(stats), "kernel_rooflines": rl}
 
 
 def serve_bench(dep, n_req, slots, solvers=("gmres-ir-fp32", "lu-fp16")):
@@ -332,13 +347,24 @@ def main():
         Fce, Xce, Fcm, Xcm = s.mst()
         return time.perf_counter() - t0, sigma, info, Fce, Fcm
 
+    rooflines = {}
+
+    def profiled(on):
+        ctx.synchronize()
+        if on:
+            dep.lib().dep_ctx_reset_stats(ctx._h)
+        dep.lib().dep_ctx_set_profiling(ctx._h, 1 if on else 0)
+
     # FP64 LU: the accuracy yardstick for every other path
     step("lu-fp64")
     lat64 = []
+    profiled(True)
     for k in range(max(1, args.side_steps)):
         dt, sig64, info64, Fce64, Fcm64 = step("lu-fp64")
         lat64.append(dt)
         log(f"bench: lu-fp64 step {k} {dt * 1e3:.1f} ms residual {info64['residual']:.2e}")
+    profiled(False)
+    kernel_rooflines(kernel_stats(dep, ctx, ["lu_gemm_fp64"]), "lu-fp64", rooflines)
     fscale = np.abs(Fce64).max()
 
     def errors(sig, Fce, Fcm):
@@ -353,11 +379,15 @@ def main():
             continue
         step(name)   # first call: workspace allocation, factor storage
         lat = []
+        profiled(True)
         for k in range(args.side_steps):
             dt, sig, info, Fce, Fcm = step(name)
             lat.append(dt)
             log(f"bench: {name} step {k} {dt * 1e3:.1f} ms residual {info['residual']:.2e} "
                 f"iterations {info['iterations']}")
+        profiled(False)
+        kernel_rooflines(kernel_stats(dep, ctx, ["lu_gemm_mixed", "zmatvec"] if name != "gmres-ir-fp32" else []),
+                         name, rooflines)
         paths[name] = dict(errors(sig, Fce, Fcm), p50_ms=float(np.median(lat) * 1e3),
                            solves_per_sec=1.0 / float(np.median(lat)), iterations=info["iterations"],
                            inner_iterations=info["inner_iterations"], residual=info["residual"])
@@ -380,6 +410,7 @@ def main():
     elapsed = time.perf_counter() - t0
     stats = kernel_stats(dep, ctx, list(KERNEL_PEAK))
     dep.lib().dep_ctx_set_profiling(ctx._h, 0)
+    kernel_rooflines(stats, args.solver, rooflines)
     ms = elapsed / args.steps * 1e3
     lat_ms = np.array(lat) * 1e3
     paths[args.solver] = dict(errors(sig, Fce, Fcm), p50_ms=float(np.percentile(lat_ms, 50)),
@@ -419,6 +450,7 @@ def main():
         "paths": paths,
         "kernels": {k: {"launches": v["launches"], "avg_ms": v["ms"] / v["launches"]} for k, v in stats.items()},
         "roofline": roofline(stats),
+        "kernel_rooflines": rooflines,
         "cpu_baseline": cpu,
         "sweep": sweep,
         "serving": serving,

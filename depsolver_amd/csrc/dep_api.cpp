@@ -364,9 +364,9 @@ extern "C" int dep_geom_create(dep_ctx* c, int N, const double* xyz, int E, int 
             std::sort(els.begin(), els.end());
             for (int e : els) {
                 pel.push_back(e);
-                for (int k = 0; k < npe; k++) {
-                    const int v = g->hconn[(size_t)e * npe + k];
-                    pslot.push_back((v >= c0 && v < c1) ? (int8_t)(v - c0) : (int8_t)-1);
+                for (int k = 0; k < 8; k++) {   // 8 bytes per element: one aligned load in the kernel
+                    const int v = k < npe ? g->hconn[(size_t)e * npe + k] : -1;
+                    pslot.push_back((k < npe && v >= c0 && v < c1) ? (int8_t)(v - c0) : (int8_t)-1);
                 }
             }
             pptr[p + 1] = (int)pel.size();
@@ -1526,9 +1526,14 @@ extern "C" int dep_gmres_ir(dep_sys* s, double tol_rel, double inner_tol, int ma
     int outer = 0, inner_total = 0;
     bool stalled = false;
     for (;;) {
-        DEP_HIP(c, launch_residual(s, s->x, r));
-        DEP_CHECK(dev_norm(s, r, &res));
-        res /= bn;
+        if (outer == 0) {   // x = 0: the residual is b
+            DEP_HIP(c, hipMemcpyAsync(r, s->b, sizeof(double) * n2, hipMemcpyDeviceToDevice, c->stream));
+            res = 1.0;
+        } else {
+            DEP_HIP(c, launch_residual(s, s->x, r));
+            DEP_CHECK(dev_norm(s, r, &res));
+            res /= bn;
+        }
         if (!(res > tol_rel) || outer >= max_outer || stalled) break;
         const double* rhs = r;
         if (!block) {

@@ -122,6 +122,8 @@ __global__ __launch_bounds__(TR) void k_assemble_regular(
     const int j0 = patch_ptr[patch], j1 = patch_ptr[patch + 1];
     for (int j = j0; j < j1; j++) {
         const int e = patch_elem[j];
+        // the element's column slots, one wave-uniform 8-byte load (byte loads would each be waited for)
+        const uint64_t slots = *(const uint64_t*)(patch_slot + (long)j * 8);
         bool sing = false;
 #pragma unroll
         for (int k = 0; k < NPE; k++) sing |= (conn[e * NPE + k] == row);
@@ -149,7 +151,7 @@ __global__ __launch_bounds__(TR) void k_assemble_regular(
         }
 #pragma unroll
         for (int k = 0; k < NPE; k++) {
-            const int s = patch_slot[j * NPE + k];
+            const int s = (int8_t)(slots >> (8 * k));
             if (s >= 0) acc[s][t] += sub[k];
         }
     }

@@ -82,11 +82,27 @@ __device__ void diag_factor_invert(int nb, T* __restrict__ Zr, T* __restrict__ Z
     const int t = threadIdx.x, r = t & 63;
     const int w = __builtin_amdgcn_readfirstlane(t >> 6);   // wave index (uniform): columns 4w .. 4w+3
 
-    for (int idx = t; idx < DIAG_NB * DIAG_NB; idx += DIAG_THREADS) {
-        const int rr = idx >> 6, cc = idx & 63;
-        const bool in = rr < nb && cc < nb;
-        Sr[rr][cc] = in ? Zr[(long)(k0 + rr) * ldZ + k0 + cc] : T(rr == cc ? 1 : 0);
-        Si[rr][cc] = in ? Zi[(long)(k0 + rr) * ldZ + k0 + cc] : T(0);
+    if (nb == DIAG_NB) {   // full block: the loads of a thread are all in flight before the LDS stores
+        T vr[DIAG_NB * DIAG_NB / DIAG_THREADS], vi[DIAG_NB * DIAG_NB / DIAG_THREADS];
+#pragma unroll
+        for (int u = 0; u < DIAG_NB * DIAG_NB / DIAG_THREADS; u++) {
+            const int idx = t + DIAG_THREADS * u;
+            vr[u] = Zr[(long)(k0 + (idx >> 6)) * ldZ + k0 + (idx & 63)];
+            vi[u] = Zi[(long)(k0 + (idx >> 6)) * ldZ + k0 + (idx & 63)];
+        }
+#pragma unroll
+        for (int u = 0; u < DIAG_NB * DIAG_NB / DIAG_THREADS; u++) {
+            const int idx = t + DIAG_THREADS * u;
+            Sr[idx >> 6][idx & 63] = vr[u];
+            Si[idx >> 6][idx & 63] = vi[u];
+        }
+    } else {
+        for (int idx = t; idx < DIAG_NB * DIAG_NB; idx += DIAG_THREADS) {
+            const int rr = idx >> 6, cc = idx & 63;
+            const bool in = rr < nb && cc < nb;
+            Sr[rr][cc] = in ? Zr[(long)(k0 + rr) * ldZ + k0 + cc] : T(rr == cc ? 1 : 0);
+            Si[rr][cc] = in ? Zi[(long)(k0 + rr) * ldZ + k0 + cc] : T(0);
+        }
     }
     __syncthreads();
     T dr[4], di[4];

@@ -83,7 +83,7 @@ struct dep_geom {
     int* inc_ek = nullptr;            // packed e*8 + k
     int* patch_ptr = nullptr;         // column patch -> elements touching it
     int* patch_elem = nullptr;
-    int8_t* patch_slot = nullptr;     // per patch element: npe column slots (-1 outside)
+    int8_t* patch_slot = nullptr;     // per patch element: 8 bytes, npe column slots (-1 outside / padding)
     int n_patch = 0;
     double* S = nullptr;              // npe*E*npe singular integrals
     double* P = nullptr;

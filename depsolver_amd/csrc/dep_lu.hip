@@ -507,16 +507,38 @@ __global__ __launch_bounds__(256) void k_apply_inv(int r0, int e, int nb, T* __r
     for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const int p0 = r0 + 64 * tile;   // first row (RIGHT) or column (LEFT) of the tile
         __syncthreads();                 // previous tile's reads of the panel operand are done
-        for (int idx = t; idx < NBI * NBI; idx += 256) {
-            const int r = idx >> 6, c = idx & 63;
-            if (RIGHT) {   // X = W[p0 + r, k0 + c]
-                const bool ok = p0 + r < e && c < nb;
-                Xr[r][c] = ok ? Zr[(long)(p0 + r) * ldZ + k0 + c] : T(0);
-                Xi[r][c] = ok ? Zi[(long)(p0 + r) * ldZ + k0 + c] : T(0);
-            } else {       // Y = W[k0 + r, p0 + c]
-                const bool ok = r < nb && p0 + c < e;
-                Yr[r][c] = ok ? Zr[(long)(k0 + r) * ldZ + p0 + c] : T(0);
-                Yi[r][c] = ok ? Zi[(long)(k0 + r) * ldZ + p0 + c] : T(0);
+        if (nb == NBI && p0 + NBI <= e) {   // full tile: all 32 loads in flight before the LDS stores
+            T vr[NBI * NBI / 256], vi[NBI * NBI / 256];
+#pragma unroll
+            for (int u = 0; u < NBI * NBI / 256; u++) {
+                const int idx = t + 256 * u, r = idx >> 6, c = idx & 63;
+                const long o = RIGHT ? (long)(p0 + r) * ldZ + k0 + c : (long)(k0 + r) * ldZ + p0 + c;
+                vr[u] = Zr[o];
+                vi[u] = Zi[o];
+            }
+#pragma unroll
+            for (int u = 0; u < NBI * NBI / 256; u++) {
+                const int idx = t + 256 * u, r = idx >> 6, c = idx & 63;
+                if (RIGHT) {
+                    Xr[r][c] = vr[u];
+                    Xi[r][c] = vi[u];
+                } else {
+                    Yr[r][c] = vr[u];
+                    Yi[r][c] = vi[u];
+                }
+            }
+        } else {
+            for (int idx = t; idx < NBI * NBI; idx += 256) {
+                const int r = idx >> 6, c = idx & 63;
+                if (RIGHT) {   // X = W[p0 + r, k0 + c]
+                    const bool ok = p0 + r < e && c < nb;
+                    Xr[r][c] = ok ? Zr[(long)(p0 + r) * ldZ + k0 + c] : T(0);
+                    Xi[r][c] = ok ? Zi[(long)(p0 + r) * ldZ + k0 + c] : T(0);
+                } else {       // Y = W[k0 + r, p0 + c]
+                    const bool ok = r < nb && p0 + c < e;
+                    Yr[r][c] = ok ? Zr[(long)(k0 + r) * ldZ + p0 + c] : T(0);
+                    Yi[r][c] = ok ? Zi[(long)(k0 + r) * ldZ + p0 + c] : T(0);
+                }
             }
         }
         __syncthreads();

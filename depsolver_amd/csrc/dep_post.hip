@@ -66,13 +66,16 @@ __global__ __launch_bounds__(256) void k_points_regular(
     double acc[NO];
 #pragma unroll
     for (int o = 0; o < NO; o++) acc[o] = 0.0;
+    // the next excluded element of this point, kept in a register: it is reloaded only when
+    // the walk reaches it (a load per element would be waited for at every element)
+    int nextx = xp < xend ? excl_e[xp] : 0x7fffffff;
     for (int e = e0; e < e1; e++) {
-        bool skip = false;
-        if (xp < xend && excl_e[xp] == e) {
-            skip = true;
+        if (e == nextx) {
             xp++;
+            nextx = xp < xend ? excl_e[xp] : 0x7fffffff;
+            continue;
         }
-        if (skip || p >= M) continue;
+        if (p >= M) continue;
 #pragma unroll
         for (int g = 0; g < 7; g++) {
             const double4 y = q[e * 7 + g];

@@ -33,9 +33,8 @@ __global__ void k_weight(int N, const double* __restrict__ P, long ldP, const do
 // per product and would only evict x); a scalar loop finishes the columns past
 // the last full group of chunks.  Shapes measured at N = 4,740 / 9,480 /
 // 15,044: FP32 blocks 6 rows, 2 chunks, one wave, NT (23 / 67 / 148 us, 6.1
-// TB/s at the largest; 4 rows, 1 chunk, 4 waves: 28 / 87 / 192 us); FP64
-// blocks 4 rows, 1 chunk, 4 waves, cached loads (0.40 ms, 4.6 TB/s; one-wave
-// shapes 0.44-0.77 ms).
+// TB/s at the largest; 4 rows, 1 chunk, 4 waves: 28 / 87 / 192 us).  The
+// FP64 blocks use k_zmatvec64 below.
 template <class TP, int ZROWS, int UNR, int WPB, bool NT>
 __global__ __launch_bounds__(64 * WPB) void k_zmatvec(int N, const TP* __restrict__ P, long ldP,
                                                 const double* __restrict__ wr,
@@ -118,6 +117,79 @@ __global__ __launch_bounds__(64 * WPB) void k_zmatvec(int N, const TP* __restric
     }
 }
 
+// The FP64 product keeps the load-then-accumulate order per row (4 rows per
+// wave, 4 waves): loading all rows of a chunk first, as the FP32 kernel does,
+// measured 0.40 ms against 0.32 ms for this form at N = 15,044.
+__global__ __launch_bounds__(256) void k_zmatvec64(int N, const double* __restrict__ P, long ldP,
+                                                   const double* __restrict__ wr,
+                                                   const double* __restrict__ wi,
+                                                   const double* __restrict__ shift,
+                                                   const double* __restrict__ x,
+                                                   const double* __restrict__ b,
+                                                   double* __restrict__ y)
+{
+    constexpr int ZROWS = 4;
+    typedef double d4 __attribute__((ext_vector_type(4)));
+    const int lane = threadIdx.x & 63;
+    const int row0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * ZROWS;
+    if (row0 >= N) return;
+    const int nr = min(ZROWS, N - row0);
+    double sr[ZROWS], si[ZROWS];
+#pragma unroll
+    for (int k = 0; k < ZROWS; k++) sr[k] = si[k] = 0.0;
+    const int full = N / 256 * 256;
+    for (int c = 4 * lane; c < full; c += 256) {
+        const d4 xr = *(const d4*)(x + c), xi = *(const d4*)(x + N + c);
+#pragma unroll
+        for (int k = 0; k < ZROWS; k++) {
+            if (k < nr) {
+                const d4 v = *(const d4*)(P + (long)(row0 + k) * ldP + c);
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    sr[k] += v[e] * xr[e];
+                    si[k] += v[e] * xi[e];
+                }
+            }
+        }
+    }
+    for (int c = full + lane; c < N; c += 64) {
+        const double xr = x[c], xi = x[N + c];
+#pragma unroll
+        for (int k = 0; k < ZROWS; k++)
+            if (k < nr) {
+                const double v = P[(long)(row0 + k) * ldP + c];
+                sr[k] += v * xr;
+                si[k] += v * xi;
+            }
+    }
+#pragma unroll
+    for (int k = 0; k < ZROWS; k++)
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            sr[k] += __shfl_xor(sr[k], o);
+            si[k] += __shfl_xor(si[k], o);
+        }
+    if (lane < nr) {
+        double zr = 0.0, zi = 0.0;
+#pragma unroll
+        for (int k = 0; k < ZROWS; k++)
+            if (k == lane) {
+                zr = sr[k];
+                zi = si[k];
+            }
+        const int row = row0 + lane;
+        const double a = wr[row], bb = wi[row], sh = shift[row];
+        double yr = a * zr - bb * zi + sh * x[row];
+        double yi = a * zi + bb * zr + sh * x[N + row];
+        if (b) {
+            yr = b[row] - yr;
+            yi = b[N + row] - yi;
+        }
+        y[row] = yr;
+        y[N + row] = yi;
+    }
+}
+
 }  // namespace
 
 hipError_t launch_weight_fp64(dep_sys* s)
@@ -140,8 +212,8 @@ hipError_t launch_zmatvec_w(dep_sys* s, const double* wr, const double* wi, cons
     dep_geom* g = s->g;
     hipEvent_t ev;
     prof_begin(g->ctx, &ev);
-    hipLaunchKernelGGL((k_zmatvec<double, 4, 1, 4, false>), dim3(dep_cdiv(g->N, 16)), dim3(256), 0, g->ctx->stream,
-                       g->N, g->P, g->ldP, wr, wi, shift, x, b, y, (const int*)nullptr);
+    hipLaunchKernelGGL(k_zmatvec64, dim3(dep_cdiv(g->N, 16)), dim3(256), 0, g->ctx->stream, g->N, g->P, g->ldP, wr,
+                       wi, shift, x, b, y);
     const double n = g->N;
     prof_end(g->ctx, ev, "zmatvec", 4.0 * n * n, 8.0 * n * n + 48.0 * n);
     return hipGetLastError();
