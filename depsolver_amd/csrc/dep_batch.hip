@@ -35,8 +35,10 @@ constexpr int BG_PAD = 4;
 template <class TP, class TA>
 __global__ __launch_bounds__(256) void k_bgemm(int N, const TP* __restrict__ P, long ldP, int kchunk,
                                                dep_blist L, const double* __restrict__ X, long xstride,
-                                               TA* __restrict__ part)
+                                               TA* __restrict__ part, const int* __restrict__ gfl)
 {
+    // with run flags: nothing to do once every system of the list has stopped
+    if (gfl && !__syncthreads_or(threadIdx.x < L.n && !gfl[4 * L.id[threadIdx.x < L.n ? threadIdx.x : 0]])) return;
     typedef typename mfma16<TA>::acc acc;
     typedef TA v4 __attribute__((ext_vector_type(4)));
     __shared__ __align__(16) TA As[BG_BM][BG_BK + BG_PAD];
@@ -130,12 +132,13 @@ __global__ __launch_bounds__(256) void k_bgemm_epi(int N, int S, const TA* __res
                                                    const double* __restrict__ X, long xstride,
                                                    const double* __restrict__ W, long wstride,
                                                    const double* __restrict__ b, double* __restrict__ Y,
-                                                   long ystride)
+                                                   long ystride, const int* __restrict__ gfl)
 {
     const int na = L.n, nc = 2 * na;
     const long idx = (long)blockIdx.x * 256 + threadIdx.x;
     if (idx >= (long)N * na) return;
     const int row = (int)(idx / na), a = (int)(idx % na);
+    if (gfl && gfl[4 * L.id[a]]) return;
     double zr = 0.0, zi = 0.0;
     for (int s = 0; s < S; s++) {
         const TA* p = part + ((long)s * N + row) * nc + 2 * a;
@@ -159,11 +162,13 @@ __global__ __launch_bounds__(256) void k_bgemm_epi(int N, int S, const TA* __res
 
 // Partial dots of system L.id[z]: part_j[v][c] = sum over chunk c of V_j[v] . V_j[wv]
 __global__ __launch_bounds__(256) void k_bdots(long n2, dep_blist L, const double* __restrict__ V, long vstride,
-                                               int wv, double* __restrict__ red, long rstride, long poff)
+                                               int wv, double* __restrict__ red, long rstride, long poff,
+                                               const int* __restrict__ gfl)
 {
     __shared__ double sh[4];
     const int c = blockIdx.x, v = blockIdx.y;
     const long j = L.id[blockIdx.z];
+    if (gfl && gfl[4 * j]) return;   // this system's run has stopped
     const long len = (n2 + DEP_DOT_CHUNKS - 1) / DEP_DOT_CHUNKS;
     const long e0 = c * len, e1 = min(n2, e0 + len);
     const double* Vj = V + j * vstride;
@@ -185,11 +190,12 @@ __global__ __launch_bounds__(256) void k_bdots(long n2, dep_blist L, const doubl
 #define BUPD_MAXV 1024
 __global__ __launch_bounds__(256) void k_bupdate(int nv, long n2, dep_blist L, double* __restrict__ V, long vstride,
                                                  int wv, double* __restrict__ red, long rstride, long poff, long hoff,
-                                                 long noff)
+                                                 long noff, const int* __restrict__ gfl)
 {
     __shared__ double h[BUPD_MAXV + 1];
     __shared__ double sq4[4];
     const long j = L.id[blockIdx.y];
+    if (gfl && gfl[4 * j]) return;
     double* rj = red + j * rstride;
     for (int v = threadIdx.x; v <= nv; v += 256) {
         double s = 0.0;
@@ -216,37 +222,12 @@ __global__ __launch_bounds__(256) void k_bupdate(int nv, long n2, dep_blist L, d
     if (threadIdx.x == 0) rj[noff + blockIdx.x] = (sq4[0] + sq4[1]) + (sq4[2] + sq4[3]);
 }
 
-// Block a of L: the Hessenberg column of step i of system j = L.id[a] into
-// out[a (i + 3) ..]: h1[v] + h2[v] (v <= i), |w|^2 before orthogonalisation,
-// |w|^2 after; then V_j[i + 1] /= |w| (unless zero).
-__global__ __launch_bounds__(256) void k_bcollect(int i, long n2, dep_blist L, double* __restrict__ V, long vstride,
-                                                  const double* __restrict__ red, long rstride, long h1off,
-                                                  long h2off, long noff, double* __restrict__ out)
-{
-    __shared__ double hn2;
-    const long j = L.id[blockIdx.x];
-    const double* rj = red + j * rstride;
-    double* o = out + (long)blockIdx.x * (i + 3);
-    for (int v = threadIdx.x; v <= i; v += 256) o[v] = rj[h1off + v] + rj[h2off + v];
-    if (threadIdx.x == 0) {
-        double s = 0.0;
-        for (int k = 0; k < DEP_UPD_BLOCKS; k++) s += rj[noff + k];
-        hn2 = s;
-        o[i + 1] = rj[h1off + i + 1];
-        o[i + 2] = s;
-    }
-    __syncthreads();
-    if (!(hn2 > 0.0)) return;
-    const double sc = 1.0 / sqrt(hn2);
-    double* wv = V + j * vstride + (long)(i + 1) * n2;
-    for (long t = threadIdx.x; t < n2; t += 256) wv[t] *= sc;
-}
-
 // Block a of L: V_j[0] = D_j r_j / |D_j r_j| (both halves scaled by D_j, the
 // Jacobi factors; D null: no scaling), beta[a] = |D_j r_j|.  r_j = R + j * rstride.
 __global__ __launch_bounds__(256) void k_bstart(int N, dep_blist L, const double* __restrict__ R, long rstride,
                                                 const double* __restrict__ D, double* __restrict__ V, long vstride,
-                                                double* __restrict__ beta)
+                                                double* __restrict__ beta, int cap, double* __restrict__ gst, long gs,
+                                                int* __restrict__ gfl, double tol_rel)
 {
     __shared__ double sq4[4];
     __shared__ double bsh;
@@ -271,9 +252,115 @@ __global__ __launch_bounds__(256) void k_bstart(int N, dep_blist L, const double
         beta[blockIdx.x] = bsh;
     }
     __syncthreads();
+    if (gst) {   // the system's device Givens state (layout of GivensState)
+        double* g = gst + j * gs + (long)(cap + 1) * (cap + 2) + 2L * (cap + 2);
+        for (int k = threadIdx.x; k < cap + 2; k += 256) g[k] = 0.0;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double* sc = g + 2L * (cap + 2);
+            int* fl = gfl + 4 * j;
+            g[0] = bsh;
+            sc[1] = bsh;
+            sc[2] = tol_rel * bsh;
+            fl[0] = (bsh < tol_rel * bsh || !(bsh > 0.0)) ? 1 : 0;
+            fl[1] = 0;
+            fl[2] = fl[0] ? 0 : 1;
+        }
+    }
     if (!(bsh > 0.0)) return;
     const double sc = 1.0 / bsh;
     for (int i = threadIdx.x; i < 2 * N; i += 256) v0[i] *= sc;
+}
+
+// Device Givens step i of the systems of L (one workgroup each): the column
+// h1 + h2, |w| from the norm partials, the previous rotations, the new one and
+// the residual estimate -- the arithmetic of k_gmres_givens -- then V_j[i+1] /= |w|.
+// State of system j at gst + j gs: H (cap + 1 columns of cap + 2), cs, sn, g,
+// err (cap + 2 each), sc[4]; flags gfl[4 j ..]: stop, steps, rc.
+__global__ __launch_bounds__(256) void k_bgivens(int i, int cap, long n2, dep_blist L, const double* __restrict__ red,
+                                                 long rstride, long h1off, long h2off, long noff,
+                                                 double* __restrict__ gst, long gs, int* __restrict__ gfl,
+                                                 double* __restrict__ V, long vstride)
+{
+    __shared__ double scale;
+    const long j = L.id[blockIdx.x];
+    int* fl = gfl + 4 * j;
+    if (fl[0]) return;
+    const double* rj = red + j * rstride;
+    const double* h1 = rj + h1off;
+    const double* h2 = rj + h2off;
+    double* H = gst + j * gs;
+    double* cs = H + (long)(cap + 1) * (cap + 2);
+    double* sn = cs + (cap + 2);
+    double* g = sn + (cap + 2);
+    double* err = g + (cap + 2);
+    double* sc = err + (cap + 2);
+    double* col = H + (long)i * (cap + 2);
+    for (int k = threadIdx.x; k <= i; k += 256) col[k] = h1[k] + h2[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double wn = sqrt(h1[i + 1]);
+        double hn2 = 0.0;
+        for (int k = 0; k < DEP_UPD_BLOCKS; k++) hn2 += rj[noff + k];
+        const double hn = sqrt(hn2);
+        col[i + 1] = hn;
+        for (int k = 0; k <= i - 1; k++) {
+            const double g1 = cs[k] * col[k] + sn[k] * col[k + 1];
+            const double g2 = -sn[k] * col[k] + cs[k] * col[k + 1];
+            col[k] = g1;
+            col[k + 1] = g2;
+        }
+        const double htmp = col[i + 1] / col[i];
+        if (wn != 0.0) {
+            cs[i] = 1.0 / sqrt(1.0 + htmp * htmp);
+            sn[i] = cs[i] * htmp;
+            col[i] = cs[i] * col[i] + sn[i] * col[i + 1];
+            col[i + 1] = 0.0;
+            g[i + 1] = -sn[i] * g[i];
+            g[i] = cs[i] * g[i];
+        }
+        const double err1 = fabs(g[i + 1]), err0 = sc[1], tol = sc[2];
+        err[i] = err1;
+        fl[1] = i + 1;
+        if (err1 < tol) {
+            fl[0] = 1;
+            fl[2] = 0;
+        } else if (fabs((err1 - err0) / err0) < 1.0e-6 * tol) {
+            fl[0] = 1;
+            fl[2] = 2;
+        } else {
+            sc[1] = err1;
+        }
+        scale = (fl[0] || hn == 0.0) ? 0.0 : 1.0 / hn;
+    }
+    __syncthreads();
+    const double a = scale;
+    if (a == 0.0) return;
+    double* wv = V + j * vstride + (long)(i + 1) * n2;
+    for (long t = threadIdx.x; t < n2; t += 256) wv[t] *= a;
+}
+
+// Coefficients of each system of L from its device state: y solves the
+// triangular Hessenberg system of the steps done (none after a stagnation),
+// into Y + a * ystride with the count in cnt[a].
+__global__ void k_bbacksub(int cap, dep_blist L, const double* __restrict__ gst, long gs, const int* __restrict__ gfl,
+                           double* __restrict__ Y, long ystride, int* __restrict__ cnt)
+{
+    if (threadIdx.x != 0) return;
+    const int a = blockIdx.x;
+    const long j = L.id[a];
+    const int* fl = gfl + 4 * j;
+    const int iters = fl[1], rc = fl[0] ? fl[2] : 1;
+    const int n = rc != 2 ? iters : 0;
+    const double* H = gst + j * gs;
+    const double* g = H + (long)(cap + 1) * (cap + 2) + 2L * (cap + 2);
+    double* y = Y + (long)a * ystride;
+    for (int k = 0; k < n; k++) y[k] = g[k];
+    for (int k = n - 1; k >= 0; k--) {
+        for (int l = k + 1; l < n; l++) y[k] -= H[(long)l * (cap + 2) + k] * y[l];
+        y[k] = y[k] / H[(long)k * (cap + 2) + k];
+    }
+    cnt[a] = n;
 }
 
 // Block a of L: out[a] = |r_j|^2
@@ -342,7 +429,7 @@ void bgemm_split(dep_ctx* c, int N, int* S, int* kchunk)
 template <class TP, class TA>
 static hipError_t bgemm(dep_ctx* c, int N, const TP* P, long ldP, const dep_blist& L, const double* X, long xstride,
                         TA* part, const double* W, long wstride, const double* b, double* Y, long ystride,
-                        const char* name)
+                        const char* name, const int* gfl)
 {
     if (L.n < 1 || L.n > DEP_BATCH_MAX) return hipErrorInvalidValue;
     int S, kc;
@@ -350,27 +437,29 @@ static hipError_t bgemm(dep_ctx* c, int N, const TP* P, long ldP, const dep_blis
     hipEvent_t ev;
     prof_begin(c, &ev);
     hipLaunchKernelGGL((k_bgemm<TP, TA>), dim3(dep_cdiv(N, BG_BM), S), dim3(256), 0, c->stream, N, P, ldP, kc, L, X,
-                       xstride, part);
+                       xstride, part, gfl);
     const double n = N, nc = 2.0 * L.n;
     prof_end(c, ev, name, 2.0 * n * n * nc, sizeof(TP) * n * n + 8.0 * n * nc);
     const long tot = (long)N * L.n;
     hipLaunchKernelGGL((k_bgemm_epi<TA>), dim3((unsigned)dep_cdiv(tot, 256)), dim3(256), 0, c->stream, N, S, part,
-                       L, X, xstride, W, wstride, b, Y, ystride);
+                       L, X, xstride, W, wstride, b, Y, ystride, gfl);
     return hipGetLastError();
 }
 
 hipError_t launch_bmatvec32(dep_ctx* c, int N, const float* P32, long ldP, const dep_blist& L, const double* X,
-                            long xstride, void* part, const double* W, long wstride, double* Y, long ystride)
+                            long xstride, void* part, const double* W, long wstride, double* Y, long ystride,
+                            const int* gfl)
 {
     return bgemm<float, float>(c, N, P32, ldP, L, X, xstride, (float*)part, W, wstride, nullptr, Y, ystride,
-                               "bgemm32");
+                               "bgemm32", gfl);
 }
 
 hipError_t launch_bresidual64(dep_ctx* c, int N, const double* P, long ldP, const dep_blist& L, const double* X,
                               long xstride, void* part, const double* W, long wstride, const double* b, double* Y,
                               long ystride)
 {
-    return bgemm<double, double>(c, N, P, ldP, L, X, xstride, (double*)part, W, wstride, b, Y, ystride, "bgemm64");
+    return bgemm<double, double>(c, N, P, ldP, L, X, xstride, (double*)part, W, wstride, b, Y, ystride, "bgemm64",
+                                 nullptr);
 }
 
 size_t bgemm_part_bytes(dep_ctx* c, int N)
@@ -381,28 +470,37 @@ size_t bgemm_part_bytes(dep_ctx* c, int N)
 }
 
 hipError_t launch_bcgs_pass(dep_ctx* c, int nvec, long n2, const dep_blist& L, double* V, long vstride, int wv,
-                            double* red, long rstride, long poff, long hoff, long noff)
+                            double* red, long rstride, long poff, long hoff, long noff, const int* gfl)
 {
     if (nvec - 1 > BUPD_MAXV) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_bdots, dim3(DEP_DOT_CHUNKS, nvec, L.n), dim3(256), 0, c->stream, n2, L, V, vstride, wv, red,
-                       rstride, poff);
+                       rstride, poff, gfl);
     hipLaunchKernelGGL(k_bupdate, dim3(DEP_UPD_BLOCKS, L.n), dim3(256), 0, c->stream, nvec - 1, n2, L, V, vstride,
-                       wv, red, rstride, poff, hoff, noff);
+                       wv, red, rstride, poff, hoff, noff, gfl);
     return hipGetLastError();
 }
 
-hipError_t launch_bcollect(dep_ctx* c, int i, long n2, const dep_blist& L, double* V, long vstride, const double* red,
-                           long rstride, long h1off, long h2off, long noff, double* out)
+hipError_t launch_bgivens(dep_ctx* c, int i, int cap, long n2, const dep_blist& L, const double* red, long rstride,
+                          long h1off, long h2off, long noff, double* gst, long gs, int* gfl, double* V, long vstride)
 {
-    hipLaunchKernelGGL(k_bcollect, dim3(L.n), dim3(256), 0, c->stream, i, n2, L, V, vstride, red, rstride, h1off,
-                       h2off, noff, out);
+    hipLaunchKernelGGL(k_bgivens, dim3(L.n), dim3(256), 0, c->stream, i, cap, n2, L, red, rstride, h1off, h2off, noff,
+                       gst, gs, gfl, V, vstride);
+    return hipGetLastError();
+}
+
+hipError_t launch_bbacksub(dep_ctx* c, int cap, const dep_blist& L, const double* gst, long gs, const int* gfl,
+                           double* Y, long ystride, int* cnt)
+{
+    hipLaunchKernelGGL(k_bbacksub, dim3(L.n), dim3(64), 0, c->stream, cap, L, gst, gs, gfl, Y, ystride, cnt);
     return hipGetLastError();
 }
 
 hipError_t launch_bstart(dep_ctx* c, int N, const dep_blist& L, const double* R, long rstride, const double* D,
-                         double* V, long vstride, double* beta)
+                         double* V, long vstride, double* beta, int cap, double* gst, long gs, int* gfl,
+                         double tol_rel)
 {
-    hipLaunchKernelGGL(k_bstart, dim3(L.n), dim3(256), 0, c->stream, N, L, R, rstride, D, V, vstride, beta);
+    hipLaunchKernelGGL(k_bstart, dim3(L.n), dim3(256), 0, c->stream, N, L, R, rstride, D, V, vstride, beta, cap, gst,
+                       gs, gfl, tol_rel);
     return hipGetLastError();
 }
 

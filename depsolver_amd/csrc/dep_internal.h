@@ -242,7 +242,8 @@ struct dep_blist {
 };
 // Y_j = diag(GW_j) (P32 X_j) + shift: W holds wr, wi, shift (N each) per system
 hipError_t launch_bmatvec32(dep_ctx* c, int N, const float* P32, long ldP, const dep_blist& L, const double* X,
-                            long xstride, void* part, const double* W, long wstride, double* Y, long ystride);
+                            long xstride, void* part, const double* W, long wstride, double* Y, long ystride,
+                            const int* gfl = nullptr);
 // Y_j = b - (diag(W_j) (P X_j) + shift) in FP64
 hipError_t launch_bresidual64(dep_ctx* c, int N, const double* P, long ldP, const dep_blist& L, const double* X,
                               long xstride, void* part, const double* W, long wstride, const double* b, double* Y,
@@ -250,11 +251,15 @@ hipError_t launch_bresidual64(dep_ctx* c, int N, const double* P, long ldP, cons
 size_t bgemm_part_bytes(dep_ctx* c, int N);
 // one CGS pass on V_j[wv] against V_j[0 .. nvec-2] (red_j: partials at poff, h at hoff, norm partials at noff >= 0)
 hipError_t launch_bcgs_pass(dep_ctx* c, int nvec, long n2, const dep_blist& L, double* V, long vstride, int wv,
-                            double* red, long rstride, long poff, long hoff, long noff);
-hipError_t launch_bcollect(dep_ctx* c, int i, long n2, const dep_blist& L, double* V, long vstride, const double* red,
-                           long rstride, long h1off, long h2off, long noff, double* out);
+                            double* red, long rstride, long poff, long hoff, long noff, const int* gfl = nullptr);
+// device Givens steps and coefficients of batched GMRES runs (state per system at gst + j gs, flags gfl + 4 j)
+hipError_t launch_bgivens(dep_ctx* c, int i, int cap, long n2, const dep_blist& L, const double* red, long rstride,
+                          long h1off, long h2off, long noff, double* gst, long gs, int* gfl, double* V, long vstride);
+hipError_t launch_bbacksub(dep_ctx* c, int cap, const dep_blist& L, const double* gst, long gs, const int* gfl,
+                           double* Y, long ystride, int* cnt);
 hipError_t launch_bstart(dep_ctx* c, int N, const dep_blist& L, const double* R, long rstride, const double* D,
-                         double* V, long vstride, double* beta);
+                         double* V, long vstride, double* beta, int cap = 0, double* gst = nullptr, long gs = 0,
+                         int* gfl = nullptr, double tol_rel = 0.0);
 hipError_t launch_bnorm2(dep_ctx* c, long n2, const dep_blist& L, const double* R, long rstride, double* out);
 hipError_t launch_bcombine(dep_ctx* c, long n2, const dep_blist& L, const int* cnt, const double* Y, long ystride,
                            const double* V, long vstride, double* X, long xstride);
@@ -268,5 +273,6 @@ size_t bj_floats(int N, int nsys);
 hipError_t launch_bj_build(dep_ctx* c, int N, const double* P, long ldP, const double* wr, const double* wi,
                            const double* sh, long wstride, int nsys, double* scratch, float* B);
 // Y_j = M_j^-1 X_j (accumulate: Y_j += M_j^-1 X_j) for the systems of L
+// stop: system sys is skipped when stop[sys * sstride] != 0
 hipError_t launch_bj_apply(dep_ctx* c, int N, const float* B, const dep_blist& L, const double* X, long xstride,
-                           double* Y, long ystride, int accumulate = 0, const int* stop = nullptr);
+                           double* Y, long ystride, int accumulate = 0, const int* stop = nullptr, int sstride = 0);

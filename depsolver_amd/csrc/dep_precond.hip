@@ -85,12 +85,13 @@ __global__ __launch_bounds__(256) void k_bj_combine(int nblk, const double* __re
 // partial sums of a row are combined by lane shuffles.
 __global__ __launch_bounds__(256) void k_bj_apply(int N, int nblk, const float* __restrict__ B, dep_blist L,
                                                   const double* __restrict__ X, long xstride, double* __restrict__ Y,
-                                                  long ystride, int accumulate, const int* __restrict__ stop)
+                                                  long ystride, int accumulate, const int* __restrict__ stop,
+                                                  int sstride)
 {
-    if (stop && *stop) return;
     __shared__ double xr[BJ], xi[BJ];
     const int b = blockIdx.x;
     const long sys = L.id[blockIdx.y];
+    if (stop && stop[sys * sstride]) return;
     const double* x = X + sys * xstride;
     double* y = Y + sys * ystride;
     const int t = threadIdx.x, r = t >> 2, q = t & 3;
@@ -148,10 +149,10 @@ hipError_t launch_bj_build(dep_ctx* c, int N, const double* P, long ldP, const d
 }
 
 hipError_t launch_bj_apply(dep_ctx* c, int N, const float* B, const dep_blist& L, const double* X, long xstride,
-                           double* Y, long ystride, int accumulate, const int* stop)
+                           double* Y, long ystride, int accumulate, const int* stop, int sstride)
 {
     const int nblk = (int)bj_blocks(N);
     hipLaunchKernelGGL(k_bj_apply, dim3(nblk, L.n), dim3(256), 0, c->stream, N, nblk, B, L, X, xstride, Y, ystride,
-                       accumulate, stop);
+                       accumulate, stop, sstride);
     return hipGetLastError();
 }

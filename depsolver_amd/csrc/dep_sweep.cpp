@@ -7,12 +7,12 @@
 // :496-512).  Here the geometric blocks P are integrated once and every
 // frequency f is the system Z_f = diag(w_f) P + diag(shift_f).  All systems
 // run the refined FP32 GMRES of dep_gmres_ir (block-Jacobi right
-// preconditioning with each frequency's own blocks) in lockstep: one batched FP64
-// residual product per correction, and per Krylov step one batched FP32
-// product on the matrix cores plus batched CGS2 kernels, with one transfer of
-// all the systems' Hessenberg columns to the host, where each system's Givens
-// recurrence runs as in src/gmres_el.c:196-241.  A system leaves the batch as
-// soon as it converges.
+// preconditioning with each frequency's own blocks) in lockstep: one batched
+// FP64 residual product per correction, and per Krylov step one batched FP32
+// product on the matrix cores, batched CGS2 kernels and one device Givens step
+// per system (the recurrence of src/gmres_el.c:196-241, as in dep_gmres_ir).
+// The host reads the systems' stop flags once per 8 steps; a system leaves the
+// batch once its run has stopped.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdlib.h>
@@ -38,15 +38,17 @@ struct dep_sweep {
     double* V = nullptr;          // nf x (cap + 1) x 2N Krylov bases
     double* red = nullptr;        // nf x rstride reduction scratch
     void* part = nullptr;         // split-K partials of the batched products
-    double* cbuf = nullptr;       // DEP_BATCH_MAX x (cap + 3) Hessenberg columns
     double* Yd = nullptr;         // DEP_BATCH_MAX x (cap + 1) combination coefficients
     int* cntd = nullptr;          // DEP_BATCH_MAX coefficient counts
     double* scal = nullptr;       // DEP_BATCH_MAX per-system scalars
+    double* gst = nullptr;        // nf x gs device Givens states (H, cs, sn, g, err, scalars)
+    int* gfl = nullptr;           // nf x 4 flags: stop, steps, rc
     bool block = true;            // block-Jacobi right preconditioning (else row scaling)
     float* bjB = nullptr;         // nf x (N / 64) inverse diagonal blocks
     double* mres = nullptr;       // nf MST result slots
     size_t mres_n = 0;
     double* hpin = nullptr;       // pinned host staging
+    hipEvent_t evf[2] = {nullptr, nullptr};   // flag copies of alternate batches
     size_t hpin_count = 0;
     std::vector<int> iters, outer;   // per system, last solve
     std::vector<double> res;
@@ -58,6 +60,8 @@ struct dep_sweep {
 namespace {
 
 long rstride_of(int cap) { return 2L * (cap + 2) + DEP_UPD_BLOCKS + (long)(cap + 2) * DEP_DOT_CHUNKS; }
+// doubles of one system's device Givens state: H (cap + 1 columns of cap + 2), cs, sn, g, err, 4 scalars
+long gstate_size(int cap) { return (long)(cap + 1) * (cap + 2) + 4L * (cap + 2) + 4; }
 
 void release(dep_sweep* w, void* p)
 {
@@ -76,16 +80,17 @@ int ensure_storage(dep_sweep* w, int max_inner)
     if (w->V && w->cap >= max_inner) return DEP_OK;
     dep_ctx* c = w->g->ctx;
     DEP_HIP(c, hipStreamSynchronize(c->stream));
-    for (void* p : {(void*)w->V, (void*)w->red, (void*)w->cbuf, (void*)w->Yd})
+    for (void* p : {(void*)w->V, (void*)w->red, (void*)w->Yd, (void*)w->gst})
         if (p) release(w, p);
     if (w->hpin) (void)hipHostFree(w->hpin);
     w->hpin = nullptr;
     const long n2 = 2L * w->g->N;
     DEP_CHECK(dep_obj_alloc(w, &w->V, (size_t)w->nf * (max_inner + 1) * n2));
     DEP_CHECK(dep_obj_alloc(w, &w->red, (size_t)w->nf * rstride_of(max_inner)));
-    DEP_CHECK(dep_obj_alloc(w, &w->cbuf, (size_t)DEP_BATCH_MAX * (max_inner + 3)));
     DEP_CHECK(dep_obj_alloc(w, &w->Yd, (size_t)DEP_BATCH_MAX * (max_inner + 1)));
-    w->hpin_count = (size_t)DEP_BATCH_MAX * (max_inner + 3) + (size_t)DEP_BATCH_MAX * (max_inner + 1) + DEP_BATCH_MAX;
+    DEP_CHECK(dep_obj_alloc(w, &w->gst, (size_t)w->nf * gstate_size(max_inner)));
+    if (!w->gfl) DEP_CHECK(dep_obj_alloc(w, &w->gfl, 4 * (size_t)w->nf));
+    w->hpin_count = std::max((size_t)DEP_BATCH_MAX * (max_inner + 3), (size_t)8 * w->nf);
     if (hipHostMalloc((void**)&w->hpin, sizeof(double) * w->hpin_count, 0) != hipSuccess) {
         w->hpin = nullptr;
         return dep_fail(c, DEP_ERR_NOMEM, "sweep: pinned host staging of %zu bytes", sizeof(double) * w->hpin_count);
@@ -102,12 +107,6 @@ int down(dep_ctx* c, T* h, const T* d, size_t n)
     return DEP_OK;
 }
 
-// Per-system state of one GMRES run (the recurrence of gmres_run in dep_api.cpp)
-struct Run {
-    std::vector<double> H, cs, sn, g;
-    double beta = 0.0, tol = 0.0, err0 = 0.0, err1 = 0.0;
-    int iters = 0, rc = 1;   // rc: 0 converged, 1 iteration limit, 2 stagnation
-};
 
 }  // namespace
 
@@ -123,6 +122,12 @@ extern "C" int dep_sweep_create(dep_geom* g, int nM, const double* mats, int nI,
     dep_sweep* w = new dep_sweep();
     w->g = g;
     w->nf = nf;
+    for (hipEvent_t& e : w->evf)
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+            e = nullptr;
+            dep_sweep_destroy(w);
+            return dep_fail(c, DEP_ERR_HIP, "dep_sweep_create: events");
+        }
     w->freqs.assign(freqs, freqs + nf);
     auto fail = [&](int r) {
         dep_sweep_destroy(w);
@@ -186,6 +191,8 @@ extern "C" int dep_sweep_destroy(dep_sweep* w)
     if (!w) return DEP_OK;
     dep_ctx* c = w->g->ctx;
     (void)hipStreamSynchronize(c->stream);
+    for (hipEvent_t& e : w->evf)
+        if (e) (void)hipEventDestroy(e);
     if (w->hpin) (void)hipHostFree(w->hpin);
     for (auto& a : w->allocs) dep_free(c, a.first, a.second);
     if (w->sys) dep_sys_destroy(w->sys);
@@ -205,8 +212,6 @@ extern "C" int dep_sweep_solve(dep_sweep* w, double tol, double inner_tol, int m
     const long n2 = 2L * N, vstride = (long)(w->cap + 1) * n2, rstride = rstride_of(w->cap);
     const long h1off = 0, h2off = w->cap + 2, noff = 2L * (w->cap + 2), poff = noff + DEP_UPD_BLOCKS;
     double* hdown = w->hpin;
-    double* hY = hdown + (size_t)DEP_BATCH_MAX * (w->cap + 3);
-    int* hcnt = (int*)(hY + (size_t)DEP_BATCH_MAX * (w->cap + 1));
     const double* b = w->sys->b;
 
     DEP_HIP(c, hipEventRecord(c->ev0, c->stream));
@@ -216,7 +221,6 @@ extern "C" int dep_sweep_solve(dep_sweep* w, double tol, double inner_tol, int m
     w->res.assign(nf, 1.0);
     std::vector<char> done(nf, 0), stalled(nf, 0);
     bool first = true;
-    std::vector<Run> run(nf);
     for (;;) {
         dep_blist L;
         for (int j = 0; j < nf; j++)
@@ -237,94 +241,59 @@ extern "C" int dep_sweep_solve(dep_sweep* w, double tol, double inner_tol, int m
                 A.id[A.n++] = j;
         }
         if (A.n == 0) break;
-        // inner right-hand sides D_j r_j, normalised into V_j[0]
+        // inner right-hand sides (D_j) r_j, normalised into V_j[0]; device Givens states reset
+        const long gs = gstate_size(w->cap);
         DEP_HIP(c, launch_bstart(c, N, A, first ? b : w->R, first ? 0 : n2, w->block ? nullptr : w->D, w->V, vstride,
-                                 w->scal));
-        DEP_CHECK(down(c, hdown, w->scal, A.n));
-        dep_blist I;
-        std::vector<int> slot(nf, -1);   // index of a system in A
+                                 w->scal, w->cap, w->gst, gs, w->gfl, inner_tol));
+        // Krylov steps of all open systems, 8 at a time: the Givens recurrence runs on the
+        // device (k_bgivens) and the per-system kernels skip a system whose run stopped (the
+        // batched product once all of its systems have).  The flags of a batch are copied to
+        // pinned memory behind it and read after the next batch is queued, so the device
+        // never waits for the host; the stopped systems leave the list one batch later.
+        constexpr int BATCH = 8;
+        int* pf[2] = {(int*)w->hpin, (int*)w->hpin + 4 * nf};
+        dep_blist I = A;
+        int i = 0, nb = 0;
+        while (I.n > 0 && i < M) {
+            for (const int iend = std::min(M, i + BATCH); i < iend; i++) {
+                if (w->block) {   // V_j[i+1] = Z_j M_j^-1 V_j[i], M_j^-1 V_j[i] staged in R (free inside a pass)
+                    DEP_HIP(c, launch_bj_apply(c, N, w->bjB, I, w->V + (long)i * n2, vstride, w->R, n2, 0, w->gfl, 4));
+                    DEP_HIP(c, launch_bmatvec32(c, N, g->P32, g->ldP, I, w->R, n2, w->part, w->W, 3L * N,
+                                                w->V + (long)(i + 1) * n2, vstride, w->gfl));
+                } else {
+                    DEP_HIP(c, launch_bmatvec32(c, N, g->P32, g->ldP, I, w->V + (long)i * n2, vstride, w->part,
+                                                w->GW, 3L * N, w->V + (long)(i + 1) * n2, vstride, w->gfl));
+                }
+                DEP_HIP(c, launch_bcgs_pass(c, i + 2, n2, I, w->V, vstride, i + 1, w->red, rstride, poff, h1off, -1,
+                                            w->gfl));
+                DEP_HIP(c, launch_bcgs_pass(c, i + 2, n2, I, w->V, vstride, i + 1, w->red, rstride, poff, h2off, noff,
+                                            w->gfl));
+                DEP_HIP(c, launch_bgivens(c, i, w->cap, n2, I, w->red, rstride, h1off, h2off, noff, w->gst, gs, w->gfl,
+                                          w->V, vstride));
+            }
+            DEP_HIP(c, hipMemcpyAsync(pf[nb & 1], w->gfl, sizeof(int) * 4 * nf, hipMemcpyDeviceToHost, c->stream));
+            DEP_HIP(c, hipEventRecord(w->evf[nb & 1], c->stream));
+            if (nb > 0) {   // the previous batch's flags
+                DEP_HIP(c, hipEventSynchronize(w->evf[(nb - 1) & 1]));
+                const int* fl = pf[(nb - 1) & 1];
+                dep_blist next;
+                for (int a = 0; a < I.n; a++)
+                    if (!fl[4 * I.id[a]]) next.id[next.n++] = I.id[a];
+                I = next;
+            }
+            nb++;
+        }
+        DEP_HIP(c, hipEventSynchronize(w->evf[(nb - 1) & 1]));
+        std::vector<int> hfl(pf[(nb - 1) & 1], pf[(nb - 1) & 1] + 4 * nf);
+        // coefficients y_j on the device; x_j += (M_j^-1) V_j y_j
+        DEP_HIP(c, launch_bbacksub(c, w->cap, A, w->gst, gs, w->gfl, w->Yd, w->cap + 1, w->cntd));
         for (int a = 0; a < A.n; a++) {
             const int j = A.id[a];
-            slot[j] = a;
-            Run& q = run[j];
-            q.H.assign((size_t)(M + 1) * (M + 1), 0.0);
-            q.cs.assign(M + 1, 0.0);
-            q.sn.assign(M + 1, 0.0);
-            q.g.assign(M + 2, 0.0);
-            q.beta = hdown[a];
-            q.g[0] = q.beta;
-            q.tol = inner_tol * q.beta;
-            q.err0 = q.err1 = q.beta;
-            q.iters = 0;
-            q.rc = q.beta < q.tol || !(q.beta > 0.0) ? 0 : 1;
-            if (q.rc == 1) I.id[I.n++] = j;
-        }
-        for (int i = 0; I.n > 0 && i < M; i++) {
-            if (w->block) {   // V_j[i+1] = Z_j M_j^-1 V_j[i], M_j^-1 V_j[i] staged in R (free inside a pass)
-                DEP_HIP(c, launch_bj_apply(c, N, w->bjB, I, w->V + (long)i * n2, vstride, w->R, n2));
-                DEP_HIP(c, launch_bmatvec32(c, N, g->P32, g->ldP, I, w->R, n2, w->part, w->W, 3L * N,
-                                            w->V + (long)(i + 1) * n2, vstride));
-            } else {
-                DEP_HIP(c, launch_bmatvec32(c, N, g->P32, g->ldP, I, w->V + (long)i * n2, vstride, w->part, w->GW,
-                                            3L * N, w->V + (long)(i + 1) * n2, vstride));
-            }
-            DEP_HIP(c, launch_bcgs_pass(c, i + 2, n2, I, w->V, vstride, i + 1, w->red, rstride, poff, h1off, -1));
-            DEP_HIP(c, launch_bcgs_pass(c, i + 2, n2, I, w->V, vstride, i + 1, w->red, rstride, poff, h2off, noff));
-            DEP_HIP(c, launch_bcollect(c, i, n2, I, w->V, vstride, w->red, rstride, h1off, h2off, noff, w->cbuf));
-            DEP_CHECK(down(c, hdown, w->cbuf, (size_t)I.n * (i + 3)));
-            dep_blist next;
-            for (int a = 0; a < I.n; a++) {
-                const int j = I.id[a];
-                Run& q = run[j];
-                const double* hc = hdown + (size_t)a * (i + 3);
-#define HH(r, col) q.H[(size_t)(r) * (M + 1) + (col)]
-                q.iters++;
-                const double wn = sqrt(hc[i + 1]);
-                for (int k = 0; k <= i; k++) HH(k, i) = hc[k];
-                HH(i + 1, i) = sqrt(hc[i + 2]);
-                for (int k = 0; k <= i - 1; k++) {
-                    const double g1 = q.cs[k] * HH(k, i) + q.sn[k] * HH(k + 1, i);
-                    const double g2 = -q.sn[k] * HH(k, i) + q.cs[k] * HH(k + 1, i);
-                    HH(k, i) = g1;
-                    HH(k + 1, i) = g2;
-                }
-                const double htmp = HH(i + 1, i) / HH(i, i);
-                if (wn != 0.0) {
-                    q.cs[i] = 1.0 / sqrt(1.0 + htmp * htmp);
-                    q.sn[i] = q.cs[i] * htmp;
-                    HH(i, i) = q.cs[i] * HH(i, i) + q.sn[i] * HH(i + 1, i);
-                    HH(i + 1, i) = 0.0;
-                    q.g[i + 1] = -q.sn[i] * q.g[i];
-                    q.g[i] = q.cs[i] * q.g[i];
-                }
-                q.err1 = fabs(q.g[i + 1]);
-                if (q.err1 < q.tol) q.rc = 0;
-                else if (fabs((q.err1 - q.err0) / q.err0) < 1.0e-6 * q.tol) q.rc = 2;
-                else q.err0 = q.err1;
-#undef HH
-                if (q.rc == 1) next.id[next.n++] = j;
-            }
-            I = next;
-        }
-        // x_j += V_j y_j with y_j from the triangular Hessenberg system
-        for (int a = 0; a < A.n; a++) {
-            const int j = A.id[a];
-            Run& q = run[j];
-            const int n = (q.rc != 2) ? q.iters : 0;
-            double* y = hY + (size_t)a * (w->cap + 1);
-            for (int k = 0; k < n; k++) y[k] = q.g[k];
-            for (int k = n - 1; k >= 0; k--) {
-                for (int l = k + 1; l < n; l++) y[k] -= q.H[(size_t)k * (M + 1) + l] * y[l];
-                y[k] /= q.H[(size_t)k * (M + 1) + k];
-            }
-            hcnt[a] = n;
-            w->iters[j] += q.iters;
+            const int steps = hfl[4 * j + 1];
+            w->iters[j] += steps;
             w->outer[j]++;
-            if (q.iters == 0) stalled[j] = 1;
+            if (steps == 0) stalled[j] = 1;
         }
-        DEP_HIP(c, hipMemcpyAsync(w->Yd, hY, sizeof(double) * (size_t)A.n * (w->cap + 1), hipMemcpyHostToDevice,
-                                  c->stream));
-        DEP_HIP(c, hipMemcpyAsync(w->cntd, hcnt, sizeof(int) * A.n, hipMemcpyHostToDevice, c->stream));
         if (w->block) {   // x_j += M_j^-1 (V_j y_j), V_j y_j gathered in R
             DEP_HIP(c, hipMemsetAsync(w->R, 0, sizeof(double) * nf * n2, c->stream));
             DEP_HIP(c, launch_bcombine(c, n2, A, w->cntd, w->Yd, w->cap + 1, w->V, vstride, w->R, n2));
