@@ -185,6 +185,13 @@ void prof_end(dep_ctx* c, hipEvent_t a, const char* name, double flops, double b
     c->pending.push_back({name, a, b, flops, bytes});
 }
 
+size_t prof_mark(const dep_ctx* c) { return c->pending.size(); }
+
+void prof_scale(dep_ctx* c, size_t idx, double f)
+{
+    if (idx < c->pending.size()) c->pending[idx].scale *= f;
+}
+
 void prof_collect(dep_ctx* c)
 {
     if (c->pending.empty()) return;
@@ -192,11 +199,11 @@ void prof_collect(dep_ctx* c)
     (void)hipStreamSynchronize(c->stream2);
     for (auto& p : c->pending) {
         float ms = 0.0f;
-        if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+        if (p.scale > 0.0 && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
             dep_kstat& s = c->stats[p.name];
             s.launches++;
             s.ms += ms;
-            s.flops += p.flops;
+            s.flops += p.flops * p.scale;
             s.bytes += p.bytes;
         }
         c->pool.push_back(p.a);
@@ -1182,9 +1189,10 @@ static int ensure_gmres_storage(dep_sys* s, int max_iters)
     DEP_CHECK(dep_obj_alloc(s, &s->gm_dev, (size_t)(max_iters + 1) * (max_iters + 2) + 4 * (size_t)(max_iters + 2) + 4));
     if (!s->gm_vec) DEP_CHECK(dep_obj_alloc(s, &s->gm_vec, 3 * N + 3 * n2));
     DEP_CHECK(dep_obj_alloc(s, &s->gm_V, (size_t)(max_iters + 1) * n2));
-    // reductions: h1 (cap + 2), h2 (cap + 2), DEP_UPD_BLOCKS norm partials, dot partials
-    DEP_CHECK(dep_obj_alloc(s, &s->gm_red,
-                            2 * (size_t)(max_iters + 2) + DEP_UPD_BLOCKS + (size_t)(max_iters + 2) * DEP_DOT_CHUNKS));
+    // reductions: h1 (cap + 2), h2 (cap + 2), DEP_UPD_BLOCKS norm partials, pass-1 dot partials,
+    // pass-2 dot partials of the fused step
+    DEP_CHECK(dep_obj_alloc(s, &s->gm_red, 2 * (size_t)(max_iters + 2) + DEP_UPD_BLOCKS +
+                                               (size_t)(max_iters + 2) * (DEP_DOT_CHUNKS + DEP_UPD_BLOCKS)));
     s->gm_cap = max_iters;
     return DEP_OK;
 }
@@ -1321,8 +1329,12 @@ static int gmres_run(dep_sys* s, MV matvec, const double* b, bool x_zero, double
 // after M steps.  The host enqueues the steps in batches and reads the stop
 // flag once per batch; steps after the stop are skipped by the kernels.  The
 // triangular solve for the coefficients stays on the host (one transfer).
+// matvec(v, w, stop, scale) enqueues w = Op v; with mv_scales it first scales
+// v in place by *scale (the normalisation of the Krylov vector, 1 / |v| as the
+// previous step left it), otherwise a separate kernel normalises each vector.
 template <class MV>
-static int gmres_run_dev(dep_sys* s, MV matvec, const double* b, double* d, double tol_rel, int M, GmresRun* out)
+static int gmres_run_dev(dep_sys* s, MV matvec, bool mv_scales, const double* b, double* d, double tol_rel, int M,
+                         GmresRun* out)
 {
     dep_ctx* c = s->g->ctx;
     const long n2 = 2L * s->g->N;
@@ -1332,6 +1344,7 @@ static int gmres_run_dev(dep_sys* s, MV matvec, const double* b, double* d, doub
     double* h2 = h1 + (cap + 2);
     double* np = h2 + (cap + 2);
     double* part = np + DEP_UPD_BLOCKS;
+    double* part2 = part + (size_t)(cap + 2) * DEP_DOT_CHUNKS;
     double* H = s->gm_dev;   // column i at H + i (cap + 2)
     double* cs = H + (size_t)(cap + 1) * (cap + 2);
     double* sn = cs + (cap + 2);
@@ -1345,23 +1358,34 @@ static int gmres_run_dev(dep_sys* s, MV matvec, const double* b, double* d, doub
     DEP_HIP(c, hipMemcpyAsync(V, b, sizeof(double) * n2, hipMemcpyDeviceToDevice, c->stream));
     DEP_HIP(c, launch_dots(c, 1, n2, V, V, h1, part));
     DEP_HIP(c, launch_gmres_init(c, h1, tol_rel, g, sc, flags));
-    DEP_HIP(c, launch_scale_dev(c, n2, V, sc, flags));
+    if (!mv_scales) DEP_HIP(c, launch_scale_dev(c, n2, V, sc, flags));
+    const bool fused = cgs_fits(n2);
     constexpr int BATCH = 8;
     int hf[3] = {0, 0, 1};
+    size_t mark[BATCH];   // the products' timing records (steps past the stop are dropped)
     for (int i0 = 0; i0 < M; i0 += BATCH) {
         for (int i = i0; i < std::min(M, i0 + BATCH); i++) {
             double* vi = V + (size_t)i * n2;
             double* wv = V + (size_t)(i + 1) * n2;
-            DEP_HIP(c, matvec(vi, wv, flags));
+            mark[i - i0] = prof_mark(c);
+            DEP_HIP(c, matvec(vi, wv, flags, mv_scales ? sc : nullptr));
             DEP_HIP(c, launch_dot_partials(c, i + 2, n2, V, wv, part, flags));
-            DEP_HIP(c, launch_update_fused(c, i + 1, n2, V, part, wv, h1, nullptr, flags));
-            DEP_HIP(c, launch_dot_partials(c, i + 2, n2, V, wv, part, flags));
-            DEP_HIP(c, launch_update_fused(c, i + 1, n2, V, part, wv, h2, np, flags));
-            DEP_HIP(c, launch_gmres_givens(c, i, cap, h1, h2, np, H, cs, sn, g, err, sc, flags));
-            DEP_HIP(c, launch_scale_dev(c, n2, wv, sc, flags));
+            if (fused) {
+                DEP_HIP(c, launch_cgs_step(c, i, cap, n2, V, part, part2, h1, np, H, cs, sn, g, err, sc, flags));
+            } else {
+                DEP_HIP(c, launch_update_fused(c, i + 1, n2, V, part, wv, h1, nullptr, flags));
+                DEP_HIP(c, launch_dot_partials(c, i + 2, n2, V, wv, part, flags));
+                DEP_HIP(c, launch_update_fused(c, i + 1, n2, V, part, wv, h2, np, flags));
+                DEP_HIP(c, launch_gmres_givens(c, i, cap, h1, h2, np, H, cs, sn, g, err, sc, flags));
+            }
+            if (!mv_scales) DEP_HIP(c, launch_scale_dev(c, n2, wv, sc, flags));
         }
         DEP_CHECK(download(c, hf, flags, 3));
-        if (hf[0]) break;
+        if (hf[0]) {
+            if (c->prof)
+                for (int i = std::max(i0, hf[1]); i < std::min(M, i0 + BATCH); i++) prof_scale(c, mark[i - i0], 0.0);
+            break;
+        }
     }
     const int iters = hf[1], rc = hf[0] ? hf[2] : 1;
     if (rc != 2 && iters > 0) {
@@ -1513,11 +1537,12 @@ extern "C" int dep_gmres_ir(dep_sys* s, double tol_rel, double inner_tol, int ma
     }
     dep_blist one;
     one.n = 1;
-    auto matvec32 = [&](const double* in, double* out, const int* stop) -> hipError_t {
+    // block-Jacobi: M^-1 v goes to the product in FP32 (w holds 2N floats)
+    auto matvec32 = [&](double* in, double* out, const int* stop, const double* scale) -> hipError_t {
         if (!block) return launch_zmatvec32_w(s, gw, gwi, gsh, in, nullptr, out, stop);
-        hipError_t e = launch_bj_apply(c, N, s->bjB, one, in, 0, w, 0, 0, stop);
+        hipError_t e = launch_bj_apply_f32(c, N, s->bjB, in, (float*)w, stop, scale);
         if (e != hipSuccess) return e;
-        return launch_zmatvec32_w(s, s->wr, s->wi, s->shift, w, nullptr, out, stop);
+        return launch_zmatvec32_xf(s, (const float*)w, out, stop);
     };
     double bn = 0.0, res = 0.0;
     DEP_CHECK(dev_norm(s, s->b, &bn));
@@ -1541,7 +1566,7 @@ extern "C" int dep_gmres_ir(dep_sys* s, double tol_rel, double inner_tol, int ma
             rhs = rs;
         }
         GmresRun run;
-        DEP_CHECK(gmres_run_dev(s, matvec32, rhs, d, inner_tol, max_inner, &run));
+        DEP_CHECK(gmres_run_dev(s, matvec32, block, rhs, d, inner_tol, max_inner, &run));
         inner_total += run.iters;
         if (run.iters == 0) stalled = true;
         if (block) {

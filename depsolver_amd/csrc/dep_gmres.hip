@@ -151,44 +151,65 @@ __global__ void k_gmres_init(const double* __restrict__ h, double tol_rel, doubl
     flags[0] = beta < tol_rel * beta || !(beta > 0.0) ? 1 : 0;
     flags[1] = 0;
     flags[2] = flags[0] ? 0 : 1;
+    flags[3] = 0;   // workgroups done in k_cgs_last
 }
 
 // Step i: the Hessenberg column from the two CGS passes (h1 + h2, |w| from the
 // norm partials), the previous rotations, the new rotation and the residual
 // estimate; the arithmetic and its order are those of gmres_el.c:196-241 as
-// restated on the host in gmres_run.
-__global__ void k_gmres_givens(int i, int cap, const double* __restrict__ h1, const double* __restrict__ h2,
-                               const double* __restrict__ np, double* __restrict__ H, double* __restrict__ cs,
-                               double* __restrict__ sn, double* __restrict__ g, double* __restrict__ err,
-                               double* __restrict__ sc, int* __restrict__ flags)
+// restated on the host in gmres_run.  Run by one whole workgroup: the column
+// and the rotations are staged in LDS (lds: 3 (i + 2) doubles), so the
+// sequential rotation loop of thread 0 waits on no global load.  hn2 is |w|^2
+// after the second pass and is read by thread 0 only.
+__device__ void givens_step(int i, int cap, const double* h1, const double* h2, double hn2,
+                            double* __restrict__ H, double* __restrict__ cs, double* __restrict__ sn,
+                            double* __restrict__ g, double* __restrict__ err, double* __restrict__ sc,
+                            int* __restrict__ flags, double* lds)
 {
-    if (flags[0]) return;
     double* col = H + (long)i * (cap + 2);
-    for (int j = threadIdx.x; j <= i; j += blockDim.x) col[j] = h1[j] + h2[j];
+    double* lc = lds;               // column, i + 2
+    double* lcs = lds + (i + 2);    // rotations, i
+    double* lsn = lcs + (i + 2);
+    for (int j = threadIdx.x; j <= i; j += blockDim.x) {
+        lc[j] = h1[j] + h2[j];
+        if (j < i) {
+            lcs[j] = cs[j];
+            lsn[j] = sn[j];
+        }
+    }
     __syncthreads();
     if (threadIdx.x != 0) return;
     const double wn = sqrt(h1[i + 1]);
-    double hn2 = 0.0;
-    for (int k = 0; k < DEP_UPD_BLOCKS; k++) hn2 += np[k];
     const double hn = sqrt(hn2);
-    col[i + 1] = hn;
+    const double gi = g[i], err0 = sc[1], tol = sc[2];
+    lc[i + 1] = hn;
     sc[0] = hn != 0.0 ? 1.0 / hn : 0.0;
+    // rotation j mixes entries j and j + 1; the new entry j + 1 is carried in
+    // a register into rotation j + 1
+    double cur = lc[0];
     for (int j = 0; j <= i - 1; j++) {
-        const double g1 = cs[j] * col[j] + sn[j] * col[j + 1];
-        const double g2 = -sn[j] * col[j] + cs[j] * col[j + 1];
-        col[j] = g1;
-        col[j + 1] = g2;
+        const double nxt = lc[j + 1];
+        const double g1 = lcs[j] * cur + lsn[j] * nxt;
+        cur = -lsn[j] * cur + lcs[j] * nxt;
+        lc[j] = g1;
     }
-    const double htmp = col[i + 1] / col[i];
+    lc[i] = cur;
+    const double htmp = lc[i + 1] / lc[i];
+    double gnext = g[i + 1], gcur = gi;
     if (wn != 0.0) {
-        cs[i] = 1.0 / sqrt(1.0 + htmp * htmp);
-        sn[i] = cs[i] * htmp;
-        col[i] = cs[i] * col[i] + sn[i] * col[i + 1];
-        col[i + 1] = 0.0;
-        g[i + 1] = -sn[i] * g[i];
-        g[i] = cs[i] * g[i];
+        const double c = 1.0 / sqrt(1.0 + htmp * htmp);
+        const double sv = c * htmp;
+        cs[i] = c;
+        sn[i] = sv;
+        lc[i] = c * lc[i] + sv * lc[i + 1];
+        lc[i + 1] = 0.0;
+        gnext = -sv * gi;
+        gcur = c * gi;
+        g[i + 1] = gnext;
+        g[i] = gcur;
     }
-    const double err1 = fabs(g[i + 1]), err0 = sc[1], tol = sc[2];
+    for (int j = 0; j <= i + 1; j++) col[j] = lc[j];
+    const double err1 = fabs(gnext);
     err[i] = err1;
     flags[1] = i + 1;
     if (err1 < tol) {
@@ -200,6 +221,177 @@ __global__ void k_gmres_givens(int i, int cap, const double* __restrict__ h1, co
     } else {
         sc[1] = err1;
     }
+}
+
+__global__ void k_gmres_givens(int i, int cap, const double* __restrict__ h1, const double* __restrict__ h2,
+                               const double* __restrict__ np, double* __restrict__ H, double* __restrict__ cs,
+                               double* __restrict__ sn, double* __restrict__ g, double* __restrict__ err,
+                               double* __restrict__ sc, int* __restrict__ flags)
+{
+    if (flags[0]) return;
+    __shared__ double lds[3 * (UPD_MAXV + 1)];
+    double hn2 = 0.0;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < DEP_UPD_BLOCKS; k++) hn2 += np[k];
+    givens_step(i, cap, h1, h2, hn2, H, cs, sn, g, err, sc, flags, lds);
+}
+
+// ---- one Krylov step's orthogonalisation in three launches -----------------
+// Pass 1's dot partials come from k_dots_partial (DOT_CHUNKS chunks).  Then
+// DEP_UPD_BLOCKS workgroups each own one contiguous chunk of w:
+//   k_cgs_mid:  h1 = reduced pass-1 partials; w -= V h1 on the chunk; the
+//               pass-2 dot partials of the chunk (V_j . w, j < nv, and |w|^2);
+//   k_cgs_last: h2 = reduced pass-2 partials; w -= V h2 on the chunk; |w|^2
+//               partial of the chunk; the workgroup that finishes last runs
+//               the Givens step (givens_step) on h1, h2 and the partials.
+// Every sum has a fixed order (chunks in index order), so the result does not
+// depend on scheduling.  A chunk holds at most CGS_CHUNK elements.
+#define CGS_CHUNK 2048
+
+__device__ inline void cgs_chunk(long n2, long* e0, int* len)
+{
+    const long l = (n2 + DEP_UPD_BLOCKS - 1) / DEP_UPD_BLOCKS;
+    *e0 = (long)blockIdx.x * l;
+    *len = (int)max(0L, min(l, n2 - *e0));
+}
+
+// h[0 .. nv] from nchunk <= 128 partials per vector (part[j * nchunk + c]):
+// each wave takes four vectors at a time, lane l adds partials l and l + 64,
+// and a shuffle tree sums the lanes.  The loads are unconditional (indices
+// clamped into the array, surplus values dropped by selects), so the eight
+// of a lane are all in flight together.
+__device__ inline void cgs_reduce(int nv, const double* __restrict__ part, int nchunk, double* h)
+{
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int c0 = min(lane, nchunk - 1), c1 = min(lane + 64, nchunk - 1);
+    const bool u0 = lane < nchunk, u1 = lane + 64 < nchunk;
+    for (int j0 = 4 * wv; j0 <= nv; j0 += 4 * nw) {
+        double s[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const long jj = min(j0 + u, nv);
+            const double a0 = part[jj * nchunk + c0], a1 = part[jj * nchunk + c1];
+            s[u] = j0 + u <= nv ? (u0 ? a0 : 0.0) + (u1 ? a1 : 0.0) : 0.0;
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+            for (int u = 0; u < 4; u++) s[u] += __shfl_xor(s[u], o);
+        if (lane == 0)
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (j0 + u <= nv) h[j0 + u] = s[u];
+    }
+    __syncthreads();
+}
+
+// v - sum_{j < nv} h[j] V_j[e] in order of j, eight loads in flight (indices
+// clamped into the basis; the surplus terms are dropped by selects)
+__device__ inline double cgs_row_update(int nv, long n2, const double* __restrict__ V, const double* h, double v,
+                                        long e)
+{
+    for (int j0 = 0; j0 < nv; j0 += 8) {
+        double a[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) a[u] = V[(long)min(j0 + u, nv - 1) * n2 + e];
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+            if (j0 + u < nv) v -= h[j0 + u] * a[u];
+    }
+    return v;
+}
+
+__global__ __launch_bounds__(256) void k_cgs_mid(int nv, long n2, const double* __restrict__ V,
+                                                 const double* __restrict__ part, double* __restrict__ w,
+                                                 double* __restrict__ hout, double* __restrict__ part2,
+                                                 const int* __restrict__ stop)
+{
+    if (*stop) return;
+    __shared__ double h[UPD_MAXV + 1];
+    __shared__ double ws[CGS_CHUNK];
+    cgs_reduce(nv, part, DOT_CHUNKS, h);
+    if (blockIdx.x == 0)
+        for (int j = threadIdx.x; j <= nv; j += 256) hout[j] = h[j];
+    long e0;
+    int len;
+    cgs_chunk(n2, &e0, &len);
+    for (int t = threadIdx.x; t < len; t += 256) {
+        const double v = cgs_row_update(nv, n2, V, h, w[e0 + t], e0 + t);
+        w[e0 + t] = v;
+        ws[t] = v;
+    }
+    __syncthreads();
+    // pass-2 partials: 32 groups of 8 lanes, one vector per group at a time;
+    // V_nv is w itself (its chunk is in ws)
+    const int grp = threadIdx.x >> 3, l8 = threadIdx.x & 7;
+    for (int j = grp; j <= nv; j += 32) {
+        const double* vj = V + (long)min(j, nv - 1) * n2 + e0;
+        double sum = 0.0;
+        for (int t0 = 0; t0 < len; t0 += 64) {   // 8 loads of the lane in flight
+            double a[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) a[u] = vj[min(t0 + 8 * u + l8, len - 1)];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int t = t0 + 8 * u + l8;
+                const double x = t < len ? ws[t] : 0.0;
+                sum += (j < nv ? a[u] : x) * x;
+            }
+        }
+        sum += __shfl_xor(sum, 1);
+        sum += __shfl_xor(sum, 2);
+        sum += __shfl_xor(sum, 4);
+        if (l8 == 0) part2[(long)j * DEP_UPD_BLOCKS + blockIdx.x] = sum;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_cgs_last(int i, int cap, long n2, const double* __restrict__ V,
+                                                  const double* __restrict__ part2, double* __restrict__ w,
+                                                  const double* __restrict__ h1, double* __restrict__ np,
+                                                  double* __restrict__ H, double* __restrict__ cs,
+                                                  double* __restrict__ sn, double* __restrict__ g,
+                                                  double* __restrict__ err, double* __restrict__ sc,
+                                                  int* __restrict__ flags)
+{
+    // flags[0] changes only in the last workgroup, after every workgroup has
+    // counted itself in, so all workgroups see the same value here
+    if (flags[0]) return;
+    const int nv = i + 1;
+    __shared__ double h[UPD_MAXV + 1];
+    __shared__ double ws[3 * (UPD_MAXV + 1)];   // the Givens step's staging
+    __shared__ double red[4];
+    __shared__ int last;
+    cgs_reduce(nv, part2, DEP_UPD_BLOCKS, h);
+    long e0;
+    int len;
+    cgs_chunk(n2, &e0, &len);
+    double sq = 0.0;
+    for (int t = threadIdx.x; t < len; t += 256) {
+        const double v = cgs_row_update(nv, n2, V, h, w[e0 + t], e0 + t);
+        w[e0 + t] = v;
+        sq += v * v;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) sq += __shfl_xor(sq, o);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = sq;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __hip_atomic_store(np + blockIdx.x, (red[0] + red[1]) + (red[2] + red[3]), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();   // the partial is visible before the count
+        const unsigned done = atomicAdd((unsigned*)(flags + 3), 1u);
+        last = done == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    double hn2 = 0.0;
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < DEP_UPD_BLOCKS; k++)
+            hn2 += __hip_atomic_load(np + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        flags[3] = 0;   // the count starts from zero at the next step
+    }
+    givens_step(i, cap, h1, h, hn2, H, cs, sn, g, err, sc, flags, ws);
 }
 
 // w *= sc[0] unless stopped or sc[0] == 0
@@ -327,6 +519,21 @@ hipError_t launch_gmres_givens(dep_ctx* c, int i, int cap, const double* h1, con
 {
     hipLaunchKernelGGL(k_gmres_givens, dim3(1), dim3(64), 0, c->stream, i, cap, h1, h2, np, H, cs, sn, g, err, sc,
                        flags);
+    return hipGetLastError();
+}
+
+bool cgs_fits(long n2) { return (n2 + DEP_UPD_BLOCKS - 1) / DEP_UPD_BLOCKS <= CGS_CHUNK; }
+
+hipError_t launch_cgs_step(dep_ctx* c, int i, int cap, long n2, double* V, const double* part, double* part2,
+                           double* h1, double* np, double* H, double* cs, double* sn, double* g, double* err,
+                           double* sc, int* flags)
+{
+    if (i + 1 > UPD_MAXV || !cgs_fits(n2)) return hipErrorInvalidValue;
+    double* w = V + (long)(i + 1) * n2;
+    hipLaunchKernelGGL(k_cgs_mid, dim3(DEP_UPD_BLOCKS), dim3(256), 0, c->stream, i + 1, n2, V, part, w, h1, part2,
+                       flags);
+    hipLaunchKernelGGL(k_cgs_last, dim3(DEP_UPD_BLOCKS), dim3(256), 0, c->stream, i, cap, n2, V, part2, w, h1, np, H,
+                       cs, sn, g, err, sc, flags);
     return hipGetLastError();
 }
 

@@ -48,6 +48,7 @@ struct dep_ctx {
         std::string name;
         hipEvent_t a, b;
         double flops, bytes;
+        double scale = 1.0;   // share of the flops that were useful; 0: not counted
     };
     std::vector<Pending> pending;
     std::vector<hipEvent_t> pool;
@@ -62,6 +63,13 @@ struct dep_ctx {
 void prof_begin(dep_ctx* c, hipEvent_t* a, hipStream_t st = nullptr);
 void prof_end(dep_ctx* c, hipEvent_t a, const char* name, double flops, double bytes, hipStream_t st = nullptr);
 void prof_collect(dep_ctx* c);   // after a synchronize: fold finished events into stats
+// The record the next prof_end makes (profiling on) is prof_mark's index; a
+// caller that learns later how much of that launch was useful scales its
+// algorithmic flops by f (a batched product some of whose systems had already
+// stopped), or drops the launch from the statistics with f = 0 (a step
+// enqueued past the end of a device-driven iteration returns at once).
+size_t prof_mark(const dep_ctx* c);
+void prof_scale(dep_ctx* c, size_t idx, double f);
 
 struct dep_geom {
     dep_ctx* ctx = nullptr;
@@ -197,6 +205,8 @@ hipError_t launch_rowscale(dep_ctx* c, int N, const double* d, const double* in,
 // y = Z x (b == nullptr) or b - Z x with the FP32 blocks P32 and the given weights
 hipError_t launch_zmatvec32_w(dep_sys* s, const double* wr, const double* wi, const double* shift,
                               const double* x, const double* b, double* y, const int* stop = nullptr);
+// y = Z x with the FP32 blocks, the system's weights and x in FP32 (split layout)
+hipError_t launch_zmatvec32_xf(dep_sys* s, const float* x, double* y, const int* stop);
 hipError_t launch_update(dep_ctx* c, int nvec, long n2, const double* V, const double* h, double* w);
 // partials only (DEP_DOT_CHUNKS per vector), consumed by launch_update_fused
 hipError_t launch_dot_partials(dep_ctx* c, int nvec, long n2, const double* V, const double* w, double* part,
@@ -211,6 +221,15 @@ hipError_t launch_gmres_init(dep_ctx* c, const double* h, double tol_rel, double
 hipError_t launch_gmres_givens(dep_ctx* c, int i, int cap, const double* h1, const double* h2, const double* np,
                                double* H, double* cs, double* sn, double* g, double* err, double* sc, int* flags);
 hipError_t launch_scale_dev(dep_ctx* c, long n2, double* w, const double* sc, const int* stop);
+// One Krylov step's orthogonalisation after the pass-1 dot partials (part,
+// DEP_DOT_CHUNKS per vector, of V_0 .. V_{i+1} against w = V_{i+1}): the two
+// CGS updates of w, the pass-2 partials in part2 ((i + 2) * DEP_UPD_BLOCKS),
+// h1 (i + 2), the norm partials np and the Givens step (flags[3]: workgroup
+// count, zeroed by launch_gmres_init).  cgs_fits: whether n2 suits it.
+bool cgs_fits(long n2);
+hipError_t launch_cgs_step(dep_ctx* c, int i, int cap, long n2, double* V, const double* part, double* part2,
+                           double* h1, double* np, double* H, double* cs, double* sn, double* g, double* err,
+                           double* sc, int* flags);
 hipError_t launch_scale(dep_ctx* c, long n2, const double* w, double alpha, double* out);
 hipError_t launch_combine(dep_ctx* c, int nvec, long n2, const double* V, const double* y, double* x);
 size_t mixed_bytes(int N);
@@ -274,5 +293,10 @@ hipError_t launch_bj_build(dep_ctx* c, int N, const double* P, long ldP, const d
                            const double* sh, long wstride, int nsys, double* scratch, float* B);
 // Y_j = M_j^-1 X_j (accumulate: Y_j += M_j^-1 X_j) for the systems of L
 // stop: system sys is skipped when stop[sys * sstride] != 0
-hipError_t launch_bj_apply(dep_ctx* c, int N, const float* B, const dep_blist& L, const double* X, long xstride,
-                           double* Y, long ystride, int accumulate = 0, const int* stop = nullptr, int sstride = 0);
+// scale (one system only): X is first scaled in place by *scale unless it is 0
+hipError_t launch_bj_apply(dep_ctx* c, int N, const float* B, const dep_blist& L, double* X, long xstride,
+                           double* Y, long ystride, int accumulate = 0, const int* stop = nullptr, int sstride = 0,
+                           const double* scale = nullptr);
+// one system: Y (FP32) = M^-1 X, X first scaled in place by *scale as above
+hipError_t launch_bj_apply_f32(dep_ctx* c, int N, const float* B, double* X, float* Y, const int* stop,
+                               const double* scale);

@@ -83,22 +83,36 @@ __global__ __launch_bounds__(256) void k_bj_combine(int nblk, const double* __re
 // of L (x, y split: re block, im block).
 // Thread t: row t / 4 of the block, columns 16 (t % 4) .. + 15; the four
 // partial sums of a row are combined by lane shuffles.
+// With scale (one system), the block's rows of X are first multiplied by
+// *scale (unless it is 0) and written back: the normalisation of a Krylov
+// vector folded into its first use.
+template <class TY>
 __global__ __launch_bounds__(256) void k_bj_apply(int N, int nblk, const float* __restrict__ B, dep_blist L,
-                                                  const double* __restrict__ X, long xstride, double* __restrict__ Y,
+                                                  double* __restrict__ X, long xstride, TY* __restrict__ Y,
                                                   long ystride, int accumulate, const int* __restrict__ stop,
-                                                  int sstride)
+                                                  int sstride, const double* __restrict__ scale)
 {
     __shared__ double xr[BJ], xi[BJ];
     const int b = blockIdx.x;
     const long sys = L.id[blockIdx.y];
     if (stop && stop[sys * sstride]) return;
-    const double* x = X + sys * xstride;
-    double* y = Y + sys * ystride;
+    double* x = X + sys * xstride;
+    TY* y = Y + sys * ystride;
     const int t = threadIdx.x, r = t >> 2, q = t & 3;
     if (t < BJ) {
         const int j = b * BJ + t;
-        xr[t] = j < N ? x[j] : 0.0;
-        xi[t] = j < N ? x[N + j] : 0.0;
+        double vr = j < N ? x[j] : 0.0, vi = j < N ? x[N + j] : 0.0;
+        if (scale) {
+            const double a = *scale;
+            if (a != 0.0 && j < N) {
+                vr *= a;
+                vi *= a;
+                x[j] = vr;
+                x[N + j] = vi;
+            }
+        }
+        xr[t] = vr;
+        xi[t] = vi;
     }
     __syncthreads();
     const float* br = B + (sys * nblk + b) * 2 * BJ * BJ + r * BJ + 16 * q;
@@ -117,11 +131,11 @@ __global__ __launch_bounds__(256) void k_bj_apply(int N, int nblk, const float* 
     const int i = b * BJ + r;
     if (q == 0 && i < N) {
         if (accumulate) {
-            y[i] += sr;
-            y[N + i] += si;
+            y[i] += (TY)sr;
+            y[N + i] += (TY)si;
         } else {
-            y[i] = sr;
-            y[N + i] = si;
+            y[i] = (TY)sr;
+            y[N + i] = (TY)si;
         }
     }
 }
@@ -148,11 +162,23 @@ hipError_t launch_bj_build(dep_ctx* c, int N, const double* P, long ldP, const d
     return hipGetLastError();
 }
 
-hipError_t launch_bj_apply(dep_ctx* c, int N, const float* B, const dep_blist& L, const double* X, long xstride,
-                           double* Y, long ystride, int accumulate, const int* stop, int sstride)
+hipError_t launch_bj_apply(dep_ctx* c, int N, const float* B, const dep_blist& L, double* X, long xstride,
+                           double* Y, long ystride, int accumulate, const int* stop, int sstride, const double* scale)
 {
     const int nblk = (int)bj_blocks(N);
-    hipLaunchKernelGGL(k_bj_apply, dim3(nblk, L.n), dim3(256), 0, c->stream, N, nblk, B, L, X, xstride, Y, ystride,
-                       accumulate, stop, sstride);
+    if (scale && L.n != 1) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_bj_apply<double>, dim3(nblk, L.n), dim3(256), 0, c->stream, N, nblk, B, L, X, xstride, Y,
+                       ystride, accumulate, stop, sstride, scale);
+    return hipGetLastError();
+}
+
+hipError_t launch_bj_apply_f32(dep_ctx* c, int N, const float* B, double* X, float* Y, const int* stop,
+                               const double* scale)
+{
+    const int nblk = (int)bj_blocks(N);
+    dep_blist one;
+    one.n = 1;
+    hipLaunchKernelGGL(k_bj_apply<float>, dim3(nblk, 1), dim3(256), 0, c->stream, N, nblk, B, one, X, 0L, Y, 0L, 0,
+                       stop, 0, scale);
     return hipGetLastError();
 }

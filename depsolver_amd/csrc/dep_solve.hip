@@ -8,6 +8,10 @@
 #include "dep_common.h"
 #include "dep_internal.h"
 
+// rows per wave and chunks in flight of the FP32-x product
+#define ZMV32_ROWS 4
+#define ZMV32_UNROLL 2
+
 namespace {
 
 // Z = w * P + shift on the diagonal (FP64 split planes)
@@ -26,27 +30,31 @@ __global__ void k_weight(int N, const double* __restrict__ P, long ldP, const do
 
 // ---- products with P -------------------------------------------------------
 // y = Z x (or b - Z x) with Z = diag(w) P + diag(shift), P real (FP64 or FP32
-// storage, FP64 arithmetic).  Each wave (WPB per workgroup) owns ZROWS rows
+// storage) and x in FP64 or FP32, FP64 arithmetic (a product of two FP32
+// values is exact in FP64).  Each wave (WPB per workgroup) owns ZROWS rows
 // and walks their columns in chunks of 256 (4 consecutive columns per lane,
 // vector loads), UNR chunks in flight, so every chunk of x is read once per
 // ZROWS rows; with NT, P is streamed with non-temporal loads (it is read once
 // per product and would only evict x); a scalar loop finishes the columns past
 // the last full group of chunks.  Shapes measured at N = 4,740 / 9,480 /
-// 15,044: FP32 blocks 6 rows, 2 chunks, one wave, NT (23 / 67 / 148 us, 6.1
-// TB/s at the largest; 4 rows, 1 chunk, 4 waves: 28 / 87 / 192 us).  The
-// FP64 blocks use k_zmatvec64 below.
-template <class TP, int ZROWS, int UNR, int WPB, bool NT>
+// 15,044: FP32 blocks and FP64 x 6 rows, 2 chunks, one wave, NT (23 / 67 /
+// 148 us, 6.1 TB/s at the largest; 4 rows, 1 chunk, 4 waves: 28 / 87 / 192
+// us).  An FP32 x halves the x stream that every wave reads from L2: 4 rows,
+// 2 chunks reach 6.2 TB/s where the FP64 x form reaches 5.1-5.2 in the same
+// back-to-back timing (a plain read of the blocks: 6.9 TB/s).  The FP64
+// blocks use k_zmatvec64 below.
+template <class TP, class TX, int ZROWS, int UNR, int WPB, bool NT>
 __global__ __launch_bounds__(64 * WPB) void k_zmatvec(int N, const TP* __restrict__ P, long ldP,
                                                 const double* __restrict__ wr,
                                                 const double* __restrict__ wi,
                                                 const double* __restrict__ shift,
-                                                const double* __restrict__ x,
+                                                const TX* __restrict__ x,
                                                 const double* __restrict__ b,
                                                 double* __restrict__ y, const int* __restrict__ stop)
 {
     if (stop && *stop) return;   // a step past the end of a device-driven GMRES run
     typedef TP p4 __attribute__((ext_vector_type(4)));
-    typedef double d4 __attribute__((ext_vector_type(4)));
+    typedef TX d4 __attribute__((ext_vector_type(4)));
     const int lane = threadIdx.x & 63;
     const int row0 = (blockIdx.x * WPB + (threadIdx.x >> 6)) * ZROWS;
     if (row0 >= N) return;
@@ -75,8 +83,8 @@ __global__ __launch_bounds__(64 * WPB) void k_zmatvec(int N, const TP* __restric
             for (int k = 0; k < ZROWS; k++)
 #pragma unroll
                 for (int e = 0; e < 4; e++) {
-                    sr[k] += (double)v[u][k][e] * xr[u][e];
-                    si[k] += (double)v[u][k][e] * xi[u][e];
+                    sr[k] += (double)v[u][k][e] * (double)xr[u][e];
+                    si[k] += (double)v[u][k][e] * (double)xi[u][e];
                 }
     }
     for (int c = full + lane; c < N; c += 64) {
@@ -106,8 +114,8 @@ __global__ __launch_bounds__(64 * WPB) void k_zmatvec(int N, const TP* __restric
             }
         const int row = row0 + lane;
         const double a = wr[row], bb = wi[row], sh = shift[row];
-        double yr = a * zr - bb * zi + sh * x[row];
-        double yi = a * zi + bb * zr + sh * x[N + row];
+        double yr = a * zr - bb * zi + sh * (double)x[row];
+        double yi = a * zi + bb * zr + sh * (double)x[N + row];
         if (b) {   // residual b - Z x
             yr = b[row] - yr;
             yi = b[N + row] - yi;
@@ -225,8 +233,21 @@ hipError_t launch_zmatvec32_w(dep_sys* s, const double* wr, const double* wi, co
     dep_geom* g = s->g;
     hipEvent_t ev;
     prof_begin(g->ctx, &ev);
-    hipLaunchKernelGGL((k_zmatvec<float, 6, 2, 1, true>), dim3(dep_cdiv(g->N, 6)), dim3(64), 0, g->ctx->stream, g->N,
-                       g->P32, g->ldP, wr, wi, shift, x, b, y, stop);
+    hipLaunchKernelGGL((k_zmatvec<float, double, 6, 2, 1, true>), dim3(dep_cdiv(g->N, 6)), dim3(64), 0,
+                       g->ctx->stream, g->N, g->P32, g->ldP, wr, wi, shift, x, b, y, stop);
+    const double n = g->N;
+    prof_end(g->ctx, ev, "zmatvec32", 4.0 * n * n, 4.0 * n * n + 56.0 * n);
+    return hipGetLastError();
+}
+
+hipError_t launch_zmatvec32_xf(dep_sys* s, const float* x, double* y, const int* stop)
+{
+    dep_geom* g = s->g;
+    hipEvent_t ev;
+    prof_begin(g->ctx, &ev);
+    constexpr int ZR = ZMV32_ROWS, UN = ZMV32_UNROLL;
+    hipLaunchKernelGGL((k_zmatvec<float, float, ZR, UN, 1, true>), dim3(dep_cdiv(g->N, ZR)), dim3(64), 0,
+                       g->ctx->stream, g->N, g->P32, g->ldP, s->wr, s->wi, s->shift, x, nullptr, y, stop);
     const double n = g->N;
     prof_end(g->ctx, ev, "zmatvec32", 4.0 * n * n, 4.0 * n * n + 48.0 * n);
     return hipGetLastError();

@@ -254,8 +254,10 @@ extern "C" int dep_sweep_solve(dep_sweep* w, double tol, double inner_tol, int m
         int* pf[2] = {(int*)w->hpin, (int*)w->hpin + 4 * nf};
         dep_blist I = A;
         int i = 0, nb = 0;
+        std::vector<std::pair<size_t, dep_blist>> marks;   // each product's timing record and systems
         while (I.n > 0 && i < M) {
             for (const int iend = std::min(M, i + BATCH); i < iend; i++) {
+                if (c->prof) marks.push_back({prof_mark(c), I});
                 if (w->block) {   // V_j[i+1] = Z_j M_j^-1 V_j[i], M_j^-1 V_j[i] staged in R (free inside a pass)
                     DEP_HIP(c, launch_bj_apply(c, N, w->bjB, I, w->V + (long)i * n2, vstride, w->R, n2, 0, w->gfl, 4));
                     DEP_HIP(c, launch_bmatvec32(c, N, g->P32, g->ldP, I, w->R, n2, w->part, w->W, 3L * N,
@@ -293,6 +295,12 @@ extern "C" int dep_sweep_solve(dep_sweep* w, double tol, double inner_tol, int m
             w->iters[j] += steps;
             w->outer[j]++;
             if (steps == 0) stalled[j] = 1;
+        }
+        for (size_t k = 0; k < marks.size(); k++) {   // count the systems each product advanced
+            const dep_blist& Lk = marks[k].second;
+            int live = 0;
+            for (int a = 0; a < Lk.n; a++) live += (int)k < hfl[4 * Lk.id[a] + 1];
+            prof_scale(c, marks[k].first, (double)live / Lk.n);
         }
         if (w->block) {   // x_j += M_j^-1 (V_j y_j), V_j y_j gathered in R
             DEP_HIP(c, hipMemsetAsync(w->R, 0, sizeof(double) * nf * n2, c->stream));
