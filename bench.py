@@ -460,6 +460,7 @@ def main():
         "sweep": sweep,
         "serving": serving,
         "position_batch": batch,
+        "autotune": dep.tune_report(),
     }
     if cpu:
         out["speedup_vs_cpu"] = value / cpu["value"]

@@ -111,6 +111,10 @@ def lib():
         "dep_server_stats": ([_p, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_size_t),
                               ctypes.POINTER(ctypes.c_size_t)], i),
         "dep_server_destroy": ([_p], i),
+        "dep_tune_set_mode": ([i], i),
+        "dep_tune_pin": ([i, ctypes.c_char_p, i, i], i),
+        "dep_tune_reset": ([], i),
+        "dep_tune_report": ([ctypes.c_char_p, ctypes.c_size_t], i),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -122,6 +126,33 @@ def lib():
 
 def _c64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
+
+
+TUNE_KNOBS = ("zmatvec32", "zmatvec64", "assemble")
+
+
+def tune_mode(mode):
+    """1: tune launch shapes on the first launch of each size class (default); 0: first shape."""
+    if lib().dep_tune_set_mode(int(mode)) != DEP_OK:
+        raise ValueError(f"tune mode {mode}")
+
+
+def tune_pin(knob, n, candidate, device=0):
+    """Fix the launch shape (0-based candidate) of a knob for n's size class."""
+    if lib().dep_tune_pin(device, knob.encode(), int(n), int(candidate)) != DEP_OK:
+        raise ValueError(f"tune_pin({knob!r}, {n}, {candidate})")
+
+
+def tune_reset():
+    """Forget every tuned choice."""
+    lib().dep_tune_reset()
+
+
+def tune_report():
+    """The tuned choices, one line per decided size class."""
+    buf = ctypes.create_string_buffer(1 << 16)
+    lib().dep_tune_report(buf, len(buf))
+    return [line for line in buf.value.decode().splitlines() if line]
 
 
 class Context:

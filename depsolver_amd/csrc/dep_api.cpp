@@ -259,7 +259,7 @@ static uint64_t spread3(uint64_t v)
 }
 
 // Morton order of the node coordinates: consecutive internal ids are close in
-// space, so a column patch of DEP_TC ids touches few elements.
+// space, so a column patch of consecutive ids touches few elements.
 static void morton_order(int N, const double* xyz, std::vector<int>& perm)
 {
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
@@ -284,6 +284,57 @@ static void morton_order(int N, const double* xyz, std::vector<int>& perm)
     perm.resize(N);
     std::iota(perm.begin(), perm.end(), 0);
     std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return code[a] < code[b]; });
+}
+
+// Column patches of one assembly shape: patch p covers the tc consecutive
+// (internal-order) nodes p tc .. p tc + tc - 1 and lists, in element order,
+// the elements with a node among them, each with the patch-local column of
+// every element position (-1 outside the patch; 8 bytes per element so the
+// kernel reads them in one load).
+static int ensure_patches(dep_geom* g, int shape)
+{
+    dep_geom::Patches& ps = g->patches[shape];
+    if (ps.ptr) return DEP_OK;
+    const int N = g->N, E = g->E, npe = g->npe, tc = assemble_tc(shape);
+    const std::vector<int>& cnt = g->hinc_ptr;
+    const std::vector<int>& inc_ek = g->hinc_ek;
+    const int np = dep_cdiv(N, tc);
+    std::vector<int> pptr(np + 1, 0), pel;
+    std::vector<int8_t> pslot;
+    std::vector<int> mark(E, -1);
+    for (int p = 0; p < np; p++) {
+        const int c0 = p * tc, c1 = std::min(N, c0 + tc);
+        std::vector<int> els;
+        for (int v = c0; v < c1; v++)
+            for (int t = cnt[v]; t < cnt[v + 1]; t++) {
+                const int e = inc_ek[t] >> 3;
+                if (mark[e] != p) {
+                    mark[e] = p;
+                    els.push_back(e);
+                }
+            }
+        std::sort(els.begin(), els.end());
+        for (int e : els) {
+            pel.push_back(e);
+            for (int k = 0; k < 8; k++) {
+                const int v = k < npe ? g->hconn[(size_t)e * npe + k] : -1;
+                pslot.push_back((k < npe && v >= c0 && v < c1) ? (int8_t)(v - c0) : (int8_t)-1);
+            }
+        }
+        pptr[p + 1] = (int)pel.size();
+    }
+    dep_ctx* c = g->ctx;
+    dep_geom::Patches q;
+    q.tc = tc;
+    q.n = np;
+    DEP_CHECK(dep_obj_alloc(g, &q.ptr, pptr.size()));
+    DEP_CHECK(dep_obj_alloc(g, &q.elem, pel.size()));
+    DEP_CHECK(dep_obj_alloc(g, &q.slot, pslot.size()));
+    DEP_CHECK(upload(c, q.ptr, pptr.data(), pptr.size()));
+    DEP_CHECK(upload(c, q.elem, pel.data(), pel.size()));
+    DEP_CHECK(upload(c, q.slot, pslot.data(), pslot.size()));
+    ps = q;
+    return DEP_OK;
 }
 
 extern "C" int dep_geom_create(dep_ctx* c, int N, const double* xyz, int E, int npe,
@@ -346,39 +397,6 @@ extern "C" int dep_geom_create(dep_ctx* c, int N, const double* xyz, int E, int 
                 if (first) inc_ek[fill[v]++] = e * 8 + k;
             }
     }
-    // column patches of tc consecutive (Morton-ordered) nodes
-    {
-        const char* tsel = getenv("DEPSOLVER_ASM_TC");
-        const int tc = tsel ? atoi(tsel) : DEP_TC;
-        g->tc = (tc == 4 || tc == 8 || tc == 16 || tc == 32 || tc == 64) ? tc : DEP_TC;
-    }
-    g->n_patch = dep_cdiv(N, g->tc);
-    std::vector<int> pptr(g->n_patch + 1, 0), pel;
-    std::vector<int8_t> pslot;
-    {
-        std::vector<int> mark(E, -1);
-        for (int p = 0; p < g->n_patch; p++) {
-            const int c0 = p * g->tc, c1 = std::min(N, c0 + g->tc);
-            std::vector<int> els;
-            for (int v = c0; v < c1; v++)
-                for (int t = cnt[v]; t < cnt[v + 1]; t++) {
-                    const int e = inc_ek[t] >> 3;
-                    if (mark[e] != p) {
-                        mark[e] = p;
-                        els.push_back(e);
-                    }
-                }
-            std::sort(els.begin(), els.end());
-            for (int e : els) {
-                pel.push_back(e);
-                for (int k = 0; k < 8; k++) {   // 8 bytes per element: one aligned load in the kernel
-                    const int v = k < npe ? g->hconn[(size_t)e * npe + k] : -1;
-                    pslot.push_back((k < npe && v >= c0 && v < c1) ? (int8_t)(v - c0) : (int8_t)-1);
-                }
-            }
-            pptr[p + 1] = (int)pel.size();
-        }
-    }
     int r = DEP_OK;
 #define GA(ptr, n)                                  \
     if (r == DEP_OK) r = dep_obj_alloc(g, &(ptr), (n));
@@ -389,18 +407,13 @@ extern "C" int dep_geom_create(dep_ctx* c, int N, const double* xyz, int E, int 
     GA(g->q, (size_t)E * 7 * 4);
     GA(g->inc_ptr, (size_t)N + 1);
     GA(g->inc_ek, inc_ek.size());
-    GA(g->patch_ptr, pptr.size());
-    GA(g->patch_elem, pel.size());
-    GA(g->patch_slot, pslot.size());
 #undef GA
     if (r == DEP_OK) r = upload(c, g->xyz, g->hxyz.data(), g->hxyz.size());
     if (r == DEP_OK) r = upload(c, g->conn, g->hconn.data(), g->hconn.size());
     if (r == DEP_OK) r = upload(c, g->kind, g->hkind.data(), g->hkind.size());
     if (r == DEP_OK) r = upload(c, g->inc_ptr, cnt.data(), cnt.size());
     if (r == DEP_OK) r = upload(c, g->inc_ek, inc_ek.data(), inc_ek.size());
-    if (r == DEP_OK) r = upload(c, g->patch_ptr, pptr.data(), pptr.size());
-    if (r == DEP_OK) r = upload(c, g->patch_elem, pel.data(), pel.size());
-    if (r == DEP_OK) r = upload(c, g->patch_slot, pslot.data(), pslot.size());
+    if (r == DEP_OK) r = ensure_patches(g, 0);
     if (r == DEP_OK) {
         hipError_t e = launch_quad_points(g);
         if (e == hipSuccess) e = launch_node_normals(g);
@@ -445,7 +458,12 @@ extern "C" int dep_geom_assemble(dep_geom* g)
         DEP_CHECK(dep_obj_alloc(g, &g->S, (size_t)g->npe * g->E * g->npe));
         DEP_HIP(c, hipMemsetAsync(g->P, 0, sizeof(double) * g->N * g->ldP, c->stream));
     }
-    DEP_HIP(c, launch_assemble(g));   // also writes the FP32 copy once one is kept
+    // the patch width is autotuned per size class (all widths give the same P)
+    if (tune_lookup(c, KNOB_ASM, g->N) < 0)
+        for (int k = 0; k < tune_ncand(KNOB_ASM); k++) DEP_CHECK(ensure_patches(g, k));
+    const int shape = tune_choose(c, KNOB_ASM, g->N, true, [&](int k) { return launch_assemble(g, k); });
+    DEP_CHECK(ensure_patches(g, shape));
+    DEP_HIP(c, launch_assemble(g, shape));   // also writes the FP32 copy once one is kept
     DEP_HIP(c, hipStreamSynchronize(c->stream));
     g->assembled = true;
     g->p32_valid = g->P32 != nullptr;
@@ -1368,7 +1386,9 @@ static int gmres_run_dev(dep_sys* s, MV matvec, bool mv_scales, const double* b,
             double* vi = V + (size_t)i * n2;
             double* wv = V + (size_t)(i + 1) * n2;
             mark[i - i0] = prof_mark(c);
-            DEP_HIP(c, matvec(vi, wv, flags, mv_scales ? sc : nullptr));
+            // the first step of a batch runs for certain (the flags read before it said so),
+            // so its product goes without the stop flag (and may time the product's shapes)
+            DEP_HIP(c, matvec(vi, wv, i == i0 ? nullptr : flags, mv_scales ? sc : nullptr));
             DEP_HIP(c, launch_dot_partials(c, i + 2, n2, V, wv, part, flags));
             if (fused) {
                 DEP_HIP(c, launch_cgs_step(c, i, cap, n2, V, part, part2, h1, np, H, cs, sn, g, err, sc, flags));
@@ -1537,12 +1557,11 @@ extern "C" int dep_gmres_ir(dep_sys* s, double tol_rel, double inner_tol, int ma
     }
     dep_blist one;
     one.n = 1;
-    // block-Jacobi: M^-1 v goes to the product in FP32 (w holds 2N floats)
     auto matvec32 = [&](double* in, double* out, const int* stop, const double* scale) -> hipError_t {
         if (!block) return launch_zmatvec32_w(s, gw, gwi, gsh, in, nullptr, out, stop);
-        hipError_t e = launch_bj_apply_f32(c, N, s->bjB, in, (float*)w, stop, scale);
+        hipError_t e = launch_bj_apply(c, N, s->bjB, one, in, 0, w, 0, 0, stop, 0, scale);
         if (e != hipSuccess) return e;
-        return launch_zmatvec32_xf(s, (const float*)w, out, stop);
+        return launch_zmatvec32_w(s, s->wr, s->wi, s->shift, w, nullptr, out, stop);
     };
     double bn = 0.0, res = 0.0;
     DEP_CHECK(dev_norm(s, s->b, &bn));

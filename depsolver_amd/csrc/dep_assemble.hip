@@ -12,7 +12,7 @@
 // geometry and reused across frequencies.
 //
 // Regular pairs: a workgroup owns TR consecutive rows (one per thread) and a
-// column patch of DEP_TC consecutive nodes.  It walks the elements touching
+// column patch of TC consecutive nodes (the patch width is autotuned).  It walks the elements touching
 // the patch (in element order), evaluates the 7-point rule for its row from
 // the precomputed quadrature points (wave-uniform loads), and accumulates
 // into an LDS tile owned row-by-row by the threads, so every sum has a fixed
@@ -227,33 +227,23 @@ __global__ void k_scatter_singular(int N, int E, const int* __restrict__ conn,
 }
 
 template <int NPE>
-hipError_t assemble_impl(dep_geom* g)
+hipError_t assemble_impl(dep_geom* g, int shape)
 {
     hipStream_t st = g->ctx->stream;
     constexpr int TR = 128;
-    dim3 grid(g->n_patch, dep_cdiv(g->N, TR));
+    const dep_geom::Patches& ps = g->patches[shape];
+    dim3 grid(ps.n, dep_cdiv(g->N, TR));
     hipEvent_t ev;
     prof_begin(g->ctx, &ev);
-    if (g->tc == 4)
-        hipLaunchKernelGGL((k_assemble_regular<NPE, TR, 4>), grid, dim3(TR), 0, st, g->N, g->xyz, g->conn,
-                           g->kind, g->normal, (const double4*)g->q, g->patch_ptr, g->patch_elem,
-                           g->patch_slot, g->P, g->ldP, g->P32);
-    else if (g->tc == 8)
-        hipLaunchKernelGGL((k_assemble_regular<NPE, TR, 8>), grid, dim3(TR), 0, st, g->N, g->xyz, g->conn,
-                           g->kind, g->normal, (const double4*)g->q, g->patch_ptr, g->patch_elem,
-                           g->patch_slot, g->P, g->ldP, g->P32);
-    else if (g->tc == 16)
-        hipLaunchKernelGGL((k_assemble_regular<NPE, TR, 16>), grid, dim3(TR), 0, st, g->N, g->xyz, g->conn,
-                           g->kind, g->normal, (const double4*)g->q, g->patch_ptr, g->patch_elem,
-                           g->patch_slot, g->P, g->ldP, g->P32);
-    else if (g->tc == 32)
-        hipLaunchKernelGGL((k_assemble_regular<NPE, TR, 32>), grid, dim3(TR), 0, st, g->N, g->xyz, g->conn,
-                           g->kind, g->normal, (const double4*)g->q, g->patch_ptr, g->patch_elem,
-                           g->patch_slot, g->P, g->ldP, g->P32);
+    if (ps.tc == 8)
+        hipLaunchKernelGGL((k_assemble_regular<NPE, TR, 8>), grid, dim3(TR), 0, st, g->N, g->xyz, g->conn, g->kind,
+                           g->normal, (const double4*)g->q, ps.ptr, ps.elem, ps.slot, g->P, g->ldP, g->P32);
+    else if (ps.tc == 32)
+        hipLaunchKernelGGL((k_assemble_regular<NPE, TR, 32>), grid, dim3(TR), 0, st, g->N, g->xyz, g->conn, g->kind,
+                           g->normal, (const double4*)g->q, ps.ptr, ps.elem, ps.slot, g->P, g->ldP, g->P32);
     else
-        hipLaunchKernelGGL((k_assemble_regular<NPE, TR, 64>), grid, dim3(TR), 0, st, g->N, g->xyz, g->conn,
-                           g->kind, g->normal, (const double4*)g->q, g->patch_ptr, g->patch_elem,
-                           g->patch_slot, g->P, g->ldP, g->P32);
+        hipLaunchKernelGGL((k_assemble_regular<NPE, TR, 16>), grid, dim3(TR), 0, st, g->N, g->xyz, g->conn, g->kind,
+                           g->normal, (const double4*)g->q, ps.ptr, ps.elem, ps.slot, g->P, g->ldP, g->P32);
     // algorithmic flops per (row, element) pair of the 7-point rule, sqrt and
     // division counted as one flop each: 7 (10 + 2 npe) for G rows, 7 (18 + 2 npe)
     // for dG/dn rows; bytes: the N x N FP64 blocks (and their FP32 copy) written once
@@ -298,7 +288,12 @@ hipError_t launch_node_normals(dep_geom* g)
     return hipGetLastError();
 }
 
-hipError_t launch_assemble(dep_geom* g)
+// shapes: columns per patch 16 (measured best at N = 15,044: 2.45 ms; 8:
+// 2.71 ms, 32: 3.27 ms, where the larger LDS tile halves occupancy), 8, 32
+int assemble_tc(int shape) { return shape == 1 ? 8 : shape == 2 ? 32 : 16; }
+
+hipError_t launch_assemble(dep_geom* g, int shape)
 {
-    return g->npe == 3 ? assemble_impl<3>(g) : assemble_impl<6>(g);
+    if (shape < 0 || shape > 2 || !g->patches[shape].ptr) return hipErrorInvalidValue;
+    return g->npe == 3 ? assemble_impl<3>(g, shape) : assemble_impl<6>(g, shape);
 }

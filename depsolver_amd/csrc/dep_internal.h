@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <functional>
 #include <map>
 #include <string>
 #include <utility>
@@ -22,7 +23,6 @@
 
 #include "../../include/depsolver_amd.h"
 
-#define DEP_TC 16   // columns per assembly patch (the LDS tile TR x DEP_TC sets occupancy)
 
 struct dep_kstat {
     long launches = 0;
@@ -89,16 +89,20 @@ struct dep_geom {
     double* q = nullptr;              // E x 7 x 4
     int* inc_ptr = nullptr;           // node -> incident (element, position), by element
     int* inc_ek = nullptr;            // packed e*8 + k
-    int* patch_ptr = nullptr;         // column patch -> elements touching it
-    int* patch_elem = nullptr;
-    int8_t* patch_slot = nullptr;     // per patch element: 8 bytes, npe column slots (-1 outside / padding)
-    int n_patch = 0;
+    // column patches of tc consecutive nodes, one set per assembly shape
+    // (patch -> elements touching it; per patch element 8 bytes of column
+    // slots, -1 outside the patch / padding); built on first use
+    struct Patches {
+        int tc = 0, n = 0;
+        int* ptr = nullptr;
+        int* elem = nullptr;
+        int8_t* slot = nullptr;
+    } patches[3];
     double* S = nullptr;              // npe*E*npe singular integrals
     double* P = nullptr;
     long ldP = 0;
     bool assembled = false;
     long asm_count = 0;                     // assemblies so far (invalidates derived data)
-    int tc = DEP_TC;                        // columns per assembly patch (16, 32 or 64)
     struct dep_mst_cache* mst = nullptr;   // Maxwell-stress point set, built on first use
     float* P32 = nullptr;                   // FP32 copy of P for refined FP32 GMRES
     bool p32_valid = false;
@@ -183,10 +187,41 @@ int dep_obj_alloc(Obj* o, T** p, size_t count)
     return DEP_OK;
 }
 
+// ---- autotuning of launch shapes (dep_tune.cpp) ----------------------------
+// Candidates of a knob compute bit-identical results; the first launch of a
+// size class on a device times them all and keeps the fastest.
+enum { KNOB_ZMV32 = 0, KNOB_ZMV64 = 1, KNOB_ASM = 2, KNOB_COUNT = 3 };
+#define DEP_TUNE_REPS 3
+int tune_class(long n);                          // eight classes per octave of n
+int tune_ncand(int knob);
+int tune_lookup(dep_ctx* c, int knob, long n);   // the choice; 0 with tuning off; -1: still to be tuned
+int tune_store(dep_ctx* c, int knob, long n, const std::vector<std::vector<float>>& ms);
+float tune_time(dep_ctx* c, const std::function<hipError_t()>& launch);   // ms on c->stream, < 0 on failure
+// The candidate for (knob, n).  An undecided class is tuned here when
+// may_time holds (launch(k) enqueues candidate k, which must be idempotent:
+// each candidate runs DEP_TUNE_REPS times), else served candidate 0.
+template <class F>
+int tune_choose(dep_ctx* c, int knob, long n, bool may_time, F&& launch)
+{
+    const int k = tune_lookup(c, knob, n);
+    if (k >= 0) return k;
+    if (!may_time) return 0;
+    const int nc = tune_ncand(knob);
+    std::vector<std::vector<float>> ms(nc);
+    for (int rep = 0; rep < DEP_TUNE_REPS; rep++)
+        for (int cand = 0; cand < nc; cand++) {
+            const float t = tune_time(c, [&] { return launch(cand); });
+            if (t < 0.0f) return 0;
+            ms[cand].push_back(t);
+        }
+    return tune_store(c, knob, n, ms);
+}
+
 // ---- launchers (return hipError_t of the launch) ---------------------------
 hipError_t launch_quad_points(dep_geom* g);
 hipError_t launch_node_normals(dep_geom* g);
-hipError_t launch_assemble(dep_geom* g);
+hipError_t launch_assemble(dep_geom* g, int shape);   // shape: index into g->patches (built)
+int assemble_tc(int shape);                         // columns per patch of an assembly shape
 hipError_t launch_weight_fp64(dep_sys* s);            // Z = w * P + shift (FP64 planes)
 hipError_t launch_lu_fp64(dep_sys* s, double* min_pivot);
 hipError_t launch_trsv_fp64(dep_sys* s, const double* rhs, double* out);
@@ -205,8 +240,6 @@ hipError_t launch_rowscale(dep_ctx* c, int N, const double* d, const double* in,
 // y = Z x (b == nullptr) or b - Z x with the FP32 blocks P32 and the given weights
 hipError_t launch_zmatvec32_w(dep_sys* s, const double* wr, const double* wi, const double* shift,
                               const double* x, const double* b, double* y, const int* stop = nullptr);
-// y = Z x with the FP32 blocks, the system's weights and x in FP32 (split layout)
-hipError_t launch_zmatvec32_xf(dep_sys* s, const float* x, double* y, const int* stop);
 hipError_t launch_update(dep_ctx* c, int nvec, long n2, const double* V, const double* h, double* w);
 // partials only (DEP_DOT_CHUNKS per vector), consumed by launch_update_fused
 hipError_t launch_dot_partials(dep_ctx* c, int nvec, long n2, const double* V, const double* w, double* part,
@@ -297,6 +330,3 @@ hipError_t launch_bj_build(dep_ctx* c, int N, const double* P, long ldP, const d
 hipError_t launch_bj_apply(dep_ctx* c, int N, const float* B, const dep_blist& L, double* X, long xstride,
                            double* Y, long ystride, int accumulate = 0, const int* stop = nullptr, int sstride = 0,
                            const double* scale = nullptr);
-// one system: Y (FP32) = M^-1 X, X first scaled in place by *scale as above
-hipError_t launch_bj_apply_f32(dep_ctx* c, int N, const float* B, double* X, float* Y, const int* stop,
-                               const double* scale);

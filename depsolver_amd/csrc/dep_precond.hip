@@ -86,9 +86,8 @@ __global__ __launch_bounds__(256) void k_bj_combine(int nblk, const double* __re
 // With scale (one system), the block's rows of X are first multiplied by
 // *scale (unless it is 0) and written back: the normalisation of a Krylov
 // vector folded into its first use.
-template <class TY>
 __global__ __launch_bounds__(256) void k_bj_apply(int N, int nblk, const float* __restrict__ B, dep_blist L,
-                                                  double* __restrict__ X, long xstride, TY* __restrict__ Y,
+                                                  double* __restrict__ X, long xstride, double* __restrict__ Y,
                                                   long ystride, int accumulate, const int* __restrict__ stop,
                                                   int sstride, const double* __restrict__ scale)
 {
@@ -97,7 +96,7 @@ __global__ __launch_bounds__(256) void k_bj_apply(int N, int nblk, const float* 
     const long sys = L.id[blockIdx.y];
     if (stop && stop[sys * sstride]) return;
     double* x = X + sys * xstride;
-    TY* y = Y + sys * ystride;
+    double* y = Y + sys * ystride;
     const int t = threadIdx.x, r = t >> 2, q = t & 3;
     if (t < BJ) {
         const int j = b * BJ + t;
@@ -131,11 +130,11 @@ __global__ __launch_bounds__(256) void k_bj_apply(int N, int nblk, const float* 
     const int i = b * BJ + r;
     if (q == 0 && i < N) {
         if (accumulate) {
-            y[i] += (TY)sr;
-            y[N + i] += (TY)si;
+            y[i] += sr;
+            y[N + i] += si;
         } else {
-            y[i] = (TY)sr;
-            y[N + i] = (TY)si;
+            y[i] = sr;
+            y[N + i] = si;
         }
     }
 }
@@ -167,18 +166,8 @@ hipError_t launch_bj_apply(dep_ctx* c, int N, const float* B, const dep_blist& L
 {
     const int nblk = (int)bj_blocks(N);
     if (scale && L.n != 1) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_bj_apply<double>, dim3(nblk, L.n), dim3(256), 0, c->stream, N, nblk, B, L, X, xstride, Y,
+    hipLaunchKernelGGL(k_bj_apply, dim3(nblk, L.n), dim3(256), 0, c->stream, N, nblk, B, L, X, xstride, Y,
                        ystride, accumulate, stop, sstride, scale);
     return hipGetLastError();
 }
 
-hipError_t launch_bj_apply_f32(dep_ctx* c, int N, const float* B, double* X, float* Y, const int* stop,
-                               const double* scale)
-{
-    const int nblk = (int)bj_blocks(N);
-    dep_blist one;
-    one.n = 1;
-    hipLaunchKernelGGL(k_bj_apply<float>, dim3(nblk, 1), dim3(256), 0, c->stream, N, nblk, B, one, X, 0L, Y, 0L, 0,
-                       stop, 0, scale);
-    return hipGetLastError();
-}

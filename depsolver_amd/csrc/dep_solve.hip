@@ -8,10 +8,6 @@
 #include "dep_common.h"
 #include "dep_internal.h"
 
-// rows per wave and chunks in flight of the FP32-x product
-#define ZMV32_ROWS 4
-#define ZMV32_UNROLL 2
-
 namespace {
 
 // Z = w * P + shift on the diagonal (FP64 split planes)
@@ -39,10 +35,11 @@ __global__ void k_weight(int N, const double* __restrict__ P, long ldP, const do
 // the last full group of chunks.  Shapes measured at N = 4,740 / 9,480 /
 // 15,044: FP32 blocks and FP64 x 6 rows, 2 chunks, one wave, NT (23 / 67 /
 // 148 us, 6.1 TB/s at the largest; 4 rows, 1 chunk, 4 waves: 28 / 87 / 192
-// us).  An FP32 x halves the x stream that every wave reads from L2: 4 rows,
-// 2 chunks reach 6.2 TB/s where the FP64 x form reaches 5.1-5.2 in the same
-// back-to-back timing (a plain read of the blocks: 6.9 TB/s).  The FP64
-// blocks use k_zmatvec64 below.
+// us).  An FP32 x would halve the x stream every wave reads from L2 (4 rows,
+// 2 chunks: 6.2 TB/s against 5.1-5.2 for the FP64 x form in back-to-back
+// timing; a plain read of the blocks: 6.9 TB/s), but rounding the Krylov
+// vector costs the refined solves accuracy: some shelled-particle systems
+// then need a third correction.  The FP64 blocks use k_zmatvec64 below.
 template <class TP, class TX, int ZROWS, int UNR, int WPB, bool NT>
 __global__ __launch_bounds__(64 * WPB) void k_zmatvec(int N, const TP* __restrict__ P, long ldP,
                                                 const double* __restrict__ wr,
@@ -125,21 +122,22 @@ __global__ __launch_bounds__(64 * WPB) void k_zmatvec(int N, const TP* __restric
     }
 }
 
-// The FP64 product keeps the load-then-accumulate order per row (4 rows per
-// wave, 4 waves): loading all rows of a chunk first, as the FP32 kernel does,
-// measured 0.40 ms against 0.32 ms for this form at N = 15,044.
-__global__ __launch_bounds__(256) void k_zmatvec64(int N, const double* __restrict__ P, long ldP,
-                                                   const double* __restrict__ wr,
-                                                   const double* __restrict__ wi,
-                                                   const double* __restrict__ shift,
-                                                   const double* __restrict__ x,
-                                                   const double* __restrict__ b,
-                                                   double* __restrict__ y)
+// The FP64 product keeps the load-then-accumulate order per row (ZROWS rows
+// per wave, WPB waves): loading all rows of a chunk first, as the FP32 kernel
+// does, measured 0.40 ms against 0.32 ms for this form (4 rows, 4 waves) at
+// N = 15,044.  The per-row order of the sums does not depend on ZROWS or WPB.
+template <int ZROWS, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_zmatvec64(int N, const double* __restrict__ P, long ldP,
+                                                        const double* __restrict__ wr,
+                                                        const double* __restrict__ wi,
+                                                        const double* __restrict__ shift,
+                                                        const double* __restrict__ x,
+                                                        const double* __restrict__ b,
+                                                        double* __restrict__ y)
 {
-    constexpr int ZROWS = 4;
     typedef double d4 __attribute__((ext_vector_type(4)));
     const int lane = threadIdx.x & 63;
-    const int row0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * ZROWS;
+    const int row0 = (blockIdx.x * WPB + (threadIdx.x >> 6)) * ZROWS;
     if (row0 >= N) return;
     const int nr = min(ZROWS, N - row0);
     double sr[ZROWS], si[ZROWS];
@@ -214,44 +212,88 @@ hipError_t launch_residual(dep_sys* s, const double* x, double* r)
     return launch_zmatvec_w(s, s->wr, s->wi, s->shift, x, s->b, r);
 }
 
+// Shapes of the FP64 product (autotuned): rows per wave x waves per workgroup
+static hipError_t zmatvec64_shape(dep_geom* g, int k, const double* wr, const double* wi, const double* shift,
+                                  const double* x, const double* b, double* y)
+{
+    const int N = g->N;
+    hipStream_t st = g->ctx->stream;
+    switch (k) {
+    case 1:
+        hipLaunchKernelGGL((k_zmatvec64<2, 4>), dim3(dep_cdiv(N, 8)), dim3(256), 0, st, N, g->P, g->ldP, wr, wi, shift,
+                           x, b, y);
+        break;
+    case 2:
+        hipLaunchKernelGGL((k_zmatvec64<4, 1>), dim3(dep_cdiv(N, 4)), dim3(64), 0, st, N, g->P, g->ldP, wr, wi, shift,
+                           x, b, y);
+        break;
+    case 3:
+        hipLaunchKernelGGL((k_zmatvec64<8, 1>), dim3(dep_cdiv(N, 8)), dim3(64), 0, st, N, g->P, g->ldP, wr, wi, shift,
+                           x, b, y);
+        break;
+    default:
+        hipLaunchKernelGGL((k_zmatvec64<4, 4>), dim3(dep_cdiv(N, 16)), dim3(256), 0, st, N, g->P, g->ldP, wr, wi,
+                           shift, x, b, y);
+    }
+    return hipGetLastError();
+}
+
+// Shapes of the FP32-block product (autotuned): rows per wave, two chunks in
+// flight, one wave per workgroup
+static hipError_t zmatvec32_shape(dep_geom* g, int k, const double* wr, const double* wi, const double* shift,
+                                  const double* x, const double* b, double* y, const int* stop)
+{
+    const int N = g->N;
+    hipStream_t st = g->ctx->stream;
+    switch (k) {
+    case 1:
+        hipLaunchKernelGGL((k_zmatvec<float, double, 4, 2, 1, true>), dim3(dep_cdiv(N, 4)), dim3(64), 0, st, N,
+                           g->P32, g->ldP, wr, wi, shift, x, b, y, stop);
+        break;
+    case 2:
+        hipLaunchKernelGGL((k_zmatvec<float, double, 3, 2, 1, true>), dim3(dep_cdiv(N, 3)), dim3(64), 0, st, N,
+                           g->P32, g->ldP, wr, wi, shift, x, b, y, stop);
+        break;
+    case 3:
+        hipLaunchKernelGGL((k_zmatvec<float, double, 8, 2, 1, true>), dim3(dep_cdiv(N, 8)), dim3(64), 0, st, N,
+                           g->P32, g->ldP, wr, wi, shift, x, b, y, stop);
+        break;
+    default:
+        hipLaunchKernelGGL((k_zmatvec<float, double, 6, 2, 1, true>), dim3(dep_cdiv(N, 6)), dim3(64), 0, st, N,
+                           g->P32, g->ldP, wr, wi, shift, x, b, y, stop);
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_zmatvec_w(dep_sys* s, const double* wr, const double* wi, const double* shift,
                             const double* x, const double* b, double* y)
 {
     dep_geom* g = s->g;
+    const int k = tune_choose(g->ctx, KNOB_ZMV64, g->N, true,
+                              [&](int c) { return zmatvec64_shape(g, c, wr, wi, shift, x, b, y); });
     hipEvent_t ev;
     prof_begin(g->ctx, &ev);
-    hipLaunchKernelGGL(k_zmatvec64, dim3(dep_cdiv(g->N, 16)), dim3(256), 0, g->ctx->stream, g->N, g->P, g->ldP, wr,
-                       wi, shift, x, b, y);
+    const hipError_t e = zmatvec64_shape(g, k, wr, wi, shift, x, b, y);
     const double n = g->N;
     prof_end(g->ctx, ev, "zmatvec", 4.0 * n * n, 8.0 * n * n + 48.0 * n);
-    return hipGetLastError();
+    return e;
 }
 
 hipError_t launch_zmatvec32_w(dep_sys* s, const double* wr, const double* wi, const double* shift,
                               const double* x, const double* b, double* y, const int* stop)
 {
     dep_geom* g = s->g;
+    // a launch behind a stop flag may return at once, so only unflagged launches time the shapes
+    const int k = tune_choose(g->ctx, KNOB_ZMV32, g->N, stop == nullptr,
+                              [&](int c) { return zmatvec32_shape(g, c, wr, wi, shift, x, b, y, nullptr); });
     hipEvent_t ev;
     prof_begin(g->ctx, &ev);
-    hipLaunchKernelGGL((k_zmatvec<float, double, 6, 2, 1, true>), dim3(dep_cdiv(g->N, 6)), dim3(64), 0,
-                       g->ctx->stream, g->N, g->P32, g->ldP, wr, wi, shift, x, b, y, stop);
+    const hipError_t e = zmatvec32_shape(g, k, wr, wi, shift, x, b, y, stop);
     const double n = g->N;
     prof_end(g->ctx, ev, "zmatvec32", 4.0 * n * n, 4.0 * n * n + 56.0 * n);
-    return hipGetLastError();
+    return e;
 }
 
-hipError_t launch_zmatvec32_xf(dep_sys* s, const float* x, double* y, const int* stop)
-{
-    dep_geom* g = s->g;
-    hipEvent_t ev;
-    prof_begin(g->ctx, &ev);
-    constexpr int ZR = ZMV32_ROWS, UN = ZMV32_UNROLL;
-    hipLaunchKernelGGL((k_zmatvec<float, float, ZR, UN, 1, true>), dim3(dep_cdiv(g->N, ZR)), dim3(64), 0,
-                       g->ctx->stream, g->N, g->P32, g->ldP, s->wr, s->wi, s->shift, x, nullptr, y, stop);
-    const double n = g->N;
-    prof_end(g->ctx, ev, "zmatvec32", 4.0 * n * n, 4.0 * n * n + 48.0 * n);
-    return hipGetLastError();
-}
 
 hipError_t launch_matvec(dep_sys* s, const double* x, double* y)
 {

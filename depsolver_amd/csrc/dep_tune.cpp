@@ -1,0 +1,161 @@
+// dep_tune.cpp -- autotuning of launch shapes.
+//
+// A knob is a kernel family whose candidates differ only in how the work is
+// mapped onto waves and workgroups (rows per wave, waves per workgroup,
+// columns per assembly patch); every candidate performs the same arithmetic
+// in the same order per output, so the choice never changes a result bit.
+// The first launch of a size class on a device (eight classes per octave of
+// the system size N) times every candidate DEP_TUNE_REPS times, interleaved,
+// on the caller's stream with events, keeps the candidate with the smallest
+// median and serves it to every later launch of that class on that device,
+// from any context or thread.  DEPSOLVER_TUNE=0 (or dep_tune_set_mode(0))
+// leaves every knob at its first candidate, the shape measured best at the
+// largest single-particle system (N = 15,044).
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <functional>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+
+#include "dep_internal.h"
+
+namespace {
+
+struct Entry {
+    int choice = -1;
+    std::vector<float> median_ms;   // per candidate, from the tuning run (empty if pinned)
+    bool pinned = false;
+    int n_lo = 0, n_hi = 0;         // sizes seen in the class
+};
+
+std::mutex g_mu;
+std::map<std::tuple<int, int, int>, Entry> g_table;   // (device, knob, class)
+int g_mode = -1;                                       // -1: read DEPSOLVER_TUNE on first use
+
+const char* const kKnobName[KNOB_COUNT] = {"zmatvec32", "zmatvec64", "assemble"};
+const char* const kCandName[KNOB_COUNT][4] = {
+    {"6 rows/wave", "4 rows/wave", "3 rows/wave", "8 rows/wave"},
+    {"4 rows/wave, 4 waves", "2 rows/wave, 4 waves", "4 rows/wave, 1 wave", "8 rows/wave, 1 wave"},
+    {"16 columns/patch", "8 columns/patch", "32 columns/patch", ""},
+};
+
+int mode_locked()
+{
+    if (g_mode < 0) {
+        const char* e = getenv("DEPSOLVER_TUNE");
+        g_mode = (e && atoi(e) == 0) ? 0 : 1;
+    }
+    return g_mode;
+}
+
+}  // namespace
+
+int tune_class(long n) { return n <= 1 ? 0 : (int)lround(8.0 * log2((double)n)); }
+
+int tune_ncand(int knob) { return knob == KNOB_ASM ? 3 : 4; }
+
+int tune_lookup(dep_ctx* c, int knob, long n)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_table.find({c->device, knob, tune_class(n)});
+    if (it != g_table.end() && it->second.choice >= 0) {
+        Entry& e = it->second;
+        e.n_lo = std::min(e.n_lo, (int)n);
+        e.n_hi = std::max(e.n_hi, (int)n);
+        return e.choice;
+    }
+    return mode_locked() ? -1 : 0;
+}
+
+int tune_store(dep_ctx* c, int knob, long n, const std::vector<std::vector<float>>& ms)
+{
+    std::vector<float> med;
+    int best = 0;
+    for (size_t k = 0; k < ms.size(); k++) {
+        std::vector<float> v = ms[k];
+        std::sort(v.begin(), v.end());
+        med.push_back(v[v.size() / 2]);
+        if (med[k] < med[best]) best = (int)k;
+    }
+    std::lock_guard<std::mutex> lk(g_mu);
+    Entry& e = g_table[{c->device, knob, tune_class(n)}];
+    if (e.choice >= 0) return e.choice;   // another thread decided first: keep one answer per class
+    e.choice = best;
+    e.median_ms = med;
+    e.n_lo = e.n_hi = (int)n;
+    return best;
+}
+
+float tune_time(dep_ctx* c, const std::function<hipError_t()>& launch)
+{
+    hipEvent_t a = nullptr, b = nullptr;
+    float ms = -1.0f;
+    if (hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess &&
+        hipEventRecord(a, c->stream) == hipSuccess && launch() == hipSuccess &&
+        hipEventRecord(b, c->stream) == hipSuccess && hipEventSynchronize(b) == hipSuccess) {
+        if (hipEventElapsedTime(&ms, a, b) != hipSuccess) ms = -1.0f;
+    }
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    return ms;
+}
+
+extern "C" int dep_tune_set_mode(int mode)
+{
+    if (mode != 0 && mode != 1) return DEP_ERR_ARG;
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_mode = mode;
+    return DEP_OK;
+}
+
+extern "C" int dep_tune_reset(void)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_table.clear();
+    return DEP_OK;
+}
+
+extern "C" int dep_tune_pin(int device, const char* knob, int n, int candidate)
+{
+    if (!knob || n <= 0) return DEP_ERR_ARG;
+    int k = -1;
+    for (int i = 0; i < KNOB_COUNT; i++)
+        if (strcmp(knob, kKnobName[i]) == 0) k = i;
+    if (k < 0 || candidate < 0 || candidate >= tune_ncand(k)) return DEP_ERR_ARG;
+    std::lock_guard<std::mutex> lk(g_mu);
+    Entry& e = g_table[{device, k, tune_class(n)}];
+    e.choice = candidate;
+    e.pinned = true;
+    e.median_ms.clear();
+    e.n_lo = e.n_hi = n;
+    return DEP_OK;
+}
+
+extern "C" int dep_tune_report(char* buf, size_t len)
+{
+    if (!buf || len == 0) return DEP_ERR_ARG;
+    std::string out;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        for (const auto& kv : g_table) {
+            const int dev = std::get<0>(kv.first), knob = std::get<1>(kv.first), cls = std::get<2>(kv.first);
+            const Entry& e = kv.second;
+            if (e.choice < 0) continue;
+            char line[512];
+            int m = snprintf(line, sizeof line, "device %d %s class %d (N %d-%d): %s%s", dev, kKnobName[knob], cls,
+                             e.n_lo, e.n_hi, kCandName[knob][e.choice], e.pinned ? " (pinned)" : "");
+            for (size_t k = 0; k < e.median_ms.size() && m < (int)sizeof line - 40; k++)
+                m += snprintf(line + m, sizeof line - m, "%s %.1f us", k ? "," : " |", 1e3 * e.median_ms[k]);
+            out += line;
+            out += "\n";
+        }
+    }
+    snprintf(buf, len, "%s", out.c_str());
+    return out.size() < len ? DEP_OK : DEP_ERR_ARG;
+}

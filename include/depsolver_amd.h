@@ -234,6 +234,19 @@ size_t dep_request_footprint(const dep_request *rq);
 int dep_server_stats(dep_server *sv, long *completed, size_t *peak_reserved, size_t *budget);
 int dep_server_destroy(dep_server *sv);   /* finishes the queued requests first */
 
+/* ---- autotuning of launch shapes (no counterpart in the reference) ------
+ * The kernel families "zmatvec32" (FP32-block product of the refined GMRES),
+ * "zmatvec64" (FP64 product: residuals, FP64 GMRES) and "assemble" (regular
+ * quadrature) have several launch shapes that give bit-identical results.
+ * The first launch of a size class (eight classes per octave of N) on a
+ * device times every shape and keeps the fastest for the process. */
+int dep_tune_set_mode(int mode);   /* 1: tune (default; DEPSOLVER_TUNE=0 starts at 0); 0: first shape */
+/* fix the shape (0-based candidate) of knob for N's size class on a device */
+int dep_tune_pin(int device, const char *knob, int n, int candidate);
+int dep_tune_reset(void);          /* forget every choice */
+/* one text line per decided class: knob, sizes seen, shape, median time per shape */
+int dep_tune_report(char *buf, size_t len);
+
 /* ---- reference-signature shims (src/depSolver.c:284-344) ----------------- */
 int electricFormA_tria3(double **mNodes, unsigned int **mElems, unsigned int *vBCType,
                         unsigned int **vInterfaces, double **vMatParam, double *vProbParam,

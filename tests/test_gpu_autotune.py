@@ -1,0 +1,98 @@
+"""GPU: autotuned launch shapes.
+
+Every launch shape of a tuned kernel family computes the same bits, so the
+tuner's choice can never change an answer:
+
+* assembly: the geometric blocks P are identical for every patch width;
+* refined FP32 GMRES: sigma is identical for every shape of the FP32-block
+  product and of the FP64 residual product;
+* FP64 GMRES: sigma is identical for every shape of the FP64 product;
+* with tuning on, the first solve of a size decides all three families
+  (the report lists them) and later solves reuse the choice.
+"""
+import numpy as np
+import pytest
+
+from depsolver_amd import mesh
+
+pytestmark = pytest.mark.gpu
+
+
+def _shapes(knob):
+    return 3 if knob == "assemble" else 4
+
+
+@pytest.fixture()
+def fresh_tuner():
+    import depsolver_amd as dep
+    dep.tune_reset()
+    yield dep
+    dep.tune_reset()
+    dep.tune_mode(1)
+
+
+def test_assembly_shapes_give_identical_blocks(ctx, fresh_tuner):
+    dep = fresh_tuner
+    p = mesh.config1()
+    out = []
+    for k in range(_shapes("assemble")):
+        print(f"[gpu] autotune assemble shape {k}", flush=True)
+        dep.tune_reset()
+        dep.tune_pin("assemble", p.n_nodes, k)
+        g = dep.Geometry(ctx, p)
+        try:
+            out.append(g.blocks())
+        finally:
+            g.close()
+    for k in range(1, len(out)):
+        assert np.array_equal(out[0], out[k]), k
+
+
+@pytest.mark.parametrize("knob", ["zmatvec32", "zmatvec64"])
+def test_product_shapes_give_identical_solutions(ctx, fresh_tuner, knob):
+    dep = fresh_tuner
+    p = mesh.config1()
+    g = dep.Geometry(ctx, p)
+    s = dep.System(g)
+    try:
+        sig = []
+        for k in range(_shapes(knob)):
+            print(f"[gpu] autotune {knob} shape {k}", flush=True)
+            dep.tune_reset()
+            dep.tune_pin(knob, p.n_nodes, k)
+            if knob == "zmatvec32":
+                x, info = s.gmres_ir(tol=1e-12, inner_tol=1e-6)
+            else:
+                x, info = s.gmres(precond=1, tol=1e-12, max_iters=600)
+            assert info["converged"] == 1
+            sig.append(x)
+        for k in range(1, len(sig)):
+            assert np.array_equal(sig[0], sig[k]), (knob, k)
+    finally:
+        s.close()
+        g.close()
+
+
+def test_first_solve_tunes_every_family(ctx, fresh_tuner):
+    dep = fresh_tuner
+    dep.tune_mode(1)
+    p = mesh.config1()
+    g = dep.Geometry(ctx, p)
+    s = dep.System(g)
+    try:
+        print("[gpu] autotune first solve", flush=True)
+        x1, _ = s.gmres_ir(tol=1e-12, inner_tol=1e-6)
+        x2, _ = s.gmres_ir(tol=1e-12, inner_tol=1e-6)
+        assert np.array_equal(x1, x2)
+        rep = dep.tune_report()
+        for knob in ("assemble", "zmatvec32", "zmatvec64"):
+            assert any(f" {knob} " in line for line in rep), (knob, rep)
+        # pinned untuned mode serves the first shape and gives the same bits
+        dep.tune_reset()
+        dep.tune_mode(0)
+        x3, _ = s.gmres_ir(tol=1e-12, inner_tol=1e-6)
+        assert np.array_equal(x1, x3)
+        assert dep.tune_report() == []
+    finally:
+        s.close()
+        g.close()
