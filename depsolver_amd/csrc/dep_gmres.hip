@@ -378,13 +378,13 @@ __global__ __launch_bounds__(256) void k_cgs_last(int i, int cap, long n2, const
     if (threadIdx.x == 0) {
         __hip_atomic_store(np + blockIdx.x, (red[0] + red[1]) + (red[2] + red[3]), __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence();   // the partial is visible before the count
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");   // the partial is visible before the count
         const unsigned done = atomicAdd((unsigned*)(flags + 3), 1u);
         last = done == gridDim.x - 1;
+        if (last) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");   // every partial is visible
     }
     __syncthreads();
     if (!last) return;
-    __threadfence();
     double hn2 = 0.0;
     if (threadIdx.x == 0) {
         for (int k = 0; k < DEP_UPD_BLOCKS; k++)
