@@ -238,9 +238,6 @@ hipError_t assemble_impl(dep_geom* g, int shape)
     if (ps.tc == 8)
         hipLaunchKernelGGL((k_assemble_regular<NPE, TR, 8>), grid, dim3(TR), 0, st, g->N, g->xyz, g->conn, g->kind,
                            g->normal, (const double4*)g->q, ps.ptr, ps.elem, ps.slot, g->P, g->ldP, g->P32);
-    else if (ps.tc == 32)
-        hipLaunchKernelGGL((k_assemble_regular<NPE, TR, 32>), grid, dim3(TR), 0, st, g->N, g->xyz, g->conn, g->kind,
-                           g->normal, (const double4*)g->q, ps.ptr, ps.elem, ps.slot, g->P, g->ldP, g->P32);
     else
         hipLaunchKernelGGL((k_assemble_regular<NPE, TR, 16>), grid, dim3(TR), 0, st, g->N, g->xyz, g->conn, g->kind,
                            g->normal, (const double4*)g->q, ps.ptr, ps.elem, ps.slot, g->P, g->ldP, g->P32);
@@ -289,11 +286,11 @@ hipError_t launch_node_normals(dep_geom* g)
 }
 
 // shapes: columns per patch 16 (measured best at N = 15,044: 2.45 ms; 8:
-// 2.71 ms, 32: 3.27 ms, where the larger LDS tile halves occupancy), 8, 32
-int assemble_tc(int shape) { return shape == 1 ? 8 : shape == 2 ? 32 : 16; }
+// 2.71 ms, 32: 3.27 ms, where the larger LDS tile halves occupancy), 8
+int assemble_tc(int shape) { return shape == 1 ? 8 : 16; }
 
 hipError_t launch_assemble(dep_geom* g, int shape)
 {
-    if (shape < 0 || shape > 2 || !g->patches[shape].ptr) return hipErrorInvalidValue;
+    if (shape < 0 || shape > 1 || !g->patches[shape].ptr) return hipErrorInvalidValue;
     return g->npe == 3 ? assemble_impl<3>(g, shape) : assemble_impl<6>(g, shape);
 }

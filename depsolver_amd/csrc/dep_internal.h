@@ -97,7 +97,7 @@ struct dep_geom {
         int* ptr = nullptr;
         int* elem = nullptr;
         int8_t* slot = nullptr;
-    } patches[3];
+    } patches[2];
     double* S = nullptr;              // npe*E*npe singular integrals
     double* P = nullptr;
     long ldP = 0;
@@ -192,6 +192,8 @@ int dep_obj_alloc(Obj* o, T** p, size_t count)
 // size class on a device times them all and keeps the fastest.
 enum { KNOB_ZMV32 = 0, KNOB_ZMV64 = 1, KNOB_ASM = 2, KNOB_COUNT = 3 };
 #define DEP_TUNE_REPS 3
+#define DEP_TUNE_MARGIN 0.03f   // a shape must beat the first one by 3% to replace it
+#define DEP_TUNE_DROP 1.3f      // shapes this much slower than the fastest after one round stop
 int tune_class(long n);                          // eight classes per octave of n
 int tune_ncand(int knob);
 int tune_lookup(dep_ctx* c, int knob, long n);   // the choice; 0 with tuning off; -1: still to be tuned
@@ -200,6 +202,9 @@ float tune_time(dep_ctx* c, const std::function<hipError_t()>& launch);   // ms 
 // The candidate for (knob, n).  An undecided class is tuned here when
 // may_time holds (launch(k) enqueues candidate k, which must be idempotent:
 // each candidate runs DEP_TUNE_REPS times), else served candidate 0.
+// One untimed launch first warms the data (caches, TLB); after the first
+// round a shape other than the first that is slower than DEP_TUNE_DROP times
+// the fastest is not timed again.
 template <class F>
 int tune_choose(dep_ctx* c, int knob, long n, bool may_time, F&& launch)
 {
@@ -208,11 +213,15 @@ int tune_choose(dep_ctx* c, int knob, long n, bool may_time, F&& launch)
     if (!may_time) return 0;
     const int nc = tune_ncand(knob);
     std::vector<std::vector<float>> ms(nc);
+    if (tune_time(c, [&] { return launch(0); }) < 0.0f) return 0;
+    float best = 0.0f;
     for (int rep = 0; rep < DEP_TUNE_REPS; rep++)
         for (int cand = 0; cand < nc; cand++) {
+            if (rep > 0 && cand > 0 && ms[cand][0] > DEP_TUNE_DROP * best) continue;
             const float t = tune_time(c, [&] { return launch(cand); });
             if (t < 0.0f) return 0;
             ms[cand].push_back(t);
+            if (rep == 0 && (cand == 0 || t < best)) best = t;
         }
     return tune_store(c, knob, n, ms);
 }

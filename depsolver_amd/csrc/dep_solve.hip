@@ -227,10 +227,6 @@ static hipError_t zmatvec64_shape(dep_geom* g, int k, const double* wr, const do
         hipLaunchKernelGGL((k_zmatvec64<4, 1>), dim3(dep_cdiv(N, 4)), dim3(64), 0, st, N, g->P, g->ldP, wr, wi, shift,
                            x, b, y);
         break;
-    case 3:
-        hipLaunchKernelGGL((k_zmatvec64<8, 1>), dim3(dep_cdiv(N, 8)), dim3(64), 0, st, N, g->P, g->ldP, wr, wi, shift,
-                           x, b, y);
-        break;
     default:
         hipLaunchKernelGGL((k_zmatvec64<4, 4>), dim3(dep_cdiv(N, 16)), dim3(256), 0, st, N, g->P, g->ldP, wr, wi,
                            shift, x, b, y);

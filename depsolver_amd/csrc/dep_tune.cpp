@@ -5,9 +5,11 @@
 // columns per assembly patch); every candidate performs the same arithmetic
 // in the same order per output, so the choice never changes a result bit.
 // The first launch of a size class on a device (eight classes per octave of
-// the system size N) times every candidate DEP_TUNE_REPS times, interleaved,
-// on the caller's stream with events, keeps the candidate with the smallest
-// median and serves it to every later launch of that class on that device,
+// the system size N) times every candidate up to DEP_TUNE_REPS times,
+// interleaved, on the caller's stream with events, keeps the candidate with
+// the smallest time (the first candidate unless another beats it by
+// DEP_TUNE_MARGIN)
+// and serves it to every later launch of that class on that device,
 // from any context or thread.  DEPSOLVER_TUNE=0 (or dep_tune_set_mode(0))
 // leaves every knob at its first candidate, the shape measured best at the
 // largest single-particle system (N = 15,044).
@@ -29,7 +31,7 @@ namespace {
 
 struct Entry {
     int choice = -1;
-    std::vector<float> median_ms;   // per candidate, from the tuning run (empty if pinned)
+    std::vector<float> median_ms;   // per candidate: fastest time of the tuning run (empty if pinned)
     bool pinned = false;
     int n_lo = 0, n_hi = 0;         // sizes seen in the class
 };
@@ -41,8 +43,8 @@ int g_mode = -1;                                       // -1: read DEPSOLVER_TUN
 const char* const kKnobName[KNOB_COUNT] = {"zmatvec32", "zmatvec64", "assemble"};
 const char* const kCandName[KNOB_COUNT][4] = {
     {"6 rows/wave", "4 rows/wave", "3 rows/wave", "8 rows/wave"},
-    {"4 rows/wave, 4 waves", "2 rows/wave, 4 waves", "4 rows/wave, 1 wave", "8 rows/wave, 1 wave"},
-    {"16 columns/patch", "8 columns/patch", "32 columns/patch", ""},
+    {"4 rows/wave, 4 waves", "2 rows/wave, 4 waves", "4 rows/wave, 1 wave", ""},
+    {"16 columns/patch", "8 columns/patch", "", ""},
 };
 
 int mode_locked()
@@ -58,7 +60,10 @@ int mode_locked()
 
 int tune_class(long n) { return n <= 1 ? 0 : (int)lround(8.0 * log2((double)n)); }
 
-int tune_ncand(int knob) { return knob == KNOB_ASM ? 3 : 4; }
+// shapes that never won at N = 4,740 .. 29,964 are not candidates: 32-column
+// patches (1.2-4.7x slower than 16: the LDS tile halves occupancy) and the
+// FP64 product with 8 rows in one wave (1.3-2.2x slower)
+int tune_ncand(int knob) { return knob == KNOB_ZMV32 ? 4 : knob == KNOB_ZMV64 ? 3 : 2; }
 
 int tune_lookup(dep_ctx* c, int knob, long n)
 {
@@ -75,14 +80,17 @@ int tune_lookup(dep_ctx* c, int knob, long n)
 
 int tune_store(dep_ctx* c, int knob, long n, const std::vector<std::vector<float>>& ms)
 {
+    // the fastest time of each shape: other work on the device (serving slots,
+    // a profiler) only ever adds time, so the minimum is the steadiest figure
     std::vector<float> med;
     int best = 0;
     for (size_t k = 0; k < ms.size(); k++) {
-        std::vector<float> v = ms[k];
-        std::sort(v.begin(), v.end());
-        med.push_back(v[v.size() / 2]);
+        med.push_back(*std::min_element(ms[k].begin(), ms[k].end()));
         if (med[k] < med[best]) best = (int)k;
     }
+    // the first shape stays unless another is clearly faster: shapes within
+    // the timing noise would otherwise be picked at random
+    if (med[best] > (1.0f - DEP_TUNE_MARGIN) * med[0]) best = 0;
     std::lock_guard<std::mutex> lk(g_mu);
     Entry& e = g_table[{c->device, knob, tune_class(n)}];
     if (e.choice >= 0) return e.choice;   // another thread decided first: keep one answer per class
@@ -151,7 +159,7 @@ extern "C" int dep_tune_report(char* buf, size_t len)
             int m = snprintf(line, sizeof line, "device %d %s class %d (N %d-%d): %s%s", dev, kKnobName[knob], cls,
                              e.n_lo, e.n_hi, kCandName[knob][e.choice], e.pinned ? " (pinned)" : "");
             for (size_t k = 0; k < e.median_ms.size() && m < (int)sizeof line - 40; k++)
-                m += snprintf(line + m, sizeof line - m, "%s %.1f us", k ? "," : " |", 1e3 * e.median_ms[k]);
+                m += snprintf(line + m, sizeof line - m, "%s %.1f us", k ? "," : " | fastest", 1e3 * e.median_ms[k]);
             out += line;
             out += "\n";
         }

@@ -239,7 +239,8 @@ int dep_server_destroy(dep_server *sv);   /* finishes the queued requests first 
  * "zmatvec64" (FP64 product: residuals, FP64 GMRES) and "assemble" (regular
  * quadrature) have several launch shapes that give bit-identical results.
  * The first launch of a size class (eight classes per octave of N) on a
- * device times every shape and keeps the fastest for the process. */
+ * device times every shape and keeps the fastest for the process (the first
+ * shape unless another is at least 3% faster). */
 int dep_tune_set_mode(int mode);   /* 1: tune (default; DEPSOLVER_TUNE=0 starts at 0); 0: first shape */
 /* fix the shape (0-based candidate) of knob for N's size class on a device */
 int dep_tune_pin(int device, const char *knob, int n, int candidate);

@@ -19,7 +19,7 @@ pytestmark = pytest.mark.gpu
 
 
 def _shapes(knob):
-    return 3 if knob == "assemble" else 4
+    return {"assemble": 2, "zmatvec64": 3, "zmatvec32": 4}[knob]
 
 
 @pytest.fixture()
