@@ -1039,6 +1039,26 @@ int dep_mst_enqueue(dep_sys* s, double eps, double* dres)
     return DEP_OK;
 }
 
+int dep_mst_batch_scratch(dep_geom* g, int nfreq, size_t* n)
+{
+    dep_mst_cache* mc = nullptr;
+    DEP_CHECK(mst_cache(g, &mc));
+    *n = mst_batch_scratch(g, &mc->P, nfreq);
+    return DEP_OK;
+}
+
+int dep_mst_batch(dep_sys* s, const double* X, long xs, int nfreq, double eps, double* dres, long rs, double* scratch)
+{
+    dep_geom* g = s->g;
+    dep_ctx* c = g->ctx;
+    dep_mst_cache* mc = nullptr;
+    DEP_CHECK(mst_cache(g, &mc));
+    const hipError_t e = launch_mst_batch(s, &mc->P, X, xs, nfreq, mc->dnodes, mc->nf, mc->dfel, 0.5 * eps, scratch,
+                                          dres, rs);
+    if (e != hipSuccess) return dep_fail(c, DEP_ERR_HIP, "MST kernels: %s", hipGetErrorString(e));
+    return DEP_OK;
+}
+
 extern "C" int dep_post_mst(dep_sys* s, double eps, double* Fce, double* Xce, double* Fcm, double* Xcm)
 {
     if (!s || !Fce || !Xce || !Fcm || !Xcm) return DEP_ERR_ARG;

@@ -292,6 +292,10 @@ int ensure_p32(dep_geom* g);
 // nf = force elements); dres == nullptr uses the geometry's own result buffer
 int dep_mst_result_size(dep_geom* g, size_t* n);
 int dep_mst_enqueue(dep_sys* s, double eps, double* dres);
+// the same for nfreq solutions (sigma_f at X + f xs, result f at dres + f rs) in one set of launches;
+// scratch: dep_mst_batch_scratch doubles
+int dep_mst_batch_scratch(dep_geom* g, int nfreq, size_t* n);
+int dep_mst_batch(dep_sys* s, const double* X, long xs, int nfreq, double eps, double* dres, long rs, double* scratch);
 int dep_mst_center(dep_geom* g, double* Xcm);   // mean of the bc-6 nodes (force.c:493-507)
 
 // ---- batched systems sharing P (dep_batch.hip) ------------------------------

@@ -35,3 +35,8 @@ hipError_t launch_multipole_terms(dep_sys* s, int m, int order, const double* pt
 hipError_t launch_mst(dep_sys* s, const dep_points* P, const double* ef, const int* nodes, int nf,
                       const int* felems, double half_eps, double* Efn, double* Fce, double* Xce,
                       double* Fcm);
+// The Maxwell-stress force of nfreq solutions at once (a sweep): sigma_f at X + f xs; result of
+// frequency f (Fce 3 nf, Xce 3 nf, Fcm 3) at res + f rs; scratch: mst_batch_scratch doubles.
+size_t mst_batch_scratch(const dep_geom* g, const dep_points* P, int nfreq);
+hipError_t launch_mst_batch(dep_sys* s, const dep_points* P, const double* X, long xs, int nfreq, const int* nodes,
+                            int nf, const int* felems, double half_eps, double* scratch, double* res, long rs);

@@ -23,12 +23,15 @@
 
 namespace {
 
+// (blockIdx.y: frequency of a sweep, x and qs xs and qss apart)
 template <int NPE>
 __global__ void k_sigma_at_points(int E, int N, const int* __restrict__ conn,
-                                  const double* __restrict__ x, double2* __restrict__ qs)
+                                  const double* __restrict__ x, double2* __restrict__ qs, long xs, long qss)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= E * 7) return;
+    x += blockIdx.y * xs;
+    qs += blockIdx.y * qss;
     const int e = t / 7, g = t % 7;
     double Nf[NPE];
     shape_fn<NPE>(kT7[g][0], kT7[g][1], Nf);
@@ -104,6 +107,77 @@ __global__ __launch_bounds__(256) void k_points_regular(
         for (int o = 0; o < NO; o++) partial[((long)s * M + p) * NO + o] = acc[o];
 }
 
+// The field at the points for FB frequencies of a sweep at once (blockIdx.z:
+// frequencies FB z .. FB z + FB - 1, sigma at the quadrature points qss apart,
+// partials pstride apart): the geometric part of each (point, element,
+// quadrature point) term is computed once and contracted with every
+// frequency's sigma.  Per frequency the arithmetic and its order are those of
+// k_points_regular<1>, so the partials are the same bits.
+template <int FB>
+__global__ __launch_bounds__(256) void k_points_field_fb(int M, const double* __restrict__ pts, int E, int chunk,
+                                                         const double4* __restrict__ q,
+                                                         const double2* __restrict__ qs, long qss, int nfreq,
+                                                         const int* __restrict__ excl_ptr,
+                                                         const int* __restrict__ excl_e,
+                                                         double* __restrict__ partial, long pstride)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = blockIdx.y, f0 = blockIdx.z * FB;
+    const int e0 = s * chunk, e1 = min(E, e0 + chunk);
+    double x0 = 0.0, x1 = 0.0, x2 = 0.0;
+    int xp = 0, xend = 0;
+    if (p < M) {
+        x0 = pts[3 * p];
+        x1 = pts[3 * p + 1];
+        x2 = pts[3 * p + 2];
+        xp = excl_ptr[p];
+        xend = excl_ptr[p + 1];
+        while (xp < xend && excl_e[xp] < e0) xp++;
+    }
+    const double2* qf[FB];   // frequencies past nfreq read the last one (never stored)
+#pragma unroll
+    for (int u = 0; u < FB; u++) qf[u] = qs + (long)min(f0 + u, nfreq - 1) * qss;
+    double acc[FB][6];
+#pragma unroll
+    for (int u = 0; u < FB; u++)
+#pragma unroll
+        for (int o = 0; o < 6; o++) acc[u][o] = 0.0;
+    int nextx = xp < xend ? excl_e[xp] : 0x7fffffff;
+    for (int e = e0; e < e1; e++) {
+        if (e == nextx) {
+            xp++;
+            nextx = xp < xend ? excl_e[xp] : 0x7fffffff;
+            continue;
+        }
+        if (p >= M) continue;
+#pragma unroll
+        for (int g = 0; g < 7; g++) {
+            const double4 y = q[e * 7 + g];
+            const double dx = x0 - y.x, dy = x1 - y.y, dz = x2 - y.z;
+            const double rsq = dx * dx + dy * dy + dz * dz;
+            const double ri = rsqrt_nr(rsq);
+            const double A = y.w * (ri * ri * ri);
+#pragma unroll
+            for (int u = 0; u < FB; u++) {
+                const double2 sg = qf[u][e * 7 + g];
+                const double ar = A * sg.x, ai = A * sg.y;
+                acc[u][0] += ar * dx;
+                acc[u][1] += ai * dx;
+                acc[u][2] += ar * dy;
+                acc[u][3] += ai * dy;
+                acc[u][4] += ar * dz;
+                acc[u][5] += ai * dz;
+            }
+        }
+    }
+    if (p < M)
+#pragma unroll
+        for (int u = 0; u < FB; u++)
+            if (f0 + u < nfreq)
+#pragma unroll
+                for (int o = 0; o < 6; o++) partial[(f0 + u) * pstride + ((long)s * M + p) * 6 + o] = acc[u][o];
+}
+
 // one thread per coincident (point, element) pair
 // Singular part of a point set, in two steps.  The coefficients of each
 // (point, singular element) pair depend on the geometry only and are computed
@@ -155,11 +229,14 @@ template <int NPE, int MODE>
 __global__ __launch_bounds__(256) void k_points_singular(int npairs, const int* __restrict__ pair_ek, int N,
                                                          const int* __restrict__ conn,
                                                          const double* __restrict__ coef,
-                                                         const double* __restrict__ x, double* __restrict__ out)
+                                                         const double* __restrict__ x, double* __restrict__ out,
+                                                         long xs, long os)
 {
     constexpr int NO = MODE ? 6 : 2, NC = 3 * NPE + 3;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= npairs) return;
+    x += blockIdx.y * xs;   // frequency blockIdx.y of a sweep
+    out += blockIdx.y * os;
     const int e = pair_ek[t] >> 3;
     const double* cf = coef + (long)t * NC;
     double res[NO];
@@ -203,11 +280,14 @@ __global__ __launch_bounds__(256) void k_points_singular(int npairs, const int* 
 template <int MODE>
 __global__ void k_points_reduce(int M, int nchunk, const double* __restrict__ partial,
                                 const int* __restrict__ excl_ptr, const double* __restrict__ sing,
-                                double scale, double* __restrict__ out)
+                                double scale, double* __restrict__ out, long ps, long ss, long os)
 {
     constexpr int NO = MODE ? 6 : 2;
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= M) return;
+    partial += blockIdx.y * ps;   // frequency blockIdx.y of a sweep
+    sing += blockIdx.y * ss;
+    out += blockIdx.y * os;
     double acc[NO];
 #pragma unroll
     for (int o = 0; o < NO; o++) acc[o] = 0.0;
@@ -225,10 +305,13 @@ __global__ void k_points_reduce(int M, int nchunk, const double* __restrict__ pa
 template <int NPE>
 __global__ __launch_bounds__(64) void k_mst_elements(int nf, const int* __restrict__ elems, const double* __restrict__ xyz,
                                const int* __restrict__ conn, const double* __restrict__ Ef,
-                               double half_eps, double* __restrict__ Fce, double* __restrict__ Xce)
+                               double half_eps, double* __restrict__ Fce, double* __restrict__ Xce, long es, long rs)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nf) return;
+    Ef += blockIdx.y * es;   // frequency blockIdx.y of a sweep
+    Fce += blockIdx.y * rs;
+    Xce += blockIdx.y * rs;
     const int e = elems[t];
     double X[NPE][3], EE[NPE][3];
 #pragma unroll
@@ -276,10 +359,12 @@ __global__ __launch_bounds__(64) void k_mst_elements(int nf, const int* __restri
 
 // scatter Re E of the evaluated points to node slots: Ef[node] = (ReEx, ReEy, ReEz)
 __global__ void k_field_to_nodes(int m, const int* __restrict__ nodes, const double* __restrict__ ef,
-                                 double* __restrict__ Ef)
+                                 double* __restrict__ Ef, long efs, long es)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= m) return;
+    ef += blockIdx.y * efs;   // frequency blockIdx.y of a sweep
+    Ef += blockIdx.y * es;
     const int v = nodes[t];
     Ef[3 * v] = ef[6 * t];
     Ef[3 * v + 1] = ef[6 * t + 2];
@@ -287,8 +372,10 @@ __global__ void k_field_to_nodes(int m, const int* __restrict__ nodes, const dou
 }
 
 // single-workgroup fixed-order sum of the element forces
-__global__ __launch_bounds__(256) void k_sum3(int n, const double* __restrict__ v, double* __restrict__ out)
+__global__ __launch_bounds__(256) void k_sum3(int n, const double* __restrict__ v, double* __restrict__ out, long rs)
 {
+    v += blockIdx.x * rs;   // frequency blockIdx.x of a sweep
+    out += blockIdx.x * rs;
     __shared__ double sh[3][256];
     const int t = threadIdx.x;
     double a = 0.0, b = 0.0, c = 0.0;
@@ -407,10 +494,10 @@ hipError_t launch_sigma_at_points(dep_sys* s)
     const int n = g->E * 7;
     if (g->npe == 3)
         hipLaunchKernelGGL(k_sigma_at_points<3>, dim3(dep_cdiv(n, 256)), dim3(256), 0, g->ctx->stream,
-                           g->E, g->N, g->conn, s->x, (double2*)s->qs);
+                           g->E, g->N, g->conn, s->x, (double2*)s->qs, 0L, 0L);
     else
         hipLaunchKernelGGL(k_sigma_at_points<6>, dim3(dep_cdiv(n, 256)), dim3(256), 0, g->ctx->stream,
-                           g->E, g->N, g->conn, s->x, (double2*)s->qs);
+                           g->E, g->N, g->conn, s->x, (double2*)s->qs, 0L, 0L);
     return hipGetLastError();
 }
 
@@ -456,7 +543,7 @@ hipError_t launch_points(dep_sys* s, int mode, const dep_points* P, double* out)
         dim3 gs(dep_cdiv(P->npairs, 256));
 #define SING(NPE_, MODE_)                                                                         \
     hipLaunchKernelGGL((k_points_singular<NPE_, MODE_>), gs, dim3(256), 0, st, P->npairs, P->pair_ek, \
-                       g->N, g->conn, P->coef, s->x, P->sing)
+                       g->N, g->conn, P->coef, s->x, P->sing, 0L, 0L)
         if (g->npe == 3) {
             if (mode == 0) SING(3, 0); else SING(3, 1);
         } else {
@@ -469,10 +556,10 @@ hipError_t launch_points(dep_sys* s, int mode, const dep_points* P, double* out)
     const double C = 1.0 / (DEP_PI4 * DEP_EPS0);
     if (mode == 0)
         hipLaunchKernelGGL(k_points_reduce<0>, dim3(dep_cdiv(M, 256)), dim3(256), 0, st, M, nchunk,
-                           P->partial, P->excl_ptr, P->sing, C, out);
+                           P->partial, P->excl_ptr, P->sing, C, out, 0L, 0L, 0L);
     else
         hipLaunchKernelGGL(k_points_reduce<1>, dim3(dep_cdiv(M, 256)), dim3(256), 0, st, M, nchunk,
-                           P->partial, P->excl_ptr, P->sing, C, out);
+                           P->partial, P->excl_ptr, P->sing, C, out, 0L, 0L, 0L);
     (void)NO;
     return hipGetLastError();
 }
@@ -484,14 +571,85 @@ hipError_t launch_mst(dep_sys* s, const dep_points* P, const double* ef, const i
     dep_geom* g = s->g;
     hipStream_t st = g->ctx->stream;
     hipLaunchKernelGGL(k_field_to_nodes, dim3(dep_cdiv(P->m, 256)), dim3(256), 0, st, P->m, nodes,
-                       ef, Efn);
+                       ef, Efn, 0L, 0L);
     if (g->npe == 3)
         hipLaunchKernelGGL(k_mst_elements<3>, dim3(dep_cdiv(nf, 64)), dim3(64), 0, st, nf, felems,
-                           g->xyz, g->conn, Efn, half_eps, Fce, Xce);
+                           g->xyz, g->conn, Efn, half_eps, Fce, Xce, 0L, 0L);
     else
         hipLaunchKernelGGL(k_mst_elements<6>, dim3(dep_cdiv(nf, 64)), dim3(64), 0, st, nf, felems,
-                           g->xyz, g->conn, Efn, half_eps, Fce, Xce);
-    hipLaunchKernelGGL(k_sum3, dim3(1), dim3(256), 0, st, nf, Fce, Fcm);
+                           g->xyz, g->conn, Efn, half_eps, Fce, Xce, 0L, 0L);
+    hipLaunchKernelGGL(k_sum3, dim3(1), dim3(256), 0, st, nf, Fce, Fcm, 0L);
+    return hipGetLastError();
+}
+
+size_t mst_batch_scratch(const dep_geom* g, const dep_points* P, int nfreq)
+{
+    const size_t per = (size_t)g->E * 7 * 2 + (size_t)P->nchunk * P->m * 6 + (size_t)P->npairs * 6 +
+                       (size_t)P->m * 6 + 3 * (size_t)g->N;
+    return per * nfreq;
+}
+
+hipError_t launch_mst_batch(dep_sys* s, const dep_points* P, const double* X, long xs, int nfreq, const int* nodes,
+                            int nf, const int* felems, double half_eps, double* scratch, double* res, long rs)
+{
+    dep_geom* g = s->g;
+    hipStream_t st = g->ctx->stream;
+    const int M = P->m, E = g->E, N = g->N;
+    const long qss = 7L * E * 2, ps = (long)P->nchunk * M * 6, ss = (long)P->npairs * 6, os = (long)M * 6,
+               es = 3L * N;
+    double* qs = scratch;
+    double* partial = qs + qss * nfreq;
+    double* sing = partial + ps * nfreq;
+    double* out = sing + ss * nfreq;
+    double* Efn = out + os * nfreq;
+    const int n = E * 7;
+    if (g->npe == 3)
+        hipLaunchKernelGGL(k_sigma_at_points<3>, dim3(dep_cdiv(n, 256), nfreq), dim3(256), 0, st, E, N, g->conn, X,
+                           (double2*)qs, xs, qss / 2);
+    else
+        hipLaunchKernelGGL(k_sigma_at_points<6>, dim3(dep_cdiv(n, 256), nfreq), dim3(256), 0, st, E, N, g->conn, X,
+                           (double2*)qs, xs, qss / 2);
+    constexpr int FB = 8;
+    const int chunk = dep_cdiv(E, P->nchunk);
+    hipEvent_t ev;
+    prof_begin(g->ctx, &ev);
+    hipLaunchKernelGGL(k_points_field_fb<FB>, dim3(dep_cdiv(M, 256), P->nchunk, dep_cdiv(nfreq, FB)), dim3(256), 0, st,
+                       M, P->pts, E, chunk, (const double4*)g->q, (const double2*)qs, qss / 2, nfreq, P->excl_ptr,
+                       P->excl_e, partial, ps);
+    prof_end(g->ctx, ev, "points_field_batch", (double)nfreq * M * E * 7.0 * 26.0, 48.0 * E * 7.0 + 56.0 * M);
+    if (P->npairs > 0) {
+        if (P->coef_mode != 1) {
+            dim3 gs(dep_cdiv(P->npairs, 64));
+            if (g->npe == 3)
+                hipLaunchKernelGGL((k_points_singular_coef<3, 1>), gs, dim3(64), 0, st, P->npairs, P->pair_pt,
+                                   P->pair_ek, P->pts, g->xyz, g->conn, g->normal, P->coef);
+            else
+                hipLaunchKernelGGL((k_points_singular_coef<6, 1>), gs, dim3(64), 0, st, P->npairs, P->pair_pt,
+                                   P->pair_ek, P->pts, g->xyz, g->conn, g->normal, P->coef);
+            P->coef_mode = 1;
+        }
+        dim3 gs(dep_cdiv(P->npairs, 256), nfreq);
+        if (g->npe == 3)
+            hipLaunchKernelGGL((k_points_singular<3, 1>), gs, dim3(256), 0, st, P->npairs, P->pair_ek, N, g->conn,
+                               P->coef, X, sing, xs, ss);
+        else
+            hipLaunchKernelGGL((k_points_singular<6, 1>), gs, dim3(256), 0, st, P->npairs, P->pair_ek, N, g->conn,
+                               P->coef, X, sing, xs, ss);
+    }
+    const double C = 1.0 / (DEP_PI4 * DEP_EPS0);
+    hipLaunchKernelGGL(k_points_reduce<1>, dim3(dep_cdiv(M, 256), nfreq), dim3(256), 0, st, M, P->nchunk, partial,
+                       P->excl_ptr, sing, C, out, ps, ss, os);
+    hipError_t e = hipMemsetAsync(Efn, 0, sizeof(double) * es * nfreq, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_field_to_nodes, dim3(dep_cdiv(M, 256), nfreq), dim3(256), 0, st, M, nodes, out, Efn, os, es);
+    const long nf1 = std::max(nf, 1);
+    if (g->npe == 3)
+        hipLaunchKernelGGL(k_mst_elements<3>, dim3(dep_cdiv(nf, 64), nfreq), dim3(64), 0, st, nf, felems, g->xyz,
+                           g->conn, Efn, half_eps, res, res + 3 * nf1, es, rs);
+    else
+        hipLaunchKernelGGL(k_mst_elements<6>, dim3(dep_cdiv(nf, 64), nfreq), dim3(64), 0, st, nf, felems, g->xyz,
+                           g->conn, Efn, half_eps, res, res + 3 * nf1, es, rs);
+    hipLaunchKernelGGL(k_sum3, dim3(nfreq), dim3(256), 0, st, nf, res, res + 6 * nf1, rs);
     return hipGetLastError();
 }
 

@@ -363,29 +363,29 @@ extern "C" int dep_sweep_system(dep_sweep* w, int f, dep_sys** sys)
     return DEP_OK;
 }
 
-// Per frequency: sigma_f into the sweep's system and the MST kernels into that
-// frequency's slot of one device buffer, all enqueued; one download at the end.
+// The MST forces of all frequencies in one set of launches (the field kernel
+// contracts each geometric term with eight frequencies' sigma at once); every
+// frequency's numbers are the bits a single-system MST gives; one download.
 extern "C" int dep_sweep_mst(dep_sweep* w, double eps_medium, double* Fcm, double* Xcm)
 {
     if (!w || !Fcm || !Xcm) return DEP_ERR_ARG;
     dep_geom* g = w->g;
     dep_ctx* c = g->ctx;
-    size_t nres = 0;
+    size_t nres = 0, nscr = 0;
     DEP_CHECK(dep_mst_result_size(g, &nres));
-    if (!w->mres || w->mres_n < nres * w->nf) {
+    DEP_CHECK(dep_mst_batch_scratch(g, w->nf, &nscr));
+    const size_t need = nres * w->nf + nscr;   // results, then the batch's scratch
+    if (!w->mres || w->mres_n < need) {
         if (w->mres) {
             DEP_HIP(c, hipStreamSynchronize(c->stream));
             release(w, w->mres);
             w->mres = nullptr;
         }
-        DEP_CHECK(dep_obj_alloc(w, &w->mres, nres * w->nf));
-        w->mres_n = nres * w->nf;
+        DEP_CHECK(dep_obj_alloc(w, &w->mres, need));
+        w->mres_n = need;
     }
-    for (int f = 0; f < w->nf; f++) {
-        DEP_HIP(c, hipMemcpyAsync(w->sys->x, w->X + (size_t)f * 2 * g->N, sizeof(double) * 2 * g->N,
-                                  hipMemcpyDeviceToDevice, c->stream));
-        DEP_CHECK(dep_mst_enqueue(w->sys, eps_medium, w->mres + (size_t)f * nres));
-    }
+    DEP_CHECK(dep_mst_batch(w->sys, w->X, 2L * g->N, w->nf, eps_medium, w->mres, (long)nres,
+                            w->mres + nres * w->nf));
     std::vector<double> h(nres * w->nf);
     DEP_CHECK(down(c, h.data(), w->mres, h.size()));
     for (int f = 0; f < w->nf; f++)

@@ -10,7 +10,9 @@ products over the shared geometric blocks).
   (Maxwell-stress element forces);
 * batching does not change a result: a frequency solved alone and inside a
   batch gives the same bits (each system's columns of the batched product,
-  its reductions and its Givens recurrence are independent of the others).
+  its reductions and its Givens recurrence are independent of the others),
+  and the frequency-batched Maxwell-stress force of a frequency is the bits
+  the single-system MST gives for that frequency's sigma.
 """
 import os
 
@@ -71,7 +73,11 @@ def test_sweep_matches_fp64_lu(ctx, cfg, nf, check):
         assert all(x["converged"] == 1 for x in info), [x["residual"] for x in info]
         F, Xcm = w.mst()
         for k in check:
+            # the batched MST of all frequencies gives each frequency the single-system bits
             s.set_frequency(freqs[k])
+            s.set_solution(w.solution(k))
+            _, _, F1, _ = s.mst()
+            assert np.array_equal(F1, F[k]), (k, F1, F[k])
             s.factor("fp64")
             want, _ = s.solve(tol=1e-13, max_steps=3)
             assert rel(sig[k], want) < 1e-10, (k, rel(sig[k], want))
