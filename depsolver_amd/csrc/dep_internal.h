@@ -337,9 +337,15 @@ size_t bj_floats(int N, int nsys);
 // inverse diagonal 64-blocks of nsys systems (weights wr/wi/sh + sys * wstride) into B
 hipError_t launch_bj_build(dep_ctx* c, int N, const double* P, long ldP, const double* wr, const double* wi,
                            const double* sh, long wstride, int nsys, double* scratch, float* B);
+// the same with the blocks stored in BF16 (bit patterns; bj_floats elements)
+hipError_t launch_bj_build(dep_ctx* c, int N, const double* P, long ldP, const double* wr, const double* wi,
+                           const double* sh, long wstride, int nsys, double* scratch, uint16_t* B);
 // Y_j = M_j^-1 X_j (accumulate: Y_j += M_j^-1 X_j) for the systems of L
 // stop: system sys is skipped when stop[sys * sstride] != 0
 // scale (one system only): X is first scaled in place by *scale unless it is 0
 hipError_t launch_bj_apply(dep_ctx* c, int N, const float* B, const dep_blist& L, double* X, long xstride,
+                           double* Y, long ystride, int accumulate = 0, const int* stop = nullptr, int sstride = 0,
+                           const double* scale = nullptr);
+hipError_t launch_bj_apply(dep_ctx* c, int N, const uint16_t* B, const dep_blist& L, double* X, long xstride,
                            double* Y, long ystride, int accumulate = 0, const int* stop = nullptr, int sstride = 0,
                            const double* scale = nullptr);

@@ -52,8 +52,23 @@ __global__ __launch_bounds__(DIAG_THREADS) void k_bj_factor(int nblk, double* __
     diag_factor_invert<double>(BJ, zr, zr + BJ * BJ, BJ, 0, inv + q * 4 * BJ * BJ, piv + q);
 }
 
-// B = U^-1 L^-1 of each block (complex, FP64 arithmetic), stored in FP32 split planes
-__global__ __launch_bounds__(256) void k_bj_combine(int nblk, const double* __restrict__ inv, float* __restrict__ B)
+// Storage of the inverse blocks: FP32, or BF16 (the bits of a bfloat16 in a
+// uint16_t, rounded to nearest even) where the blocks of many systems are
+// streamed per step; a preconditioner only changes the operator GMRES sees,
+// and the same blocks map the Krylov solution back, so the solve stays exact.
+__device__ __forceinline__ void bj_store(float* p, double v) { *p = (float)v; }
+__device__ __forceinline__ void bj_store(uint16_t* p, double v)
+{
+    uint32_t u = __float_as_uint((float)v);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    *p = (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ double bj_load(const float* p) { return *p; }
+__device__ __forceinline__ double bj_load(const uint16_t* p) { return __uint_as_float((uint32_t)*p << 16); }
+
+// B = U^-1 L^-1 of each block (complex, FP64 arithmetic), stored in split planes
+template <class TB>
+__global__ __launch_bounds__(256) void k_bj_combine(int nblk, const double* __restrict__ inv, TB* __restrict__ B)
 {
     __shared__ double Lr[BJ][BJ + 1], Li[BJ][BJ + 1];
     const long q = (long)blockIdx.y * nblk + blockIdx.x;
@@ -64,8 +79,8 @@ __global__ __launch_bounds__(256) void k_bj_combine(int nblk, const double* __re
         Li[idx / BJ][idx % BJ] = L[2 * idx + 1];
     }
     __syncthreads();
-    float* br = B + q * 2 * BJ * BJ;
-    float* bi = br + BJ * BJ;
+    TB* br = B + q * 2 * BJ * BJ;
+    TB* bi = br + BJ * BJ;
     for (int idx = threadIdx.x; idx < BJ * BJ; idx += 256) {
         const int r = idx / BJ, c = idx % BJ;
         double sr = 0.0, si = 0.0;
@@ -74,8 +89,8 @@ __global__ __launch_bounds__(256) void k_bj_combine(int nblk, const double* __re
             sr += ur * Lr[k][c] - ui * Li[k][c];
             si += ur * Li[k][c] + ui * Lr[k][c];
         }
-        br[idx] = (float)sr;
-        bi[idx] = (float)si;
+        bj_store(br + idx, sr);
+        bj_store(bi + idx, si);
     }
 }
 
@@ -86,7 +101,8 @@ __global__ __launch_bounds__(256) void k_bj_combine(int nblk, const double* __re
 // With scale (one system), the block's rows of X are first multiplied by
 // *scale (unless it is 0) and written back: the normalisation of a Krylov
 // vector folded into its first use.
-__global__ __launch_bounds__(256) void k_bj_apply(int N, int nblk, const float* __restrict__ B, dep_blist L,
+template <class TB>
+__global__ __launch_bounds__(256) void k_bj_apply(int N, int nblk, const TB* __restrict__ B, dep_blist L,
                                                   double* __restrict__ X, long xstride, double* __restrict__ Y,
                                                   long ystride, int accumulate, const int* __restrict__ stop,
                                                   int sstride, const double* __restrict__ scale)
@@ -114,12 +130,12 @@ __global__ __launch_bounds__(256) void k_bj_apply(int N, int nblk, const float* 
         xi[t] = vi;
     }
     __syncthreads();
-    const float* br = B + (sys * nblk + b) * 2 * BJ * BJ + r * BJ + 16 * q;
-    const float* bi = br + BJ * BJ;
+    const TB* br = B + (sys * nblk + b) * 2 * BJ * BJ + r * BJ + 16 * q;
+    const TB* bi = br + BJ * BJ;
     double sr = 0.0, si = 0.0;
 #pragma unroll
     for (int c = 0; c < 16; c++) {
-        const double ar = br[c], ai = bi[c], vr = xr[16 * q + c], vi = xi[16 * q + c];
+        const double ar = bj_load(br + c), ai = bj_load(bi + c), vr = xr[16 * q + c], vi = xi[16 * q + c];
         sr += ar * vr - ai * vi;
         si += ar * vi + ai * vr;
     }
@@ -147,8 +163,9 @@ size_t bj_blocks(int N) { return (size_t)dep_cdiv(N, BJ); }
 size_t bj_scratch_doubles(int N, int nsys) { return (size_t)nsys * bj_blocks(N) * (6 * BJ * BJ + 1); }
 size_t bj_floats(int N, int nsys) { return (size_t)nsys * bj_blocks(N) * 2 * BJ * BJ; }
 
-hipError_t launch_bj_build(dep_ctx* c, int N, const double* P, long ldP, const double* wr, const double* wi,
-                           const double* sh, long wstride, int nsys, double* scratch, float* B)
+template <class TB>
+static hipError_t bj_build(dep_ctx* c, int N, const double* P, long ldP, const double* wr, const double* wi,
+                           const double* sh, long wstride, int nsys, double* scratch, TB* B)
 {
     const int nblk = (int)bj_blocks(N);
     double* Zb = scratch;
@@ -157,17 +174,41 @@ hipError_t launch_bj_build(dep_ctx* c, int N, const double* P, long ldP, const d
     hipLaunchKernelGGL(k_bj_gather, dim3(nblk, nsys), dim3(256), 0, c->stream, N, P, ldP, wr, wi, sh, wstride, nblk,
                        Zb);
     hipLaunchKernelGGL(k_bj_factor, dim3(nblk, nsys), dim3(DIAG_THREADS), 0, c->stream, nblk, Zb, inv, piv);
-    hipLaunchKernelGGL(k_bj_combine, dim3(nblk, nsys), dim3(256), 0, c->stream, nblk, inv, B);
+    hipLaunchKernelGGL(k_bj_combine<TB>, dim3(nblk, nsys), dim3(256), 0, c->stream, nblk, inv, B);
     return hipGetLastError();
+}
+
+template <class TB>
+static hipError_t bj_apply(dep_ctx* c, int N, const TB* B, const dep_blist& L, double* X, long xstride, double* Y,
+                           long ystride, int accumulate, const int* stop, int sstride, const double* scale)
+{
+    const int nblk = (int)bj_blocks(N);
+    if (scale && L.n != 1) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_bj_apply<TB>, dim3(nblk, L.n), dim3(256), 0, c->stream, N, nblk, B, L, X, xstride, Y,
+                       ystride, accumulate, stop, sstride, scale);
+    return hipGetLastError();
+}
+
+hipError_t launch_bj_build(dep_ctx* c, int N, const double* P, long ldP, const double* wr, const double* wi,
+                           const double* sh, long wstride, int nsys, double* scratch, float* B)
+{
+    return bj_build(c, N, P, ldP, wr, wi, sh, wstride, nsys, scratch, B);
+}
+
+hipError_t launch_bj_build(dep_ctx* c, int N, const double* P, long ldP, const double* wr, const double* wi,
+                           const double* sh, long wstride, int nsys, double* scratch, uint16_t* B)
+{
+    return bj_build(c, N, P, ldP, wr, wi, sh, wstride, nsys, scratch, B);
 }
 
 hipError_t launch_bj_apply(dep_ctx* c, int N, const float* B, const dep_blist& L, double* X, long xstride,
                            double* Y, long ystride, int accumulate, const int* stop, int sstride, const double* scale)
 {
-    const int nblk = (int)bj_blocks(N);
-    if (scale && L.n != 1) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_bj_apply, dim3(nblk, L.n), dim3(256), 0, c->stream, N, nblk, B, L, X, xstride, Y,
-                       ystride, accumulate, stop, sstride, scale);
-    return hipGetLastError();
+    return bj_apply(c, N, B, L, X, xstride, Y, ystride, accumulate, stop, sstride, scale);
 }
 
+hipError_t launch_bj_apply(dep_ctx* c, int N, const uint16_t* B, const dep_blist& L, double* X, long xstride,
+                           double* Y, long ystride, int accumulate, const int* stop, int sstride, const double* scale)
+{
+    return bj_apply(c, N, B, L, X, xstride, Y, ystride, accumulate, stop, sstride, scale);
+}

@@ -44,7 +44,7 @@ struct dep_sweep {
     double* gst = nullptr;        // nf x gs device Givens states (H, cs, sn, g, err, scalars)
     int* gfl = nullptr;           // nf x 4 flags: stop, steps, rc
     bool block = true;            // block-Jacobi right preconditioning (else row scaling)
-    float* bjB = nullptr;         // nf x (N / 64) inverse diagonal blocks
+    uint16_t* bjB = nullptr;      // nf x (N / 64) inverse diagonal blocks, BF16 (the apply streams them)
     double* mres = nullptr;       // nf MST result slots
     size_t mres_n = 0;
     double* hpin = nullptr;       // pinned host staging
