@@ -59,7 +59,8 @@ struct dep_sweep {
 
 namespace {
 
-long rstride_of(int cap) { return 2L * (cap + 2) + DEP_UPD_BLOCKS + (long)(cap + 2) * DEP_DOT_CHUNKS; }
+// per system: h1, h2 (cap + 2 each), norm partials, pass-1 and pass-2 dot partials
+long rstride_of(int cap) { return 2L * (cap + 2) + DEP_UPD_BLOCKS + 2L * (cap + 2) * DEP_DOT_CHUNKS; }
 // doubles of one system's device Givens state: H (cap + 1 columns of cap + 2), cs, sn, g, err, 4 scalars
 long gstate_size(int cap) { return (long)(cap + 1) * (cap + 2) + 4L * (cap + 2) + 4; }
 
@@ -210,7 +211,9 @@ extern "C" int dep_sweep_solve(dep_sweep* w, double tol, double inner_tol, int m
     DEP_CHECK(ensure_p32(g));
     const int N = g->N, nf = w->nf, M = max_inner;
     const long n2 = 2L * N, vstride = (long)(w->cap + 1) * n2, rstride = rstride_of(w->cap);
-    const long h1off = 0, h2off = w->cap + 2, noff = 2L * (w->cap + 2), poff = noff + DEP_UPD_BLOCKS;
+    const long h1off = 0, h2off = w->cap + 2, noff = 2L * (w->cap + 2), poff = noff + DEP_UPD_BLOCKS,
+               poff2 = poff + (long)(w->cap + 2) * DEP_DOT_CHUNKS;
+    const bool chunked = bcgs_chunked(n2);
     double* hdown = w->hpin;
     const double* b = w->sys->b;
 
@@ -266,10 +269,15 @@ extern "C" int dep_sweep_solve(dep_sweep* w, double tol, double inner_tol, int m
                     DEP_HIP(c, launch_bmatvec32(c, N, g->P32, g->ldP, I, w->V + (long)i * n2, vstride, w->part,
                                                 w->GW, 3L * N, w->V + (long)(i + 1) * n2, vstride, w->gfl));
                 }
-                DEP_HIP(c, launch_bcgs_pass(c, i + 2, n2, I, w->V, vstride, i + 1, w->red, rstride, poff, h1off, -1,
-                                            w->gfl));
-                DEP_HIP(c, launch_bcgs_pass(c, i + 2, n2, I, w->V, vstride, i + 1, w->red, rstride, poff, h2off, noff,
-                                            w->gfl));
+                if (chunked) {
+                    DEP_HIP(c, launch_bcgs_step(c, i, n2, I, w->V, vstride, w->red, rstride, poff, poff2, h1off, h2off,
+                                                noff, w->gfl));
+                } else {
+                    DEP_HIP(c, launch_bcgs_pass(c, i + 2, n2, I, w->V, vstride, i + 1, w->red, rstride, poff, h1off,
+                                                -1, w->gfl));
+                    DEP_HIP(c, launch_bcgs_pass(c, i + 2, n2, I, w->V, vstride, i + 1, w->red, rstride, poff, h2off,
+                                                noff, w->gfl));
+                }
                 DEP_HIP(c, launch_bgivens(c, i, w->cap, n2, I, w->red, rstride, h1off, h2off, noff, w->gst, gs, w->gfl,
                                           w->V, vstride));
             }
