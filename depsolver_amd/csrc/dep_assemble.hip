@@ -120,33 +120,53 @@ __global__ __launch_bounds__(TR) void k_assemble_regular(
         }
     }
     const int j0 = patch_ptr[patch], j1 = patch_ptr[patch + 1];
+    // Per element the loads (its node ids, column slots and seven points, all
+    // wave-uniform) are issued together and the next element's id one element
+    // ahead, so an element waits for memory once; a singular pair (the row's
+    // node is a node of the element) is masked out, not branched around (its
+    // points are interior, so the masked terms are finite), and the row kind is
+    // branched on once per element.
+    int e_next = j0 < j1 ? patch_elem[j0] : 0;
     for (int j = j0; j < j1; j++) {
-        const int e = patch_elem[j];
+        const int e = e_next;
+        e_next = patch_elem[min(j + 1, j1 - 1)];
         // the element's column slots, one wave-uniform 8-byte load (byte loads would each be waited for)
         const uint64_t slots = *(const uint64_t*)(patch_slot + (long)j * 8);
+        int cn[NPE];
+#pragma unroll
+        for (int k = 0; k < NPE; k++) cn[k] = conn[e * NPE + k];
+        double4 pq[7];
+#pragma unroll
+        for (int g = 0; g < 7; g++) pq[g] = q[e * 7 + g];
         bool sing = false;
 #pragma unroll
-        for (int k = 0; k < NPE; k++) sing |= (conn[e * NPE + k] == row);
+        for (int k = 0; k < NPE; k++) sing |= (cn[k] == row);
+        const double keep = (active && !sing) ? 1.0 : 0.0;
         double sub[NPE];
 #pragma unroll
         for (int k = 0; k < NPE; k++) sub[k] = 0.0;
-        if (active && !sing) {
+        if (!iface) {
 #pragma unroll
             for (int g = 0; g < 7; g++) {
-                const double4 p = q[e * 7 + g];
+                const double4 p = pq[g];
                 const double dx = xe0 - p.x, dy = xe1 - p.y, dz = xe2 - p.z;
                 const double rsq = dx * dx + dy * dy + dz * dz;
                 const double ri = rsqrt_nr(rsq);
-                if (!iface) {
-                    const double A = p.w * ri;
+                const double A = p.w * ri * keep;
 #pragma unroll
-                    for (int k = 0; k < NPE; k++) sub[k] += A * SF.v[g][k];
-                } else {
-                    double A = p.w * (ri * ri * ri);
-                    A = A * (dx * n0 + dy * n1 + dz * n2);
+                for (int k = 0; k < NPE; k++) sub[k] += A * SF.v[g][k];
+            }
+        } else {
 #pragma unroll
-                    for (int k = 0; k < NPE; k++) sub[k] -= A * SF.v[g][k];
-                }
+            for (int g = 0; g < 7; g++) {
+                const double4 p = pq[g];
+                const double dx = xe0 - p.x, dy = xe1 - p.y, dz = xe2 - p.z;
+                const double rsq = dx * dx + dy * dy + dz * dz;
+                const double ri = rsqrt_nr(rsq);
+                double A = p.w * (ri * ri * ri);
+                A = A * (dx * n0 + dy * n1 + dz * n2) * keep;
+#pragma unroll
+                for (int k = 0; k < NPE; k++) sub[k] -= A * SF.v[g][k];
             }
         }
 #pragma unroll
