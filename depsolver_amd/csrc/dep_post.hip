@@ -79,10 +79,18 @@ __global__ __launch_bounds__(256) void k_points_regular(
             continue;
         }
         if (p >= M) continue;
+        // the element's seven points and sigma values, loaded together (one wait per element)
+        double4 yq[7];
+        double2 sq[7];
 #pragma unroll
         for (int g = 0; g < 7; g++) {
-            const double4 y = q[e * 7 + g];
-            const double2 sg = qs[e * 7 + g];
+            yq[g] = q[e * 7 + g];
+            sq[g] = qs[e * 7 + g];
+        }
+#pragma unroll
+        for (int g = 0; g < 7; g++) {
+            const double4 y = yq[g];
+            const double2 sg = sq[g];
             const double dx = x0 - y.x, dy = x1 - y.y, dz = x2 - y.z;
             const double rsq = dx * dx + dy * dy + dz * dz;
             const double ri = rsqrt_nr(rsq);
@@ -150,9 +158,12 @@ __global__ __launch_bounds__(256) void k_points_field_fb(int M, const double* __
             continue;
         }
         if (p >= M) continue;
+        double4 yq[7];
+#pragma unroll
+        for (int g = 0; g < 7; g++) yq[g] = q[e * 7 + g];
 #pragma unroll
         for (int g = 0; g < 7; g++) {
-            const double4 y = q[e * 7 + g];
+            const double4 y = yq[g];
             const double dx = x0 - y.x, dy = x1 - y.y, dz = x2 - y.z;
             const double rsq = dx * dx + dy * dy + dz * dz;
             const double ri = rsqrt_nr(rsq);
