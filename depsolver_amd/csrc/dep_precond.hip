@@ -10,6 +10,9 @@
 // complex product per block (right preconditioning: GMRES solves Z M^-1 y = r
 // and the correction is M^-1 y).  Systems of a sweep have their own blocks:
 // every kernel takes a system index in blockIdx.y and strides.
+#include <stdlib.h>
+#include <string.h>
+
 #include "dep_common.h"
 #include "dep_diag.h"
 #include "dep_internal.h"
@@ -163,11 +166,110 @@ size_t bj_blocks(int N) { return (size_t)dep_cdiv(N, BJ); }
 size_t bj_scratch_doubles(int N, int nsys) { return (size_t)nsys * bj_blocks(N) * (6 * BJ * BJ + 1); }
 size_t bj_floats(int N, int nsys) { return (size_t)nsys * bj_blocks(N) * 2 * BJ * BJ; }
 
+// M_b^-1 of every block in one kernel: the block is read straight from P and
+// the weights and inverted in place by Gauss-Jordan elimination without
+// pivoting (as the LU it replaces), one 1024-thread workgroup per block with
+// the block in registers: thread t owns row r = t & 63 and columns 4w .. 4w+3
+// (w = t >> 6).  Step p publishes row p and column p (unscaled) and 1/pivot
+// through double-buffered LDS vectors and synchronises once; then row p is
+// scaled and every other row r subtracts (a_rp / a_pp) times row p, its
+// column p becoming -a_rp / a_pp.  64 steps, against the 127 of the LU with
+// both triangular inverses, and no product of the two inverses afterwards.
+template <class TB>
+__global__ __launch_bounds__(1024) void k_bj_invert(int N, const double* __restrict__ P, long ldP,
+                                                    const double* __restrict__ wr, const double* __restrict__ wi,
+                                                    const double* __restrict__ sh, long wstride, int nblk,
+                                                    TB* __restrict__ B)
+{
+    __shared__ __align__(16) double rr_[2][BJ], ri_[2][BJ];   // row p
+    __shared__ double cr_[2][BJ], ci_[2][BJ];                  // column p
+    __shared__ double ipr_[2], ipi_[2];
+    const int b = blockIdx.x, sys = blockIdx.y;
+    const long wo = (long)sys * wstride;
+    const int t = threadIdx.x, r = t & 63;
+    const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int i = b * BJ + r;
+    double dr[4], di[4];
+    {
+        const bool irow = i < N;
+        const double a = irow ? wr[wo + i] : 0.0, bb = irow ? wi[wo + i] : 0.0, s0 = irow ? sh[wo + i] : 0.0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int c = 4 * w + k, j = b * BJ + c;
+            const bool in = irow && j < N;
+            const double p = P[(long)min(i, N - 1) * ldP + min(j, N - 1)];   // clamped, masked below
+            dr[k] = in ? a * p + (i == j ? s0 : 0.0) : (r == c ? 1.0 : 0.0);
+            di[k] = in ? bb * p : 0.0;
+        }
+    }
+    for (int p = 0; p < BJ; p++) {
+        const int q = p & 1, kp = p & 3;
+        const bool own_col = w == (p >> 2);
+        double vr = 0.0, vi = 0.0;   // this row's entry in column p (owner wave only)
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (k == kp) {
+                vr = dr[k];
+                vi = di[k];
+            }
+        if (own_col) {
+            cr_[q][r] = vr;
+            ci_[q][r] = vi;
+            if (r == p) {
+                const double d = vr * vr + vi * vi;
+                ipr_[q] = vr / d;
+                ipi_[q] = -vi / d;
+            }
+        }
+        if (r == p) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                rr_[q][4 * w + k] = dr[k];
+                ri_[q][4 * w + k] = di[k];
+            }
+        }
+        __syncthreads();
+        const double ir = ipr_[q], ii = ipi_[q];
+        typedef double v4 __attribute__((ext_vector_type(4)));
+        const v4 ar = *(const v4*)&rr_[q][4 * w], ai = *(const v4*)&ri_[q][4 * w];
+        if (r == p) {   // row p: scaled by 1/pivot, its own column p becomes 1/pivot
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const double xr = ar[k] * ir - ai[k] * ii, xi = ar[k] * ii + ai[k] * ir;
+                dr[k] = 4 * w + k == p ? ir : xr;
+                di[k] = 4 * w + k == p ? ii : xi;
+            }
+        } else {
+            const double c0 = cr_[q][r], c1 = ci_[q][r];
+            const double fr = c0 * ir - c1 * ii, fi = c0 * ii + c1 * ir;   // a_rp / a_pp
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const double xr = dr[k] - (fr * ar[k] - fi * ai[k]), xi = di[k] - (fr * ai[k] + fi * ar[k]);
+                dr[k] = 4 * w + k == p ? -fr : xr;
+                di[k] = 4 * w + k == p ? -fi : xi;
+            }
+        }
+    }
+    TB* br = B + ((long)sys * nblk + b) * 2 * BJ * BJ;
+    TB* bi = br + BJ * BJ;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        bj_store(br + r * BJ + 4 * w + k, dr[k]);
+        bj_store(bi + r * BJ + 4 * w + k, di[k]);
+    }
+}
+
 template <class TB>
 static hipError_t bj_build(dep_ctx* c, int N, const double* P, long ldP, const double* wr, const double* wi,
                            const double* sh, long wstride, int nsys, double* scratch, TB* B)
 {
     const int nblk = (int)bj_blocks(N);
+    const char* sel = getenv("DEPSOLVER_BJ_BUILD");
+    if (!(sel && strcmp(sel, "lu") == 0)) {
+        hipLaunchKernelGGL(k_bj_invert<TB>, dim3(nblk, nsys), dim3(1024), 0, c->stream, N, P, ldP, wr, wi, sh, wstride,
+                           nblk, B);
+        return hipGetLastError();
+    }
     double* Zb = scratch;
     double* inv = Zb + (size_t)nsys * nblk * 2 * BJ * BJ;
     double* piv = inv + (size_t)nsys * nblk * 4 * BJ * BJ;
