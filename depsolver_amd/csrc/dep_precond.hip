@@ -4,12 +4,13 @@
 // of the internal (Morton) node order, so a block holds a compact patch of
 // the surface and its strongest near-field couplings (on a shelled particle
 // the two close surfaces land in the same or neighbouring blocks).  Each
-// block is factored without pivoting by the register-resident 64x64 LU of
-// dep_diag.h, which also forms L^-1 and U^-1; the preconditioner keeps
-// M_b^-1 = U^-1 L^-1 in FP32 split planes and applies it as one dense 64x64
-// complex product per block (right preconditioning: GMRES solves Z M^-1 y = r
-// and the correction is M^-1 y).  Systems of a sweep have their own blocks:
-// every kernel takes a system index in blockIdx.y and strides.
+// block is inverted without pivoting by an in-register Gauss-Jordan
+// elimination (k_bj_invert; DEPSOLVER_BJ_BUILD=lu selects the LU of dep_diag.h
+// with both triangular inverses and their product instead), and M_b^-1 is
+// kept in FP32 (BF16 for sweeps) split planes and applied as one dense 64x64
+// complex product per block (right preconditioning: GMRES solves
+// Z M^-1 y = r and the correction is M^-1 y).  Systems of a sweep have their
+// own blocks: every kernel takes a system index in blockIdx.y and strides.
 #include <stdlib.h>
 #include <string.h>
 
