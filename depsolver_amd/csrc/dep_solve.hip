@@ -122,6 +122,107 @@ __global__ __launch_bounds__(64 * WPB) void k_zmatvec(int N, const TP* __restric
     }
 }
 
+// The FP32-block product with x staged in LDS: a workgroup of WPB >= 4 waves
+// (ZROWS rows each) loads each pair of 256-column chunks of x (re and im, 8 KB)
+// once into a double-buffered LDS slot, one barrier per pair, so x is read
+// from L2 once per workgroup instead of once per wave.  Per row the sums run
+// in the order of k_zmatvec<float, double, ZROWS, 2, WPB, true> (chunk pairs in
+// order, the same columns per lane, the same tail and reduction), so the
+// results are those bits.
+template <int ZROWS, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_zmatvec32_lx(int N, const float* __restrict__ P, long ldP,
+                                                           const double* __restrict__ wr,
+                                                           const double* __restrict__ wi,
+                                                           const double* __restrict__ shift,
+                                                           const double* __restrict__ x,
+                                                           const double* __restrict__ b,
+                                                           double* __restrict__ y, const int* __restrict__ stop)
+{
+    static_assert(WPB >= 4, "256 threads stage a chunk pair of x");
+    if (stop && *stop) return;   // uniform: every wave of the workgroup returns
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    typedef double d4 __attribute__((ext_vector_type(4)));
+    __shared__ d4 xs[2][2][128];   // [buffer][re, im][2 chunks x 64 lanes]
+    const int t = threadIdx.x, lane = t & 63;
+    const int row0 = (blockIdx.x * WPB + (t >> 6)) * ZROWS;
+    const int nr = max(0, min(ZROWS, N - row0));   // rows past N read row N - 1 (never stored)
+    const int last = max(nr, 1) - 1;
+    double sr[ZROWS], si[ZROWS];
+#pragma unroll
+    for (int k = 0; k < ZROWS; k++) sr[k] = si[k] = 0.0;
+    const int full = N / 512 * 512, niter = full / 512;
+    const int sq = t & 127, shalf = (t >> 7) & 1;   // staging: d4 number sq of half shalf (t < 256)
+    d4 xn = {0.0, 0.0, 0.0, 0.0};
+    if (niter > 0 && t < 256) xn = *(const d4*)(x + (long)shalf * N + 4 * sq);
+    if (t < 256) xs[0][shalf][sq] = xn;
+    __syncthreads();
+    for (int it = 0; it < niter; it++) {
+        const int buf = it & 1;
+        f4 v[2][ZROWS];
+#pragma unroll
+        for (int u = 0; u < 2; u++)
+#pragma unroll
+            for (int k = 0; k < ZROWS; k++)
+                v[u][k] = __builtin_nontemporal_load(
+                    (const f4*)(P + (long)min(row0 + min(k, last), N - 1) * ldP + it * 512 + 256 * u + 4 * lane));
+        if (it + 1 < niter && t < 256) xn = *(const d4*)(x + (long)shalf * N + (it + 1) * 512 + 4 * sq);
+        d4 xr[2], xi[2];
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            xr[u] = xs[buf][0][64 * u + lane];
+            xi[u] = xs[buf][1][64 * u + lane];
+        }
+#pragma unroll
+        for (int u = 0; u < 2; u++)
+#pragma unroll
+            for (int k = 0; k < ZROWS; k++)
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    sr[k] += (double)v[u][k][e] * xr[u][e];
+                    si[k] += (double)v[u][k][e] * xi[u][e];
+                }
+        if (it + 1 < niter && t < 256) xs[buf ^ 1][shalf][sq] = xn;
+        __syncthreads();
+    }
+    if (nr == 0) return;
+    for (int c = full + lane; c < N; c += 64) {
+        const double xr = x[c], xi = x[N + c];
+#pragma unroll
+        for (int k = 0; k < ZROWS; k++)
+            if (k < nr) {
+                const double v = (double)P[(long)(row0 + k) * ldP + c];
+                sr[k] += v * xr;
+                si[k] += v * xi;
+            }
+    }
+#pragma unroll
+    for (int k = 0; k < ZROWS; k++)
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            sr[k] += __shfl_xor(sr[k], o);
+            si[k] += __shfl_xor(si[k], o);
+        }
+    if (lane < nr) {
+        double zr = 0.0, zi = 0.0;
+#pragma unroll
+        for (int k = 0; k < ZROWS; k++)
+            if (k == lane) {
+                zr = sr[k];
+                zi = si[k];
+            }
+        const int row = row0 + lane;
+        const double a = wr[row], bb = wi[row], sh = shift[row];
+        double yr = a * zr - bb * zi + sh * x[row];
+        double yi = a * zi + bb * zr + sh * x[N + row];
+        if (b) {
+            yr = b[row] - yr;
+            yi = b[N + row] - yi;
+        }
+        y[row] = yr;
+        y[N + row] = yi;
+    }
+}
+
 // The FP64 product keeps the load-then-accumulate order per row (ZROWS rows
 // per wave, WPB waves): loading all rows of a chunk first, as the FP32 kernel
 // does, measured 0.40 ms against 0.32 ms for this form (4 rows, 4 waves) at
@@ -234,8 +335,10 @@ static hipError_t zmatvec64_shape(dep_geom* g, int k, const double* wr, const do
     return hipGetLastError();
 }
 
-// Shapes of the FP32-block product (autotuned): rows per wave, two chunks in
-// flight, one wave per workgroup
+// Shapes of the FP32-block product (autotuned): x staged in LDS for 4 waves of
+// 4 or 8 rows each (4 rows: 142 us against 176 us for 6 rows in one wave in a
+// back-to-back timing at N = 15,044), or x read by each wave (6 or 4 rows per
+// wave); two chunks in flight in every shape
 static hipError_t zmatvec32_shape(dep_geom* g, int k, const double* wr, const double* wi, const double* shift,
                                   const double* x, const double* b, double* y, const int* stop)
 {
@@ -243,20 +346,20 @@ static hipError_t zmatvec32_shape(dep_geom* g, int k, const double* wr, const do
     hipStream_t st = g->ctx->stream;
     switch (k) {
     case 1:
-        hipLaunchKernelGGL((k_zmatvec<float, double, 4, 2, 1, true>), dim3(dep_cdiv(N, 4)), dim3(64), 0, st, N,
-                           g->P32, g->ldP, wr, wi, shift, x, b, y, stop);
+        hipLaunchKernelGGL((k_zmatvec32_lx<8, 4>), dim3(dep_cdiv(N, 32)), dim3(256), 0, st, N, g->P32, g->ldP, wr, wi,
+                           shift, x, b, y, stop);
         break;
     case 2:
-        hipLaunchKernelGGL((k_zmatvec<float, double, 3, 2, 1, true>), dim3(dep_cdiv(N, 3)), dim3(64), 0, st, N,
+        hipLaunchKernelGGL((k_zmatvec<float, double, 6, 2, 1, true>), dim3(dep_cdiv(N, 6)), dim3(64), 0, st, N,
                            g->P32, g->ldP, wr, wi, shift, x, b, y, stop);
         break;
     case 3:
-        hipLaunchKernelGGL((k_zmatvec<float, double, 8, 2, 1, true>), dim3(dep_cdiv(N, 8)), dim3(64), 0, st, N,
+        hipLaunchKernelGGL((k_zmatvec<float, double, 4, 2, 1, true>), dim3(dep_cdiv(N, 4)), dim3(64), 0, st, N,
                            g->P32, g->ldP, wr, wi, shift, x, b, y, stop);
         break;
     default:
-        hipLaunchKernelGGL((k_zmatvec<float, double, 6, 2, 1, true>), dim3(dep_cdiv(N, 6)), dim3(64), 0, st, N,
-                           g->P32, g->ldP, wr, wi, shift, x, b, y, stop);
+        hipLaunchKernelGGL((k_zmatvec32_lx<4, 4>), dim3(dep_cdiv(N, 16)), dim3(256), 0, st, N, g->P32, g->ldP, wr, wi,
+                           shift, x, b, y, stop);
     }
     return hipGetLastError();
 }

@@ -42,7 +42,7 @@ int g_mode = -1;                                       // -1: read DEPSOLVER_TUN
 
 const char* const kKnobName[KNOB_COUNT] = {"zmatvec32", "zmatvec64", "assemble"};
 const char* const kCandName[KNOB_COUNT][4] = {
-    {"6 rows/wave", "4 rows/wave", "3 rows/wave", "8 rows/wave"},
+    {"x in LDS, 4 rows/wave, 4 waves", "x in LDS, 8 rows/wave, 4 waves", "6 rows/wave", "4 rows/wave"},
     {"4 rows/wave, 4 waves", "2 rows/wave, 4 waves", "4 rows/wave, 1 wave", ""},
     {"16 columns/patch", "8 columns/patch", "", ""},
 };
