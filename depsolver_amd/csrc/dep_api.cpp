@@ -112,10 +112,9 @@ extern "C" int dep_ctx_create(int device, dep_ctx** out)
     if (device < 0 || device >= n) return DEP_ERR_ARG;
     dep_ctx* c = new dep_ctx();
     c->device = device;
-    // the main stream at the default priority; the LU's second stream is made on first use
-    // (streams share the process's few hardware queues, and a context that never factors
-    // should not take one: serving slots running GMRES then each keep a queue of their own)
+    // both streams at the default priority
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
         hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c->join, hipEventDisableTiming) != hipSuccess) {
@@ -134,7 +133,7 @@ extern "C" int dep_ctx_destroy(dep_ctx* c)
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->stream2) (void)hipStreamSynchronize(c->stream2);
-    if (c->stream) prof_collect(c);
+    if (c->stream && c->stream2) prof_collect(c);
     for (hipEvent_t e : c->pool) (void)hipEventDestroy(e);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -197,7 +196,7 @@ void prof_collect(dep_ctx* c)
 {
     if (c->pending.empty()) return;
     (void)hipStreamSynchronize(c->stream);
-    if (c->stream2) (void)hipStreamSynchronize(c->stream2);
+    (void)hipStreamSynchronize(c->stream2);
     for (auto& p : c->pending) {
         float ms = 0.0f;
         if (p.scale > 0.0 && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {

@@ -1099,10 +1099,6 @@ static hipError_t lu_blocked(dep_sys* s, T* Wr, T* Wi, long ld, T* inv, T* piv, 
 {
     dep_geom* g = s->g;
     dep_ctx* c = g->ctx;
-    if (!c->stream2) {   // the lookahead's second stream, default priority, made on first use
-        const hipError_t es = hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking);
-        if (es != hipSuccess) return es;
-    }
     hipStream_t st = c->stream, st2 = c->stream2;
     const int N = g->N;
     hipError_t e;
