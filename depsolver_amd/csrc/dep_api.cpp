@@ -112,7 +112,9 @@ extern "C" int dep_ctx_create(int device, dep_ctx** out)
     if (device < 0 || device >= n) return DEP_ERR_ARG;
     dep_ctx* c = new dep_ctx();
     c->device = device;
-    // both streams at the default priority
+    // both streams at the default priority, created together: streams share the process's
+    // hardware queues in creation order, so the lookahead stream lands on another queue than
+    // this context's main stream
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
