@@ -583,10 +583,10 @@ hipError_t launch_mst(dep_sys* s, const dep_points* P, const double* ef, const i
     hipStream_t st = g->ctx->stream;
     hipLaunchKernelGGL(k_field_to_nodes, dim3(dep_cdiv(P->m, 256)), dim3(256), 0, st, P->m, nodes,
                        ef, Efn, 0L, 0L);
-    if (g->npe == 3)
+    if (nf > 0 && g->npe == 3)   // (no force elements: the total below is zero)
         hipLaunchKernelGGL(k_mst_elements<3>, dim3(dep_cdiv(nf, 64)), dim3(64), 0, st, nf, felems,
                            g->xyz, g->conn, Efn, half_eps, Fce, Xce, 0L, 0L);
-    else
+    else if (nf > 0)
         hipLaunchKernelGGL(k_mst_elements<6>, dim3(dep_cdiv(nf, 64)), dim3(64), 0, st, nf, felems,
                            g->xyz, g->conn, Efn, half_eps, Fce, Xce, 0L, 0L);
     hipLaunchKernelGGL(k_sum3, dim3(1), dim3(256), 0, st, nf, Fce, Fcm, 0L);
@@ -654,10 +654,10 @@ hipError_t launch_mst_batch(dep_sys* s, const dep_points* P, const double* X, lo
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_field_to_nodes, dim3(dep_cdiv(M, 256), nfreq), dim3(256), 0, st, M, nodes, out, Efn, os, es);
     const long nf1 = std::max(nf, 1);
-    if (g->npe == 3)
+    if (nf > 0 && g->npe == 3)
         hipLaunchKernelGGL(k_mst_elements<3>, dim3(dep_cdiv(nf, 64), nfreq), dim3(64), 0, st, nf, felems, g->xyz,
                            g->conn, Efn, half_eps, res, res + 3 * nf1, es, rs);
-    else
+    else if (nf > 0)
         hipLaunchKernelGGL(k_mst_elements<6>, dim3(dep_cdiv(nf, 64), nfreq), dim3(64), 0, st, nf, felems, g->xyz,
                            g->conn, Efn, half_eps, res, res + 3 * nf1, es, rs);
     hipLaunchKernelGGL(k_sum3, dim3(nfreq), dim3(256), 0, st, nf, res, res + 6 * nf1, rs);
