@@ -98,6 +98,7 @@ def lib():
         "dep_post_field": ([_p, i, _f64p, _f64p], i),
         "dep_post_mst": ([_p, d, _f64p, _f64p, _f64p, _f64p], i),
         "dep_post_multipole": ([_p, i, d, i, _f64p, _f64p], i),
+        "dep_post_ellipsoid": ([_p, i, _f64p, i, _f64p, _f64p], i),
         "dep_sweep_create": ([_p, i, _f64p, i, _u32p, i, _f64p, _f64p, ctypes.POINTER(_p)], i),
         "dep_sweep_destroy": ([_p], i),
         "dep_sweep_solve": ([_p, d, d, i, i, _p, _p], i),
@@ -299,6 +300,16 @@ class System:
         F = np.zeros((len(pts), 3))
         self.ctx.check(lib().dep_post_multipole(self._h, order, radius, len(pts), pts, F),
                        "dep_post_multipole")
+        return F
+
+    def ellipsoid(self, pts, axis, analysis=10):
+        """Dipolar DEP force on an ellipsoid of semi-axes `axis` centred at each
+        point: analysis 10 homogeneous, 11 single-shell oblate (force.c
+        forceEllipsoid_*)."""
+        pts = _c64(pts).reshape(-1, 3)
+        F = np.zeros((len(pts), 3))
+        self.ctx.check(lib().dep_post_ellipsoid(self._h, analysis, _c64(np.asarray(axis, float).reshape(3)),
+                                                len(pts), pts, F), "dep_post_ellipsoid")
         return F
 
     def close(self):

@@ -1187,6 +1187,137 @@ extern "C" int dep_post_multipole(dep_sys* s, int order, double R, int m, const 
     return DEP_OK;
 }
 
+// 64-point Gauss-Legendre rule on (-1, 1), ascending: Newton's method on P_64,
+// weights 2 / ((1 - x^2) P'_64(x)^2).  The reference tabulates the same rule
+// to 15 digits (src/gaussData.h:76, GaussH).
+namespace {
+struct GaussLegendre64 {
+    double x[64], w[64];
+    GaussLegendre64()
+    {
+        const int n = 64;
+        auto legendre = [n](double z, double* dp) {
+            double p0 = 1.0, p1 = z;
+            for (int k = 2; k <= n; k++) {
+                const double p2 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
+                p0 = p1;
+                p1 = p2;
+            }
+            *dp = n * (z * p1 - p0) / (z * z - 1.0);
+            return p1;
+        };
+        for (int i = 0; i < n / 2; i++) {
+            double z = cos(3.14159265358979323846 * (i + 0.75) / (n + 0.5)), dp = 0.0;
+            for (int it = 0; it < 100; it++) {
+                const double dz = legendre(z, &dp) / dp;
+                z -= dz;
+                if (fabs(dz) < 1e-17) break;
+            }
+            legendre(z, &dp);
+            const double wt = 2.0 / ((1.0 - z * z) * dp * dp);
+            x[i] = -z;
+            w[i] = wt;
+            x[n - 1 - i] = z;
+            w[n - 1 - i] = wt;
+        }
+    }
+};
+}   // namespace
+
+// Depolarisation factors of an ellipsoid (src/depolarization.c:44-115): the
+// 64-point rule on L_k = (abc/2) int_0^{100 a_k^2} ds / ((s + a_k^2)
+// sqrt((s+a^2)(s+b^2)(s+c^2))), the infinite integral truncated where the
+// reference truncates it.
+static int depolarization_factors(const double* axis, double* Ld)
+{
+    static const GaussLegendre64 gl;
+    const double A = 0.5 * axis[0] * axis[1] * axis[2];
+    const double a2[3] = {axis[0] * axis[0], axis[1] * axis[1], axis[2] * axis[2]};
+    for (int k = 0; k < 3; k++) {
+        const double top = 100.0 * a2[k];
+        const double J = 0.5 * top;
+        double acc = 0.0;
+        for (int i = 0; i < 64; i++) {
+            const double t = 0.5 * (1.0 + gl.x[i]) * top;
+            acc += J * gl.w[i] / sqrt((t + a2[0]) * (t + a2[1]) * (t + a2[2])) / (t + a2[k]);
+        }
+        Ld[k] = A * acc;
+    }
+    return (Ld[0] < 0.0 || Ld[1] < 0.0 || Ld[2] < 0.0) ? -1 : 0;
+}
+
+// Dipolar DEP force on an ellipsoid at each point (force.c:65-183
+// forceEllipsoid_tria3, :204-322 forceEllipsoid_tria6): Re E at the point and
+// its first derivatives (the order-2 derivative integrals of the potential,
+// from the device), the generalised Clausius-Mossotti factor per axis, and
+// F_j = (2 pi eps_m abc / 3) sum_k K_k E_k dE_k/dx_j.  For analysis 11 the
+// reference scales the particle's (sigma, eps_r) by C = ac / (2c + a) in its
+// material table on every call (force.c:148-155): point k here sees them
+// scaled by C^(k+1); the system's own table is left as it is.
+extern "C" int dep_post_ellipsoid(dep_sys* s, int analysis, const double* axis, int m, const double* pts,
+                                  double* F)
+{
+    if (!s || !axis || m < 0 || (m > 0 && (!pts || !F))) return DEP_ERR_ARG;
+    dep_geom* g = s->g;
+    dep_ctx* c = g->ctx;
+    if (analysis != 10 && analysis != 11)
+        return dep_fail(c, DEP_ERR_ARG, "dep_post_ellipsoid: analysis must be 10 or 11, got %d", analysis);
+    if (!(axis[0] > 0.0 && axis[1] > 0.0 && axis[2] > 0.0))
+        return dep_fail(c, DEP_ERR_ARG, "dep_post_ellipsoid: semi-axes must be positive");
+    if (s->nM < 2) return dep_fail(c, DEP_ERR_ARG, "dep_post_ellipsoid: needs materials 1 (medium) and 2 (particle)");
+    if (m == 0) return DEP_OK;
+    double Ld[3];
+    if (depolarization_factors(axis, Ld) != 0)
+        return dep_fail(c, DEP_ERR_ARG, "dep_post_ellipsoid: depolarization factors must be positive");
+    std::vector<double> ef((size_t)m * 6), terms((size_t)m * 31);
+    DEP_CHECK(eval_points(s, 1, m, pts, ef.data()));
+    double* dp = nullptr;
+    double* dt = nullptr;
+    struct Tmp {
+        dep_ctx* c;
+        std::vector<std::pair<void*, size_t>> allocs;
+        dep_ctx* ctx_of() { return c; }
+        ~Tmp() { for (auto& a : allocs) dep_free(c, a.first, a.second); }
+    } t{c, {}};
+    DEP_CHECK(dep_obj_alloc(&t, &dp, 3 * (size_t)m));
+    DEP_CHECK(dep_obj_alloc(&t, &dt, 31 * (size_t)m));
+    DEP_CHECK(upload(c, dp, pts, 3 * (size_t)m));
+    DEP_HIP(c, launch_multipole_terms(s, m, 1, dp, dt));
+    DEP_CHECK(download(c, terms.data(), dt, terms.size()));
+    const double B = 1.0 / (DEP_PI4 * DEP_EPS0);
+    const double w = s->freq * DEP_PI2 * DEP_EPS0;
+    const double* mt = s->mats.data();   // (sigma, eps_r) of material 1 (medium) and 2 (particle)
+    const double A = DEP_PI2 * (mt[1] * DEP_EPS0) * axis[0] * axis[1] * axis[2] / 3.0;
+    const double C = axis[0] * axis[2] / (2.0 * axis[2] + axis[0]);
+    double sig_p = mt[2], eps_p = mt[3];
+    for (int k = 0; k < m; k++) {
+        double ep[3], sg[3], K[3];
+        if (analysis == 11) {
+            eps_p *= C;
+            sig_p *= C;
+            for (int q = 0; q < 3; q++) { ep[q] = eps_p / Ld[q]; sg[q] = sig_p / Ld[q]; }
+        } else {
+            for (int q = 0; q < 3; q++) { ep[q] = eps_p; sg[q] = sig_p; }
+        }
+        for (int q = 0; q < 3; q++) {
+            const double EpsDif = w * (ep[q] - mt[1]);
+            const double SigDif = sg[q] - mt[0];
+            const double EpsPlus = w * (mt[1] + (ep[q] - mt[1]) * Ld[q]);
+            const double SigPlus = mt[0] + (sg[q] - mt[0]) * Ld[q];
+            K[q] = (EpsDif * EpsPlus + SigDif * SigPlus) / (EpsPlus * EpsPlus + SigPlus * SigPlus);
+        }
+        // E and its gradient: E_x, E_y, E_z; d[0..5] = xx, xy, xz, yy, yz, zz
+        const double Ex = ef[6 * k], Ey = ef[6 * k + 2], Ez = ef[6 * k + 4];
+        const double* d = &terms[31 * (size_t)k];
+        const double Exx = B * d[0], Exy = B * d[1], Exz = B * d[2], Eyy = B * d[3], Eyz = B * d[4], Ezz = B * d[5];
+        double* Fk = F + 3 * (size_t)k;
+        Fk[0] = A * (K[0] * Ex * Exx + K[1] * Ey * Exy + K[2] * Ez * Exz);
+        Fk[1] = A * (K[0] * Ex * Exy + K[1] * Ey * Eyy + K[2] * Ez * Eyz);
+        Fk[2] = A * (K[0] * Ex * Exz + K[1] * Ey * Eyz + K[2] * Ez * Ezz);
+    }
+    return DEP_OK;
+}
+
 // GMRES (src/gmres_el.c:66-354): unrestarted, Givens rotations on the host,
 // Krylov vectors and products on the device.
 // GMRES workspace, kept in the system between calls (a serving loop does not

@@ -318,7 +318,8 @@ namespace {
 // postProcess_tria{3,6} and vtkPostProcess_tria{3,6} (src/postProcess.c:256-738,
 // headers :58-116): potential and field at the internal points (STD files, or
 // one results.vtk structured-points file), the Maxwell-stress force (ANALYSIS
-// 3, 4) or the multipolar force at the force points (ANALYSIS 5-9), written in
+// 3, 4), the multipolar force (ANALYSIS 5-9) or the dipolar ellipsoid force
+// (ANALYSIS 10, 11) at the force points, written in
 // the reference's formats.  The header rules follow headerSetup: an output
 // type other than STD or VTK opens no potential/field file (the driver passes
 // such a type when the input has no internal points).  Runs on the system of
@@ -327,8 +328,6 @@ int post(int npe, bool vtk, unsigned ANALYSIS, const char* type, const double* a
          double** vMatParam, const unsigned* vBCType, unsigned** mElems, const double* vB)
 {
     if (!st.sys) fatal("postProcess: no assembled system on the device (electricForm* runs first)");
-    if (ANALYSIS == 10 || ANALYSIS == 11)
-        fatal("postProcess: the dipolar ellipsoid force (ANALYSIS 10/11) is not available in this build");
     dep_sys* sy = st.sys;
     const int E = st.geom->E;
     const bool is_std = type && strcmp(type, "STD") == 0, is_vtk = type && strcmp(type, "VTK") == 0;
@@ -428,7 +427,18 @@ int post(int npe, bool vtk, unsigned ANALYSIS, const char* type, const double* a
         std::vector<double> pts(3 * (size_t)nF), F(3 * (size_t)nF);
         for (int i = 0; i < nF; i++)
             for (int k = 0; k < 3; k++) pts[3 * i + k] = XF[i][k];
-        if (nF) check(dep_post_multipole(sy, order, axis[0], nF, pts.data(), F.data()), "multipolar force");
+        if (nF && ANALYSIS >= 10) {
+            check(dep_post_ellipsoid(sy, (int)ANALYSIS, axis, nF, pts.data(), F.data()), "ellipsoid force");
+            if (ANALYSIS == 11) {   // the table as forceEllipsoid_* leaves it (force.c:149-153)
+                const double C = axis[0] * axis[2] / (2.0 * axis[2] + axis[0]);
+                for (int i = 0; i < nF; i++) {
+                    vMatParam[1][1] = vMatParam[1][1] * C;
+                    vMatParam[1][0] = vMatParam[1][0] * C;
+                }
+            }
+        } else if (nF) {
+            check(dep_post_multipole(sy, order, axis[0], nF, pts.data(), F.data()), "multipolar force");
+        }
         for (int i = 0; i < nF; i++) {
             fprintf(fp[2], "%e\t%e\t%e\t", pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
             fprintf(fp[2], "%e\t%e\t%e\n", F[3 * i], F[3 * i + 1], F[3 * i + 2]);

@@ -179,8 +179,10 @@ int main(int argc, char **argv)
         nF = atoi(need(&t, "force points"));
         axis[0] = atof(need(&t, "radius"));
         snprintf(fpfile, sizeof fpfile, "%s", need(&t, "force point file"));
-    } else if (ANALYSIS == 10 || ANALYSIS == 11) {
-        die("ANALYSIS 10/11 (dipolar ellipsoid) is not available in this build", NULL);
+    } else if (ANALYSIS == 10 || ANALYSIS == 11) {   /* depSolver.c:147-154 */
+        nF = atoi(need(&t, "force points"));
+        for (int k = 0; k < 3; k++) axis[k] = atof(need(&t, "semi-axis"));
+        snprintf(fpfile, sizeof fpfile, "%s", need(&t, "force point file"));
     }
     need(&t, "INTERNALPOINTS");
     int nP = atoi(need(&t, "internal point count"));
@@ -411,7 +413,8 @@ int main(int argc, char **argv)
     } else if (ANALYSIS > 4) {
         const int order = ANALYSIS - 4;
         double *F = malloc(sizeof(double) * 3 * (size_t)nF);
-        check(ctx, dep_post_multipole(s, order, axis[0], nF, fpts, F), "multipolar force");
+        if (ANALYSIS >= 10) check(ctx, dep_post_ellipsoid(s, ANALYSIS, axis, nF, fpts, F), "ellipsoid force");
+        else check(ctx, dep_post_multipole(s, order, axis[0], nF, fpts, F), "multipolar force");
         for (int i = 0; i < nF; i++) {
             fprintf(fp2, "%e\t%e\t%e\t", fpts[3 * i], fpts[3 * i + 1], fpts[3 * i + 2]);
             fprintf(fp2, "%e\t%e\t%e\n", F[3 * i], F[3 * i + 1], F[3 * i + 2]);

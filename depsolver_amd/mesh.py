@@ -333,7 +333,7 @@ def shelled_trap(particles, freq: float, n_grid: int = 16, quadratic: bool = Tru
 def write_bem(dirpath: str, p: Problem, solver: str = "gaussBksb", analysis: int = 4,
               internal_points: np.ndarray | None = None, gmres_precond: int = 1,
               force_points: np.ndarray | None = None, radius: float = 1.0,
-              reposition=None) -> None:
+              reposition=None, axis=None) -> None:
     os.makedirs(dirpath, exist_ok=True)
     et = "tria6" if p.npe == 6 else "tria3"
     with open(os.path.join(dirpath, "nodes.bem"), "w") as f:
@@ -369,7 +369,11 @@ def write_bem(dirpath: str, p: Problem, solver: str = "gaussBksb", analysis: int
     else:
         lines.append(f"\nANALYSIS\ngaussBksb\n{analysis}\n")
     if analysis > 4 and force_points is not None:
-        lines.append(f"{len(force_points)} {radius:.17e} fpoints.bem\n")
+        if analysis in (10, 11):   # ellipsoid semi-axes (depSolver.c:147-154)
+            a = (radius, radius, radius) if axis is None else tuple(axis)
+            lines.append(f"{len(force_points)} {a[0]:.17e} {a[1]:.17e} {a[2]:.17e} fpoints.bem\n")
+        else:
+            lines.append(f"{len(force_points)} {radius:.17e} fpoints.bem\n")
         with open(os.path.join(dirpath, "fpoints.bem"), "w") as f:
             for i, (x, y, z) in enumerate(force_points, 1):
                 f.write(f"{i}\t{x:.17e}\t{y:.17e}\t{z:.17e}\n")

@@ -150,6 +150,16 @@ int dep_post_mst(dep_sys *s, double eps_medium, double *Fce, double *Xce, double
                  double *Xcm);
 int dep_post_multipole(dep_sys *s, int order, double radius, int m, const double *pts,
                        double *F /* m x 3 */);
+/* Dipolar DEP force on an ellipsoid of semi-axes axis[3] centred at each of
+ * m points: replaces forceEllipsoid_tria3/6 (src/force.c:65, :204) as
+ * postProcess_* calls it for ANALYSIS 10 (homogeneous) and 11 (single-shell
+ * oblate) (src/postProcess.c:332-341, :448-457), with depolarization()
+ * (src/depolarization.c:44).  For 11 the reference scales the particle's
+ * (sigma, eps_r) by C = a c / (2c + a) in its material table before every
+ * point, so point k sees C^(k+1); the same is done here, without changing
+ * the system's materials. */
+int dep_post_ellipsoid(dep_sys *s, int analysis, const double *axis, int m, const double *pts,
+                       double *F /* m x 3 */);
 
 /* ---- frequency sweeps on one geometry ----------------------------------
  * No single counterpart in the reference, which runs depSolver once per

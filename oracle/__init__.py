@@ -110,6 +110,10 @@ class Oracle(_Lib):
         L.orc_force_mst.restype = _c_int
         L.orc_force_multipole.argtypes = [_c_int, _c_int, _c_dbl, _f64p, _c_int, _f64p, _c_int,
                                           _i32p, _f64p, _f64p, _c_dbl, _f64p]
+        L.orc_depolarization.argtypes = [_f64p, _f64p]
+        L.orc_depolarization.restype = _c_int
+        L.orc_force_ellipsoid.argtypes = [_c_int, _c_int, _f64p, _c_int, _f64p, _c_int, _f64p, _c_int,
+                                          _i32p, _f64p, _f64p, _c_dbl, _f64p]
 
     def node_normal(self, p, node):
         n = np.zeros(3)
@@ -174,6 +178,21 @@ class Oracle(_Lib):
                                      _c32(p.conn), _c64(sigma), _c64(p.mats), p.freq, F)
         return F
 
+    def depolarization(self, axis):
+        Ld = np.zeros(3)
+        if self.lib.orc_depolarization(_c64(axis), Ld) != 0:
+            raise ValueError("depolarization: negative factor")
+        return Ld
+
+    def force_ellipsoid(self, p, sigma, pts, axis, analysis=10):
+        """Forces at pts (m x 3), and the material table as the calls left it."""
+        pts = _c64(np.atleast_2d(pts))
+        F = np.zeros((len(pts), 3))
+        mats = _c64(p.mats).copy()
+        self.lib.orc_force_ellipsoid(p.npe, analysis, _c64(axis), len(pts), pts, p.n_nodes, _c64(p.xyz),
+                                     p.n_elems, _c32(p.conn), _c64(sigma), mats, p.freq, F)
+        return F, mats
+
 
 class Reference(_Lib):
     """The reference's own functions (oracle/_ref/libdepref.so)."""
@@ -192,6 +211,9 @@ class Reference(_Lib):
         L.ref_force_mst.argtypes = [_c_int, _c_int, _f64p, _c_int, _i32p, _u32p, _f64p, _c_dbl,
                                     _f64p, _f64p, _f64p, _f64p]
         L.ref_force_multipole.argtypes = [_c_int, _c_int, _c_dbl, _f64p, _c_int, _f64p, _c_int,
+                                          _i32p, _f64p, _c_int, _f64p, _c_dbl, _f64p]
+        L.ref_depolarization.argtypes = [_f64p, _f64p]
+        L.ref_force_ellipsoid.argtypes = [_c_int, _c_int, _f64p, _c_int, _f64p, _c_int, _f64p, _c_int,
                                           _i32p, _f64p, _c_int, _f64p, _c_dbl, _f64p]
 
     def node_normal(self, p, node):
@@ -241,6 +263,19 @@ class Reference(_Lib):
         self.lib.ref_force_multipole(p.npe, order, R, _c64(x), p.n_nodes, _c64(p.xyz), p.n_elems,
                                      _c32(p.conn), _c64(sigma), len(p.mats), _c64(p.mats), p.freq, F)
         return F
+
+    def depolarization(self, axis):
+        Ld = np.zeros(3)
+        self.lib.ref_depolarization(_c64(axis), Ld)
+        return Ld
+
+    def force_ellipsoid(self, p, sigma, pts, axis, analysis=10):
+        pts = _c64(np.atleast_2d(pts))
+        F = np.zeros((len(pts), 3))
+        mats = _c64(p.mats).copy()
+        self.lib.ref_force_ellipsoid(p.npe, analysis, _c64(axis), len(pts), pts, p.n_nodes, _c64(p.xyz),
+                                     p.n_elems, _c32(p.conn), _c64(sigma), len(p.mats), mats, p.freq, F)
+        return F, mats
 
 
 def oracle() -> Oracle:

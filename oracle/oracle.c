@@ -1087,3 +1087,135 @@ void orc_force_multipole(int npe, int order, double R, const double *x, int N,
     free(Ecube);
     free(intE);
 }
+
+/* 64-point Gauss-Legendre rule on (-1, 1), ascending nodes: Newton's method on
+ * P_64 from the Chebyshev-like start cos(pi (i + 3/4) / (n + 1/2)), weights
+ * 2 / ((1 - x^2) P'_64(x)^2).  The reference tabulates the same rule to 15
+ * digits (src/gaussData.h:76, GaussH); computed here from its definition. */
+static void gauss_legendre64(double *x, double *w)
+{
+    const int n = 64;
+    for (int i = 0; i < n / 2; i++) {
+        double z = cos(3.14159265358979323846 * (i + 0.75) / (n + 0.5)), dp = 0.0;
+        for (int it = 0; it < 100; it++) {
+            double p0 = 1.0, p1 = z;
+            for (int k = 2; k <= n; k++) {
+                const double p2 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
+                p0 = p1;
+                p1 = p2;
+            }
+            dp = n * (z * p1 - p0) / (z * z - 1.0);
+            const double dz = p1 / dp;
+            z -= dz;
+            if (fabs(dz) < 1e-17) break;
+        }
+        {
+            double p0 = 1.0, p1 = z;
+            for (int k = 2; k <= n; k++) {
+                const double p2 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
+                p0 = p1;
+                p1 = p2;
+            }
+            dp = n * (z * p1 - p0) / (z * z - 1.0);
+        }
+        const double wt = 2.0 / ((1.0 - z * z) * dp * dp);
+        x[i] = -z;
+        w[i] = wt;
+        x[n - 1 - i] = z;
+        w[n - 1 - i] = wt;
+    }
+}
+
+/* Depolarisation factors of an ellipsoid (src/depolarization.c:44-115):
+ * L_k = (abc/2) int_0^{100 a_k^2} ds / ((s + a_k^2) sqrt((s+a^2)(s+b^2)(s+c^2)))
+ * by the 64-point rule on the segment [0, 100 a_k^2] (the reference truncates
+ * the infinite integral there).  Returns -1 if a factor is negative (the
+ * reference stops with an error, depolarization.c:106-108). */
+int orc_depolarization(const double *axis, double *Ld)
+{
+    double gx[64], gw[64];
+    gauss_legendre64(gx, gw);
+    const double A = 0.5 * axis[0] * axis[1] * axis[2];
+    const double a2[3] = {axis[0] * axis[0], axis[1] * axis[1], axis[2] * axis[2]};
+    for (int k = 0; k < 3; k++) {
+        const double top = 100.0 * a2[k];
+        const double J = 0.5 * top;   /* X2L_line2 on the segment (0,0)-(top,0) */
+        double acc = 0.0;
+        for (int i = 0; i < 64; i++) {
+            const double s = 0.5 * (1.0 + gx[i]) * top;
+            const double R = J * gw[i] / sqrt((s + a2[0]) * (s + a2[1]) * (s + a2[2]));
+            acc += R / (s + a2[k]);
+        }
+        Ld[k] = A * acc;
+    }
+    for (int k = 0; k < 3; k++)
+        if (Ld[k] < 0.0) return -1;
+    return 0;
+}
+
+/* Dipolar DEP force on an ellipsoid centred at each of m points
+ * (src/force.c:65-183 forceEllipsoid_tria3, :204-322 forceEllipsoid_tria6):
+ * Re E at the point and its first derivatives from intDE of order 1, the
+ * generalised Clausius-Mossotti factor per axis, then
+ * F = (2 pi eps_m abc / 3) sum_k K_k E_k dE_k/dx_j.
+ * analysis 10: homogeneous ellipsoid; 11: single-shell oblate ellipsoid, whose
+ * particle (sigma, eps_r) the reference scales by C = ac / (2c + a) in its
+ * material table on every call (force.c:148-155), so point k sees them scaled
+ * by C^(k+1); mats (nM x 2) is updated in place the same way. */
+void orc_force_ellipsoid(int npe, int analysis, const double *axis, int m, const double *pts,
+                         int N, const double *xyz, int E, const int *conn, const double *sigma,
+                         double *mats, double freq, double *F)
+{
+    const double w = freq * ORC_PI2 * ORC_EPS0;
+    const double B = 1.0 / (ORC_PI4 * ORC_EPS0);
+    const double C = axis[0] * axis[2] / (2.0 * axis[2] + axis[0]);
+    double *Ecube = (double *)malloc(sizeof(double) * 343);
+    double *intE = (double *)malloc(sizeof(double) * 6 * 343);
+    double ef[6], X[18], Ld[3];
+    if (!Ecube || !intE) { free(Ecube); free(intE); return; }
+    for (int p = 0; p < m; p++) {
+        const double *x = pts + 3 * p;
+        double *Fp = F + 3 * p;
+        const double Eps = mats[1] * ORC_EPS0;
+        const double A = ORC_PI2 * Eps * axis[0] * axis[1] * axis[2] / 3.0;
+        double K[3] = {0.0, 0.0, 0.0};
+        memset(Ecube, 0, sizeof(double) * 343);
+        orc_field(npe, N, xyz, E, conn, sigma, x, ef);
+        Ecube[D3(1, 0, 0)] = ef[0];
+        Ecube[D3(0, 1, 0)] = ef[2];
+        Ecube[D3(0, 0, 1)] = ef[4];
+        for (int e = 0; e < E; e++) {
+            gather_X(npe, xyz, conn, e, X);
+            intDE(npe, 1, X, x, intE);
+            for (int j = 0; j < npe; j++) {
+                const double s = sigma[conn[e * npe + j]];
+                for (int t = 0; t < 343; t++) Ecube[t] += B * intE[j * 343 + t] * s;
+            }
+        }
+        orc_depolarization(axis, Ld);
+        if (analysis == 10 || analysis == 11) {
+            double ep[3], sg[3];
+            if (analysis == 10) {
+                for (int k = 0; k < 3; k++) { ep[k] = mats[3]; sg[k] = mats[2]; }
+            } else {
+                mats[3] = mats[3] * C;
+                mats[2] = mats[2] * C;
+                for (int k = 0; k < 3; k++) { ep[k] = mats[3] / Ld[k]; sg[k] = mats[2] / Ld[k]; }
+            }
+            for (int k = 0; k < 3; k++) {
+                const double EpsDif = w * (ep[k] - mats[1]);
+                const double SigDif = sg[k] - mats[0];
+                const double EpsPlus = w * (mats[1] + (ep[k] - mats[1]) * Ld[k]);
+                const double SigPlus = mats[0] + (sg[k] - mats[0]) * Ld[k];
+                K[k] = (EpsDif * EpsPlus + SigDif * SigPlus) / (EpsPlus * EpsPlus + SigPlus * SigPlus);
+            }
+        }
+#define EC(p, q, r) Ecube[D3(p, q, r)]
+        Fp[0] = A * (K[0] * EC(1,0,0) * EC(2,0,0) + K[1] * EC(0,1,0) * EC(1,1,0) + K[2] * EC(0,0,1) * EC(1,0,1));
+        Fp[1] = A * (K[0] * EC(1,0,0) * EC(1,1,0) + K[1] * EC(0,1,0) * EC(0,2,0) + K[2] * EC(0,0,1) * EC(0,1,1));
+        Fp[2] = A * (K[0] * EC(1,0,0) * EC(1,0,1) + K[1] * EC(0,1,0) * EC(0,1,1) + K[2] * EC(0,0,1) * EC(0,0,2));
+#undef EC
+    }
+    free(Ecube);
+    free(intE);
+}

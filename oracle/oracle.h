@@ -90,6 +90,17 @@ void orc_force_multipole(int npe, int order, double R, const double *x, int N,
                          const double *xyz, int E, const int *conn, const double *sigma,
                          const double *mats, double freq, double *F);
 
+/* Depolarisation factors of an ellipsoid of semi-axes axis[3]
+ * (depolarization.c:44-115, the 64-point rule on the truncated integral). */
+int orc_depolarization(const double *axis, double *Ld);
+
+/* Dipolar DEP force on an ellipsoid at each of m points (force.c
+ * forceEllipsoid_*), analysis 10 (homogeneous) or 11 (single-shell oblate,
+ * which scales the particle row of mats by C per point, in place). */
+void orc_force_ellipsoid(int npe, int analysis, const double *axis, int m, const double *pts,
+                         int N, const double *xyz, int E, const int *conn, const double *sigma,
+                         double *mats, double freq, double *F);
+
 #ifdef __cplusplus
 }
 #endif

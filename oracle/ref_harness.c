@@ -17,6 +17,7 @@
 
 #include "gaussData.h"
 #include "assembly_el.h"
+#include "depolarization.h"
 #include "field.h"
 #include "force.h"
 #include "gaussBksb.h"
@@ -240,6 +241,41 @@ void ref_force_multipole(int npe, int order, double R, const double *x, int N, c
     memcpy(s, sigma, sizeof(double) * 2 * N);
     if (npe == 3) forceMultipole_tria3((unsigned)order, R, xi, nodes, el, s, mat, prob, F);
     else forceMultipole_tria6((unsigned)order, R, xi, nodes, el, s, mat, prob, F);
+    free(s);
+    free_rows((void **)nodes, N);
+    free_rows((void **)el, E);
+    free_rows((void **)mat, nM);
+}
+
+void ref_depolarization(const double *axis, double *Ld)
+{
+    double a[3] = {axis[0], axis[1], axis[2]};
+    depolarization(a, Ld);
+}
+
+/* forceEllipsoid_* at m points in turn with one material table, as
+ * postProcess_* calls it (postProcess.c:332-341, :448-457); the table comes
+ * back as the calls left it (analysis 11 scales the particle row). */
+void ref_force_ellipsoid(int npe, int analysis, const double *axis, int m, const double *pts, int N,
+                         const double *xyz, int E, const int *conn0, const double *sigma, int nM,
+                         double *mats, double freq, double *F)
+{
+    set_sizes(N, E, 0);
+    double **nodes = copy_rows(xyz, N, 3);
+    unsigned **el = conn1(conn0, E, npe);
+    double **mat = copy_rows(mats, nM, 2);
+    double ax[3] = {axis[0], axis[1], axis[2]}, prob[1] = {freq};
+    double *s = (double *)malloc(sizeof(double) * 2 * N);
+    memcpy(s, sigma, sizeof(double) * 2 * N);
+    for (int p = 0; p < m; p++) {
+        double xi[3] = {pts[3 * p], pts[3 * p + 1], pts[3 * p + 2]};
+        if (npe == 3) forceEllipsoid_tria3((unsigned)analysis, 1, ax, xi, nodes, el, s, mat, prob, F + 3 * p);
+        else forceEllipsoid_tria6((unsigned)analysis, 1, ax, xi, nodes, el, s, mat, prob, F + 3 * p);
+    }
+    for (int i = 0; i < nM; i++) {
+        mats[2 * i] = mat[i][0];
+        mats[2 * i + 1] = mat[i][1];
+    }
     free(s);
     free_rows((void **)nodes, N);
     free_rows((void **)el, E);

@@ -79,6 +79,27 @@ def kernel_case(r):
     np.savez_compressed(os.path.join(HERE, "kernels.npz"), **out)
 
 
+def ellipsoid_case(r):
+    """Dipolar ellipsoid force (ANALYSIS 10 and 11) on the solved fixtures, and
+    depolarisation factors for a few semi-axis sets."""
+    out = {"axes": np.array([[1.0, 1.0, 1.0], [2.0, 1.0, 0.5], [1.5, 1.5, 0.4], [3.0, 0.7, 0.9],
+                             [0.5, 0.5, 2.0]])}
+    out["Ld"] = np.array([r.depolarization(a) for a in out["axes"]])
+    ax = np.array([1.0, 0.8, 0.5])
+    out["axis"] = ax
+    for name in ("tiny_tria3", "tiny_tria6", "config1"):
+        z = np.load(os.path.join(HERE, f"{name}.npz"))
+        p = mesh.Problem(z["xyz"], z["conn"], z["bctype"], z["bc"], z["mats"], z["ifaces"],
+                         float(z["freq"]), None, name)
+        pts = np.concatenate([z["xf"], [[0.0, -0.2, 0.25]]])
+        out[f"{name}_pts"] = pts
+        for an in (10, 11):
+            F, mats = r.force_ellipsoid(p, z["sigma"], pts, ax, an)
+            out[f"{name}_F{an}"] = F
+            out[f"{name}_mats{an}"] = mats
+    np.savez_compressed(os.path.join(HERE, "ellipsoid.npz"), **out)
+
+
 def copy_examples():
     d = os.path.join(HERE, "dep-trap-2")
     os.makedirs(d, exist_ok=True)
@@ -95,6 +116,7 @@ def main():
     solve_case(r, mesh.sphere_between_plates(1, 4, name="tiny-tria6"), "tiny_tria6")
     solve_case(r, tria3_problem(), "tiny_tria3")
     solve_case(r, mesh.config1(), "config1", npts=12)
+    ellipsoid_case(r)
     copy_examples()
 
 

@@ -116,6 +116,47 @@ def test_multipole_force_matches_reference(ctx, name):
         g.close()
 
 
+@pytest.mark.parametrize("name", ["tiny_tria6", "tiny_tria3", "config1"])
+def test_ellipsoid_force_matches_reference(ctx, name):
+    """Dipolar ellipsoid force, ANALYSIS 10 and 11 (three points in one call, so
+    the per-point material scaling of 11 is exercised), against the reference's
+    vectors."""
+    import depsolver_amd as dep
+    print(f"[gpu] ellipsoid {name}", flush=True)
+    p, z = load_case(name)
+    e = np.load(os.path.join(GOLD, "ellipsoid.npz"))
+    g = dep.Geometry(ctx, p)
+    s = dep.System(g)
+    try:
+        s.set_solution(z["sigma"])
+        for an in (10, 11):
+            F = s.ellipsoid(e[f"{name}_pts"], e["axis"], an)
+            ref = e[f"{name}_F{an}"]
+            assert np.abs(F - ref).max() <= 1e-8 * np.abs(ref).max(), (an, F, ref)
+        with pytest.raises(RuntimeError):
+            s.ellipsoid(e[f"{name}_pts"], e["axis"], 9)
+    finally:
+        s.close()
+        g.close()
+
+
+def test_driver_ellipsoid_force(tmp_path, orc):
+    """The driver on an ANALYSIS 11 input: force-mp.dat rows against the
+    oracle's dipolar ellipsoid force on the oracle's LU solution."""
+    print("[gpu] driver ANALYSIS 11", flush=True)
+    p = mesh.sphere_between_plates(1, 4)
+    xf = np.array([[0.1, 0.0, 0.2], [-0.2, 0.1, 0.0]])
+    ax = np.array([1.0, 0.8, 0.5])
+    mesh.write_bem(str(tmp_path), p, solver="gaussBksb", analysis=11, force_points=xf, axis=ax)
+    run_driver(tmp_path)
+    A, b = orc.assemble(p, True)
+    x = orc.gauss_bksb(A, b)
+    want, _ = orc.force_ellipsoid(p, x, xf, ax, 11)
+    rows = np.loadtxt(tmp_path / "force-mp.dat")
+    assert np.abs(rows[:, :3] - xf).max() < 1e-6
+    assert np.abs(rows[:, 3:] - want).max() <= 1e-6 * np.abs(want).max()
+
+
 def run_driver(workdir, env=None):
     exe = os.path.join(ROOT, "depsolver_amd", "bin", "depSolver-amd")
     e = dict(os.environ)

@@ -87,3 +87,56 @@ def test_dep_trap_2_shipped_outputs(orc, tmp_path):
     sel = [e for e in range(p.n_elems) if p.bctype[p.conn[e, 0]] == 6]
     assert np.allclose(rows[:, :3], Xce[sel], rtol=5e-7, atol=1e-20)
     assert np.allclose(rows[:, 3:], Fce[sel], rtol=5e-7, atol=1e-6 * np.abs(Fce).max())
+
+
+def test_depolarization_factors(orc):
+    """Depolarisation factors (depolarization.c:44-115) against the reference's
+    vectors, the closed form of the truncated integral for a sphere
+    ((1 - 101^-1.5) / 3, which fixes the truncation point), the same 64-point
+    rule with numpy's nodes (weights good to ~1e-13), and the integral itself by
+    adaptive quadrature (a coarse check: the 64-point rule is off by up to ~4e-3
+    along the longest of elongated axes, as the reference's is)."""
+    from scipy.integrate import quad
+    z = np.load(os.path.join(GOLD, "ellipsoid.npz"))
+    gx, gw = np.polynomial.legendre.leggauss(64)
+    for a, Lref in zip(z["axes"], z["Ld"]):
+        L = orc.depolarization(a)
+        assert np.abs(L - Lref).max() <= 1e-14, (a, L, Lref)
+        a2 = a * a
+        for k in range(3):
+            s = 0.5 * (1 + gx) * 100 * a2[k]
+            f = 1.0 / ((s + a2[k]) * np.sqrt((s + a2[0]) * (s + a2[1]) * (s + a2[2])))
+            assert abs(0.5 * a.prod() * 50 * a2[k] * np.dot(gw, f) - L[k]) <= 1e-12 * L[k]
+            exact = 0.5 * a.prod() * quad(lambda t: 1.0 / ((t + a2[k]) * np.sqrt((t + a2[0]) * (t + a2[1]) * (t + a2[2]))),
+                                          0, 100 * a2[k], epsabs=0, epsrel=1e-12, limit=400)[0]
+            assert abs(L[k] - exact) <= 1e-2 * exact
+    L = orc.depolarization(np.array([0.7, 0.7, 0.7]))
+    assert np.abs(L - (1 - 101 ** -1.5) / 3).max() <= 1e-9
+
+
+@pytest.mark.parametrize("name", ["tiny_tria6", "tiny_tria3", "config1"])
+def test_ellipsoid_force(orc, name):
+    """Dipolar ellipsoid force (force.c forceEllipsoid_*, ANALYSIS 10 and 11)
+    against the reference's vectors, including the material table ANALYSIS 11
+    leaves scaled by C per point."""
+    p, zs = load(name)
+    z = np.load(os.path.join(GOLD, "ellipsoid.npz"))
+    for an in (10, 11):
+        F, mats = orc.force_ellipsoid(p, zs["sigma"], z[f"{name}_pts"], z["axis"], an)
+        ref = z[f"{name}_F{an}"]
+        assert np.abs(F - ref).max() <= 1e-13 * np.abs(ref).max(), (an, F, ref)
+        assert np.array_equal(mats, z[f"{name}_mats{an}"])
+    # a sphere with the depolarisation factor 1/3 reduces the homogeneous case to the
+    # dipole term of the multipolar force; with the truncated factor L the ratio
+    # of the two is K(L) / (3 CM1) on every component
+    R = 0.6
+    F, _ = orc.force_ellipsoid(p, zs["sigma"], z[f"{name}_pts"][:1], np.array([R, R, R]), 10)
+    Fm = orc.force_multipole(p, zs["sigma"], z[f"{name}_pts"][0], R, 1)
+    L = orc.depolarization(np.array([R, R, R]))[0]
+    m = p.mats
+    w = p.freq * 2 * np.pi * 8.854187817e-12
+    cm = lambda ed, ep, sd, sp: (ed * ep + sd * sp) / (ep * ep + sp * sp)
+    K = cm(w * (m[1, 1] - m[0, 1]), w * (m[0, 1] + (m[1, 1] - m[0, 1]) * L), m[1, 0] - m[0, 0],
+           m[0, 0] + (m[1, 0] - m[0, 0]) * L)
+    CM1 = cm(w * (m[1, 1] - m[0, 1]), w * (m[1, 1] + 2 * m[0, 1]), m[1, 0] - m[0, 0], m[1, 0] + 2 * m[0, 0])
+    assert np.abs(F[0] - Fm * K / (3 * CM1)).max() <= 1e-12 * np.abs(Fm).max()

@@ -1,6 +1,8 @@
 """The oracle is bitwise equal to the reference build (oracle/_ref) on the
 same inputs, function by function (skipped where the reference sources are
 absent)."""
+import os
+
 import numpy as np
 import pytest
 
@@ -44,3 +46,20 @@ def test_bitwise(orc, ref, case, tmp_path, monkeypatch):
         assert np.array_equal(a_, b_)
     for node in range(0, p.n_nodes, 23):
         assert np.array_equal(orc.node_normal(p, node), ref.node_normal(p, node))
+
+
+def test_ellipsoid_force_vs_reference(orc, ref):
+    """Depolarisation factors and the dipolar ellipsoid force (both analyses,
+    several points in one call) against the reference build."""
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "tiny_tria6.npz"))
+    p = mesh.Problem(z["xyz"], z["conn"].astype(np.int64), z["bctype"], z["bc"], z["mats"], z["ifaces"],
+                     float(z["freq"]), None, "tiny-tria6")
+    for a in ([1.2, 0.9, 0.4], [0.3, 0.3, 0.3], [0.5, 2.0, 1.0]):
+        a = np.array(a)
+        assert np.abs(orc.depolarization(a) - ref.depolarization(a)).max() <= 1e-14
+        pts = np.array([[0.1, 0.0, 0.2], [-0.3, 0.2, 0.0], [0.0, 0.0, -0.1], [0.2, 0.2, 0.2]])
+        for an in (10, 11):
+            F, m1 = orc.force_ellipsoid(p, z["sigma"], pts, a, an)
+            G, m2 = ref.force_ellipsoid(p, z["sigma"], pts, a, an)
+            assert np.abs(F - G).max() <= 1e-13 * np.abs(G).max()
+            assert np.array_equal(m1, m2)
