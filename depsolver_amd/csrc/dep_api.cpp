@@ -1356,7 +1356,7 @@ static int ensure_gmres_storage(dep_sys* s, int max_iters)
         sys_release(s, s->gm_dev);
         s->gm_V = s->gm_red = s->gm_dev = nullptr;
     }
-    if (!s->gm_flags) DEP_CHECK(dep_obj_alloc(s, &s->gm_flags, 4));
+    if (!s->gm_flags) DEP_CHECK(dep_obj_alloc(s, &s->gm_flags, 8));
     DEP_CHECK(dep_obj_alloc(s, &s->gm_dev, (size_t)(max_iters + 1) * (max_iters + 2) + 4 * (size_t)(max_iters + 2) + 4));
     if (!s->gm_vec) DEP_CHECK(dep_obj_alloc(s, &s->gm_vec, 3 * N + 3 * n2));
     DEP_CHECK(dep_obj_alloc(s, &s->gm_V, (size_t)(max_iters + 1) * n2));
@@ -1500,9 +1500,12 @@ static int gmres_run(dep_sys* s, MV matvec, const double* b, bool x_zero, double
 // after M steps.  The host enqueues the steps in batches and reads the stop
 // flag once per batch; steps after the stop are skipped by the kernels.  The
 // triangular solve for the coefficients stays on the host (one transfer).
-// matvec(v, w, stop, scale) enqueues w = Op v; with mv_scales it first scales
-// v in place by *scale (the normalisation of the Krylov vector, 1 / |v| as the
-// previous step left it), otherwise a separate kernel normalises each vector.
+// matvec(v, w, stop, scale, dg) enqueues w = Op v; with mv_scales it first
+// scales v in place by *scale (the normalisation of the Krylov vector, 1 / |v|
+// as the previous step left it), otherwise a separate kernel normalises each
+// vector.  With dg (mv_scales and the fused CGS step) the previous step's
+// Givens step is pending: the product's first launch runs it and takes 1 / |v|
+// from the norm partials itself.
 template <class MV>
 static int gmres_run_dev(dep_sys* s, MV matvec, bool mv_scales, const double* b, double* d, double tol_rel, int M,
                          GmresRun* out)
@@ -1531,6 +1534,11 @@ static int gmres_run_dev(dep_sys* s, MV matvec, bool mv_scales, const double* b,
     DEP_HIP(c, launch_gmres_init(c, h1, tol_rel, g, sc, flags));
     if (!mv_scales) DEP_HIP(c, launch_scale_dev(c, n2, V, sc, flags));
     const bool fused = cgs_fits(n2);
+    // with the block-Jacobi apply in the product, each step's Givens step runs in
+    // the next step's apply (or at the end of a batch) instead of in a workgroup
+    // that waits on the others
+    const bool defer = fused && mv_scales;
+    const dep_defer dg{cap, h1, h2, np, H, cs, sn, g, err, sc, flags};
     constexpr int BATCH = 8;
     int hf[3] = {0, 0, 1};
     size_t mark[BATCH];   // the products' timing records (steps past the stop are dropped)
@@ -1541,10 +1549,11 @@ static int gmres_run_dev(dep_sys* s, MV matvec, bool mv_scales, const double* b,
             mark[i - i0] = prof_mark(c);
             // the first step of a batch runs for certain (the flags read before it said so),
             // so its product goes without the stop flag (and may time the product's shapes)
-            DEP_HIP(c, matvec(vi, wv, i == i0 ? nullptr : flags, mv_scales ? sc : nullptr));
+            DEP_HIP(c, matvec(vi, wv, i == i0 ? nullptr : flags, mv_scales ? sc : nullptr, defer ? &dg : nullptr));
             DEP_HIP(c, launch_dot_partials(c, i + 2, n2, V, wv, part, flags));
             if (fused) {
-                DEP_HIP(c, launch_cgs_step(c, i, cap, n2, V, part, part2, h1, np, H, cs, sn, g, err, sc, flags));
+                DEP_HIP(c, launch_cgs_step(c, i, cap, n2, V, part, part2, h1, h2, defer, np, H, cs, sn, g, err, sc,
+                                           flags));
             } else {
                 DEP_HIP(c, launch_update_fused(c, i + 1, n2, V, part, wv, h1, nullptr, flags));
                 DEP_HIP(c, launch_dot_partials(c, i + 2, n2, V, wv, part, flags));
@@ -1553,6 +1562,7 @@ static int gmres_run_dev(dep_sys* s, MV matvec, bool mv_scales, const double* b,
             }
             if (!mv_scales) DEP_HIP(c, launch_scale_dev(c, n2, wv, sc, flags));
         }
+        if (defer) DEP_HIP(c, launch_gmres_finish(c, dg));
         DEP_CHECK(download(c, hf, flags, 3));
         if (hf[0]) {
             if (c->prof)
@@ -1710,9 +1720,11 @@ extern "C" int dep_gmres_ir(dep_sys* s, double tol_rel, double inner_tol, int ma
     }
     dep_blist one;
     one.n = 1;
-    auto matvec32 = [&](double* in, double* out, const int* stop, const double* scale) -> hipError_t {
+    auto matvec32 = [&](double* in, double* out, const int* stop, const double* scale,
+                        const dep_defer* dg) -> hipError_t {
         if (!block) return launch_zmatvec32_w(s, gw, gwi, gsh, in, nullptr, out, stop);
-        hipError_t e = launch_bj_apply(c, N, s->bjB, one, in, 0, w, 0, 0, stop, 0, scale);
+        hipError_t e = dg ? launch_bj_apply_dg(c, N, s->bjB, in, w, stop, *dg)
+                          : launch_bj_apply(c, N, s->bjB, one, in, 0, w, 0, 0, stop, 0, scale);
         if (e != hipSuccess) return e;
         return launch_zmatvec32_w(s, s->wr, s->wi, s->shift, w, nullptr, out, stop);
     };

@@ -152,6 +152,7 @@ __global__ void k_gmres_init(const double* __restrict__ h, double tol_rel, doubl
     flags[1] = 0;
     flags[2] = flags[0] ? 0 : 1;
     flags[3] = 0;   // workgroups done in k_cgs_last
+    flags[4] = 0;   // 1 + the column whose Givens step is deferred (0: none)
 }
 
 // Step i: the Hessenberg column from the two CGS passes (h1 + h2, |w| from the
@@ -234,6 +235,107 @@ __global__ void k_gmres_givens(int i, int cap, const double* __restrict__ h1, co
     if (threadIdx.x == 0)
         for (int k = 0; k < DEP_UPD_BLOCKS; k++) hn2 += np[k];
     givens_step(i, cap, h1, h2, hn2, H, cs, sn, g, err, sc, flags, lds);
+}
+
+// |w|^2 from the DEP_UPD_BLOCKS norm partials, by one wave: lane l adds
+// partials l and l + 64, then a shuffle tree.  Every lane, and every
+// workgroup that calls it, gets the same bits.
+static_assert(DEP_UPD_BLOCKS == 128, "np_sum adds two partials per lane");
+__device__ inline double np_sum(const double* np)
+{
+    const int l = threadIdx.x & 63;
+    double v = np[l] + np[l + 64];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// The deferred Givens step of column flags[4] - 1, if it has not run (flags[1]
+// counts the columns done); one wave.  Ends a batch of deferred steps so that
+// the flags the host reads are complete.
+__global__ void k_gmres_finish(int cap, const double* __restrict__ h1, const double* __restrict__ h2,
+                               const double* __restrict__ np, double* __restrict__ H, double* __restrict__ cs,
+                               double* __restrict__ sn, double* __restrict__ g, double* __restrict__ err,
+                               double* __restrict__ sc, int* __restrict__ flags)
+{
+    const int p = flags[4];
+    if (flags[0] || p <= flags[1]) return;
+    __shared__ double lds[3 * (UPD_MAXV + 1)];
+    const double hn2 = np_sum(np);
+    givens_step(p - 1, cap, h1, h2, hn2, H, cs, sn, g, err, sc, flags, lds);
+}
+
+// The block-Jacobi apply of the refined GMRES with the previous step's Givens
+// step folded in (float 64 x 64 blocks, one system): workgroups 0 .. nblk-1
+// scale x = V_i by 1/|V_i| (from the norm partials np when a Givens step is
+// deferred, flags[4] > 0, else sc[0]), write it back and form y = B x per
+// block, as k_bj_apply; workgroup nblk runs the deferred Givens step, which
+// stores the same 1/|V_i| in sc[0] from the same sum.  That step may set the
+// stop flag while the other workgroups start, so each of them reads the flag
+// once (thread 0) and returns as a whole; a y formed after a stop is not used.
+__global__ __launch_bounds__(256) void k_bj_apply_dg(int N, int nblk, const float* __restrict__ B,
+                                                     double* __restrict__ x, double* __restrict__ y,
+                                                     const int* __restrict__ stop, int cap,
+                                                     const double* __restrict__ h1, const double* __restrict__ h2,
+                                                     const double* __restrict__ np, double* __restrict__ H,
+                                                     double* __restrict__ cs, double* __restrict__ sn,
+                                                     double* __restrict__ g, double* __restrict__ err,
+                                                     double* __restrict__ sc, int* __restrict__ flags)
+{
+    constexpr int BJ = 64;
+    __shared__ double xr[BJ], xi[BJ];
+    __shared__ double lds[3 * (UPD_MAXV + 1)];
+    const int b = blockIdx.x;
+    const int p = flags[4];
+    if (b == nblk) {
+        if (flags[0] || p <= flags[1]) return;
+        const double hn2 = threadIdx.x < 64 ? np_sum(np) : 0.0;
+        givens_step(p - 1, cap, h1, h2, hn2, H, cs, sn, g, err, sc, flags, lds);
+        return;
+    }
+    __shared__ int quit;
+    if (threadIdx.x == 0) quit = stop ? stop[0] : 0;
+    __syncthreads();
+    if (quit) return;
+    const int t = threadIdx.x, r = t >> 2, q = t & 3;
+    if (t < BJ) {   // wave 0
+        double a;
+        if (p > 0) {
+            const double hn = sqrt(np_sum(np));
+            a = hn != 0.0 ? 1.0 / hn : 0.0;
+        } else {
+            a = sc[0];
+        }
+        const int j = b * BJ + t;
+        double vr = j < N ? x[j] : 0.0, vi = j < N ? x[N + j] : 0.0;
+        if (a != 0.0 && j < N) {
+            vr *= a;
+            vi *= a;
+            x[j] = vr;
+            x[N + j] = vi;
+        }
+        xr[t] = vr;
+        xi[t] = vi;
+    }
+    __syncthreads();
+    const float* br = B + (long)b * 2 * BJ * BJ + r * BJ + 16 * q;
+    const float* bi = br + BJ * BJ;
+    double sr = 0.0, si = 0.0;
+#pragma unroll
+    for (int c = 0; c < 16; c++) {
+        const double ar = br[c], ai = bi[c], vr = xr[16 * q + c], vi = xi[16 * q + c];
+        sr += ar * vr - ai * vi;
+        si += ar * vi + ai * vr;
+    }
+    sr += __shfl_xor(sr, 1);
+    si += __shfl_xor(si, 1);
+    sr += __shfl_xor(sr, 2);
+    si += __shfl_xor(si, 2);
+    const int i = b * BJ + r;
+    if (q == 0 && i < N) {
+        y[i] = sr;
+        y[N + i] = si;
+    }
 }
 
 // ---- one Krylov step's orthogonalisation in three launches -----------------
@@ -347,7 +449,8 @@ __global__ __launch_bounds__(256) void k_cgs_mid(int nv, long n2, const double* 
 
 __global__ __launch_bounds__(256) void k_cgs_last(int i, int cap, long n2, const double* __restrict__ V,
                                                   const double* __restrict__ part2, double* __restrict__ w,
-                                                  const double* __restrict__ h1, double* __restrict__ np,
+                                                  const double* __restrict__ h1, double* __restrict__ h2,
+                                                  int defer, double* __restrict__ np,
                                                   double* __restrict__ H, double* __restrict__ cs,
                                                   double* __restrict__ sn, double* __restrict__ g,
                                                   double* __restrict__ err, double* __restrict__ sc,
@@ -375,6 +478,14 @@ __global__ __launch_bounds__(256) void k_cgs_last(int i, int cap, long n2, const
     for (int o = 32; o > 0; o >>= 1) sq += __shfl_xor(sq, o);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = sq;
     __syncthreads();
+    if (defer) {   // the next launch reduces np and runs the Givens step
+        if (threadIdx.x == 0) np[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+        if (blockIdx.x == 0) {
+            for (int j = threadIdx.x; j < nv; j += 256) h2[j] = h[j];
+            if (threadIdx.x == 0) flags[4] = i + 1;
+        }
+        return;
+    }
     if (threadIdx.x == 0) {
         __hip_atomic_store(np + blockIdx.x, (red[0] + red[1]) + (red[2] + red[3]), __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
@@ -525,15 +636,31 @@ hipError_t launch_gmres_givens(dep_ctx* c, int i, int cap, const double* h1, con
 bool cgs_fits(long n2) { return (n2 + DEP_UPD_BLOCKS - 1) / DEP_UPD_BLOCKS <= CGS_CHUNK; }
 
 hipError_t launch_cgs_step(dep_ctx* c, int i, int cap, long n2, double* V, const double* part, double* part2,
-                           double* h1, double* np, double* H, double* cs, double* sn, double* g, double* err,
-                           double* sc, int* flags)
+                           double* h1, double* h2, int defer, double* np, double* H, double* cs, double* sn,
+                           double* g, double* err, double* sc, int* flags)
 {
     if (i + 1 > UPD_MAXV || !cgs_fits(n2)) return hipErrorInvalidValue;
     double* w = V + (long)(i + 1) * n2;
     hipLaunchKernelGGL(k_cgs_mid, dim3(DEP_UPD_BLOCKS), dim3(256), 0, c->stream, i + 1, n2, V, part, w, h1, part2,
                        flags);
-    hipLaunchKernelGGL(k_cgs_last, dim3(DEP_UPD_BLOCKS), dim3(256), 0, c->stream, i, cap, n2, V, part2, w, h1, np, H,
-                       cs, sn, g, err, sc, flags);
+    hipLaunchKernelGGL(k_cgs_last, dim3(DEP_UPD_BLOCKS), dim3(256), 0, c->stream, i, cap, n2, V, part2, w, h1, h2,
+                       defer, np, H, cs, sn, g, err, sc, flags);
+    return hipGetLastError();
+}
+
+hipError_t launch_gmres_finish(dep_ctx* c, const dep_defer& d)
+{
+    hipLaunchKernelGGL(k_gmres_finish, dim3(1), dim3(64), 0, c->stream, d.cap, d.h1, d.h2, d.np, d.H, d.cs, d.sn,
+                       d.g, d.err, d.sc, d.flags);
+    return hipGetLastError();
+}
+
+hipError_t launch_bj_apply_dg(dep_ctx* c, int N, const float* B, double* x, double* y, const int* stop,
+                              const dep_defer& d)
+{
+    const int nblk = (int)bj_blocks(N);
+    hipLaunchKernelGGL(k_bj_apply_dg, dim3(nblk + 1), dim3(256), 0, c->stream, N, nblk, B, x, y, stop, d.cap, d.h1,
+                       d.h2, d.np, d.H, d.cs, d.sn, d.g, d.err, d.sc, d.flags);
     return hipGetLastError();
 }
 

@@ -267,11 +267,26 @@ hipError_t launch_scale_dev(dep_ctx* c, long n2, double* w, const double* sc, co
 // DEP_DOT_CHUNKS per vector, of V_0 .. V_{i+1} against w = V_{i+1}): the two
 // CGS updates of w, the pass-2 partials in part2 ((i + 2) * DEP_UPD_BLOCKS),
 // h1 (i + 2), the norm partials np and the Givens step (flags[3]: workgroup
-// count, zeroed by launch_gmres_init).  cgs_fits: whether n2 suits it.
+// count, zeroed by launch_gmres_init).  With defer, the Givens step is left to
+// the next launch_bj_apply_dg or launch_gmres_finish: h2 gets the pass-2
+// coefficients and flags[4] = i + 1, and no workgroup waits on another.
+// cgs_fits: whether n2 suits it.
 bool cgs_fits(long n2);
 hipError_t launch_cgs_step(dep_ctx* c, int i, int cap, long n2, double* V, const double* part, double* part2,
-                           double* h1, double* np, double* H, double* cs, double* sn, double* g, double* err,
-                           double* sc, int* flags);
+                           double* h1, double* h2, int defer, double* np, double* H, double* cs, double* sn,
+                           double* g, double* err, double* sc, int* flags);
+// The device GMRES state a deferred Givens step reads and writes.
+struct dep_defer {
+    int cap;
+    double *h1, *h2, *np, *H, *cs, *sn, *g, *err, *sc;
+    int* flags;
+};
+// The deferred Givens step, if one is pending (ends a batch of steps).
+hipError_t launch_gmres_finish(dep_ctx* c, const dep_defer& d);
+// Block-Jacobi apply y = B (x / |x|) of one system (float blocks) that also
+// runs the pending Givens step; x is scaled in place.
+hipError_t launch_bj_apply_dg(dep_ctx* c, int N, const float* B, double* x, double* y, const int* stop,
+                              const dep_defer& d);
 hipError_t launch_scale(dep_ctx* c, long n2, const double* w, double alpha, double* out);
 hipError_t launch_combine(dep_ctx* c, int nvec, long n2, const double* V, const double* y, double* x);
 size_t mixed_bytes(int N);
