@@ -170,10 +170,12 @@ static hipEvent_t pool_get(dep_ctx* c)
     return e;
 }
 
-void prof_begin(dep_ctx* c, hipEvent_t* a, hipStream_t st)
+void prof_begin(dep_ctx* c, hipEvent_t* a, const char* name, hipStream_t st)
 {
     *a = nullptr;
     if (!c->prof) return;
+    c->prof_seq++;
+    if (c->prof_count[name]++ % (unsigned)c->prof_every != 0) return;
     *a = pool_get(c);
     if (*a) (void)hipEventRecord(*a, st ? st : c->stream);
 }
@@ -184,14 +186,22 @@ void prof_end(dep_ctx* c, hipEvent_t a, const char* name, double flops, double b
     hipEvent_t b = pool_get(c);
     if (!b) return;
     (void)hipEventRecord(b, st ? st : c->stream);
-    c->pending.push_back({name, a, b, flops, bytes});
+    dep_ctx::Pending p{name, a, b, flops, bytes};
+    p.seq = c->prof_seq;
+    c->pending.push_back(p);
 }
 
-size_t prof_mark(const dep_ctx* c) { return c->pending.size(); }
+size_t prof_mark(const dep_ctx* c) { return c->prof_seq + 1; }
 
-void prof_scale(dep_ctx* c, size_t idx, double f)
+void prof_scale(dep_ctx* c, size_t seq, double f)
 {
-    if (idx < c->pending.size()) c->pending[idx].scale *= f;
+    for (size_t k = c->pending.size(); k-- > 0;) {
+        if (c->pending[k].seq == seq) {
+            c->pending[k].scale *= f;
+            return;
+        }
+        if (c->pending[k].seq < seq) return;
+    }
 }
 
 void prof_collect(dep_ctx* c)
@@ -219,6 +229,8 @@ extern "C" int dep_ctx_set_profiling(dep_ctx* c, int enable)
     if (!c) return DEP_ERR_ARG;
     prof_collect(c);
     c->prof = enable != 0;
+    c->prof_every = enable > 1 ? enable : 1;
+    c->prof_count.clear();
     return DEP_OK;
 }
 
@@ -227,6 +239,7 @@ extern "C" int dep_ctx_reset_stats(dep_ctx* c)
     if (!c) return DEP_ERR_ARG;
     prof_collect(c);
     c->stats.clear();
+    c->prof_count.clear();
     return DEP_OK;
 }
 
@@ -1195,8 +1208,8 @@ struct GaussLegendre64 {
     double x[64], w[64];
     GaussLegendre64()
     {
-        const int n = 64;
-        auto legendre = [n](double z, double* dp) {
+        constexpr int n = 64;
+        auto legendre = [](double z, double* dp) {
             double p0 = 1.0, p1 = z;
             for (int k = 2; k <= n; k++) {
                 const double p2 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;

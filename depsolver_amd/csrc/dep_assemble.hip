@@ -254,7 +254,7 @@ hipError_t assemble_impl(dep_geom* g, int shape)
     const dep_geom::Patches& ps = g->patches[shape];
     dim3 grid(ps.n, dep_cdiv(g->N, TR));
     hipEvent_t ev;
-    prof_begin(g->ctx, &ev);
+    prof_begin(g->ctx, &ev, "assemble_regular");
     if (ps.tc == 8)
         hipLaunchKernelGGL((k_assemble_regular<NPE, TR, 8>), grid, dim3(TR), 0, st, g->N, g->xyz, g->conn, g->kind,
                            g->normal, (const double4*)g->q, ps.ptr, ps.elem, ps.slot, g->P, g->ldP, g->P32);

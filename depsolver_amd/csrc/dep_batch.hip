@@ -527,7 +527,7 @@ static hipError_t bgemm(dep_ctx* c, int N, const TP* P, long ldP, const dep_blis
     int S, kc;
     bgemm_split(c, N, &S, &kc);
     hipEvent_t ev;
-    prof_begin(c, &ev);
+    prof_begin(c, &ev, name);
     hipLaunchKernelGGL((k_bgemm<TP, TA>), dim3(dep_cdiv(N, BG_BM), S), dim3(256), 0, c->stream, N, P, ldP, kc, L, X,
                        xstride, part, gfl);
     const double n = N, nc = 2.0 * L.n;

@@ -43,12 +43,18 @@ struct dep_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // per-kernel timing with events on `stream` (off unless enabled)
     bool prof = false;
+    // events bracket every prof_every-th launch of each kernel name (an event
+    // record costs the stream a few microseconds on each side)
+    int prof_every = 1;
+    size_t prof_seq = 0;   // prof_begin calls so far, timed or not
+    std::map<std::string, unsigned> prof_count;
     std::map<std::string, dep_kstat> stats;
     struct Pending {
         std::string name;
         hipEvent_t a, b;
         double flops, bytes;
         double scale = 1.0;   // share of the flops that were useful; 0: not counted
+        size_t seq = 0;       // the prof_begin call that made it
     };
     std::vector<Pending> pending;
     std::vector<hipEvent_t> pool;
@@ -59,15 +65,17 @@ struct dep_ctx {
     size_t arena_size = 0, arena_used = 0, arena_peak = 0;
 };
 
-// time one launch: prof_begin before, prof_end after (no-ops unless profiling)
-void prof_begin(dep_ctx* c, hipEvent_t* a, hipStream_t st = nullptr);
+// time one launch: prof_begin before, prof_end after (no-ops unless profiling,
+// and for launches of `name` that the sampling period skips)
+void prof_begin(dep_ctx* c, hipEvent_t* a, const char* name, hipStream_t st = nullptr);
 void prof_end(dep_ctx* c, hipEvent_t a, const char* name, double flops, double bytes, hipStream_t st = nullptr);
 void prof_collect(dep_ctx* c);   // after a synchronize: fold finished events into stats
-// The record the next prof_end makes (profiling on) is prof_mark's index; a
-// caller that learns later how much of that launch was useful scales its
-// algorithmic flops by f (a batched product some of whose systems had already
-// stopped), or drops the launch from the statistics with f = 0 (a step
-// enqueued past the end of a device-driven iteration returns at once).
+// prof_mark names the next prof_begin call; a caller that learns later how
+// much of that launch was useful scales its algorithmic flops by f (a batched
+// product some of whose systems had already stopped), or drops the launch
+// from the statistics with f = 0 (a step enqueued past the end of a
+// device-driven iteration returns at once).  Nothing happens if that launch
+// was not timed.
 size_t prof_mark(const dep_ctx* c);
 void prof_scale(dep_ctx* c, size_t idx, double f);
 

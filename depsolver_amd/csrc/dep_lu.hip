@@ -1190,7 +1190,7 @@ static hipError_t lu_blocked(dep_sys* s, T* Wr, T* Wi, long ld, T* inv, T* piv, 
         const int nbo = std::min(NBO, N - k0), pe = k0 + nbo, M = N - pe, NC = c1 - c0;
         hipEvent_t ev;
         if (mode == 0) {
-            prof_begin(c, &ev, q);
+            prof_begin(c, &ev, "lu_gemm_fp64", q);
             dim3 grid(dep_cdiv(NC, 64), dep_cdiv(M, 64));
             if (f64_valu)
                 hipLaunchKernelGGL(k_gemmT<T>, grid, dim3(256), 0, q, pe, N, c0, c1, k0, nbo, Wr, Wi, ld);
@@ -1213,7 +1213,7 @@ static hipError_t lu_blocked(dep_sys* s, T* Wr, T* Wi, long ld, T* inv, T* piv, 
         else
             hipLaunchKernelGGL(k_pack_b<_Float16>, dim3(dep_cdiv(NCp, 64)), dim3(256), 0, q, NC, NCp, nbo, K2p,
                                k0, c0, Fr, Fi, ld, B);
-        prof_begin(c, &ev, q);
+        prof_begin(c, &ev, "lu_gemm_mixed", q);
         const dim3 grid = v1 ? dim3(2 * NCp / 128, dep_cdiv(M, 128)) : tiles((long)(2 * NCp / 128) * dep_cdiv(M, 128));
         float* Cr = (float*)Wr;
         float* Ci = (float*)Wi;

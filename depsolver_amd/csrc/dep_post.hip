@@ -523,7 +523,7 @@ hipError_t launch_points(dep_sys* s, int mode, const dep_points* P, double* out)
     const int nchunk = P->nchunk, chunk = dep_cdiv(g->E, nchunk);
     dim3 grid(dep_cdiv(M, 256), nchunk);
     hipEvent_t ev;
-    prof_begin(g->ctx, &ev);
+    prof_begin(g->ctx, &ev, mode ? "points_field" : "points_potential");
     if (mode == 0)
         hipLaunchKernelGGL(k_points_regular<0>, grid, dim3(256), 0, st, M, P->pts, g->E, chunk,
                            (const double4*)g->q, (const double2*)s->qs, P->excl_ptr, P->excl_e,
@@ -623,7 +623,7 @@ hipError_t launch_mst_batch(dep_sys* s, const dep_points* P, const double* X, lo
     constexpr int FB = 8;
     const int chunk = dep_cdiv(E, P->nchunk);
     hipEvent_t ev;
-    prof_begin(g->ctx, &ev);
+    prof_begin(g->ctx, &ev, "points_field_batch");
     hipLaunchKernelGGL(k_points_field_fb<FB>, dim3(dep_cdiv(M, 256), P->nchunk, dep_cdiv(nfreq, FB)), dim3(256), 0, st,
                        M, P->pts, E, chunk, (const double4*)g->q, (const double2*)qs, qss / 2, nfreq, P->excl_ptr,
                        P->excl_e, partial, ps);

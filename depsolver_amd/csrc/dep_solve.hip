@@ -371,7 +371,7 @@ hipError_t launch_zmatvec_w(dep_sys* s, const double* wr, const double* wi, cons
     const int k = tune_choose(g->ctx, KNOB_ZMV64, g->N, true,
                               [&](int c) { return zmatvec64_shape(g, c, wr, wi, shift, x, b, y); });
     hipEvent_t ev;
-    prof_begin(g->ctx, &ev);
+    prof_begin(g->ctx, &ev, "zmatvec");
     const hipError_t e = zmatvec64_shape(g, k, wr, wi, shift, x, b, y);
     const double n = g->N;
     prof_end(g->ctx, ev, "zmatvec", 4.0 * n * n, 8.0 * n * n + 48.0 * n);
@@ -386,7 +386,7 @@ hipError_t launch_zmatvec32_w(dep_sys* s, const double* wr, const double* wi, co
     const int k = tune_choose(g->ctx, KNOB_ZMV32, g->N, stop == nullptr,
                               [&](int c) { return zmatvec32_shape(g, c, wr, wi, shift, x, b, y, nullptr); });
     hipEvent_t ev;
-    prof_begin(g->ctx, &ev);
+    prof_begin(g->ctx, &ev, "zmatvec32");
     const hipError_t e = zmatvec32_shape(g, k, wr, wi, shift, x, b, y, stop);
     const double n = g->N;
     prof_end(g->ctx, ev, "zmatvec32", 4.0 * n * n, 4.0 * n * n + 56.0 * n);

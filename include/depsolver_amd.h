@@ -75,9 +75,13 @@ int dep_ctx_synchronize(dep_ctx *ctx);
 /* device memory in use by this context, bytes */
 size_t dep_ctx_device_bytes(const dep_ctx *ctx);
 /* Per-kernel timing with HIP events on the context's stream (for roofline
- * reporting; off by default).  Names: "assemble_regular", "lu_gemm_fp64",
- * "lu_gemm_mixed", "points_regular", "zmatvec".  flops/bytes are the
- * algorithmic counts of the timed launches. */
+ * reporting; off by default).  enable: 0 off, 1 every launch, k > 1 every
+ * k-th launch of each kernel (an event record costs the stream a few
+ * microseconds, so sampling keeps the timed work close to an untimed run).
+ * Names: "assemble_regular", "zmatvec", "zmatvec32", "points_field",
+ * "points_potential", "points_field_batch", "lu_gemm_fp64", "lu_gemm_mixed",
+ * "bgemm32", "bgemm64".  flops/bytes are the algorithmic counts of the timed
+ * launches. */
 int dep_ctx_set_profiling(dep_ctx *ctx, int enable);
 int dep_ctx_reset_stats(dep_ctx *ctx);
 int dep_ctx_kernel_stats(dep_ctx *ctx, const char *kernel, long *launches, double *total_ms,
