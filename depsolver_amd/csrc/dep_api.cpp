@@ -191,6 +191,28 @@ void prof_end(dep_ctx* c, hipEvent_t a, const char* name, double flops, double b
     c->pending.push_back(p);
 }
 
+void prof_pair(dep_ctx* c, const char* name, hipEvent_t* a, hipEvent_t* b)
+{
+    *a = *b = nullptr;
+    if (!c->prof) return;
+    c->prof_seq++;
+    if (c->prof_count[name]++ % (unsigned)c->prof_every != 0) return;
+    *a = pool_get(c);
+    *b = *a ? pool_get(c) : nullptr;
+    if (*a && !*b) {
+        c->pool.push_back(*a);
+        *a = nullptr;
+    }
+}
+
+void prof_done(dep_ctx* c, hipEvent_t a, hipEvent_t b, const char* name, double flops, double bytes)
+{
+    if (!c->prof || !a || !b) return;
+    dep_ctx::Pending p{name, a, b, flops, bytes};
+    p.seq = c->prof_seq;
+    c->pending.push_back(p);
+}
+
 size_t prof_mark(const dep_ctx* c) { return c->prof_seq + 1; }
 
 void prof_scale(dep_ctx* c, size_t seq, double f)

@@ -69,6 +69,13 @@ struct dep_ctx {
 // and for launches of `name` that the sampling period skips)
 void prof_begin(dep_ctx* c, hipEvent_t* a, const char* name, hipStream_t st = nullptr);
 void prof_end(dep_ctx* c, hipEvent_t a, const char* name, double flops, double bytes, hipStream_t st = nullptr);
+// For a launch that stamps its own dispatch's start and end into two events
+// (hipExtLaunchKernelGGL: the time of the kernel alone, no marker packets in
+// the stream, though such a launch still costs a few microseconds more):
+// prof_pair hands out both events (null when profiling is off or the sampling
+// period skips this launch), prof_done files them.
+void prof_pair(dep_ctx* c, const char* name, hipEvent_t* a, hipEvent_t* b);
+void prof_done(dep_ctx* c, hipEvent_t a, hipEvent_t b, const char* name, double flops, double bytes);
 void prof_collect(dep_ctx* c);   // after a synchronize: fold finished events into stats
 // prof_mark names the next prof_begin call; a caller that learns later how
 // much of that launch was useful scales its algorithmic flops by f (a batched

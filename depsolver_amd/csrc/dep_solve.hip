@@ -5,6 +5,8 @@
 // FP64 for the FP64 LU (dep_lu.hip), and k_zmatvec evaluates Z x (or the
 // residual b - Z x) straight from the real blocks P, streaming 8N^2 bytes per
 // product instead of the 16N^2 of a complex FP64 Z.
+#include <hip/hip_ext.h>
+
 #include "dep_common.h"
 #include "dep_internal.h"
 
@@ -313,24 +315,36 @@ hipError_t launch_residual(dep_sys* s, const double* x, double* r)
     return launch_zmatvec_w(s, s->wr, s->wi, s->shift, x, s->b, r);
 }
 
+// A launch that, given events ea and eb, records its own dispatch's start and
+// end in them (no marker packets in the stream around it)
+template <typename K, typename... A>
+static void launch_timed(K kernel, dim3 grid, dim3 block, hipStream_t st, hipEvent_t ea, hipEvent_t eb, A... args)
+{
+    if (ea)
+        hipExtLaunchKernelGGL(kernel, grid, block, 0, st, ea, eb, 0, args...);
+    else
+        hipLaunchKernelGGL(kernel, grid, block, 0, st, args...);
+}
+
 // Shapes of the FP64 product (autotuned): rows per wave x waves per workgroup
 static hipError_t zmatvec64_shape(dep_geom* g, int k, const double* wr, const double* wi, const double* shift,
-                                  const double* x, const double* b, double* y)
+                                  const double* x, const double* b, double* y, hipEvent_t ea = nullptr,
+                                  hipEvent_t eb = nullptr)
 {
     const int N = g->N;
     hipStream_t st = g->ctx->stream;
     switch (k) {
     case 1:
-        hipLaunchKernelGGL((k_zmatvec64<2, 4>), dim3(dep_cdiv(N, 8)), dim3(256), 0, st, N, g->P, g->ldP, wr, wi, shift,
-                           x, b, y);
+        launch_timed(k_zmatvec64<2, 4>, dim3(dep_cdiv(N, 8)), dim3(256), st, ea, eb, N, (const double*)g->P, g->ldP,
+                     wr, wi, shift, x, b, y);
         break;
     case 2:
-        hipLaunchKernelGGL((k_zmatvec64<4, 1>), dim3(dep_cdiv(N, 4)), dim3(64), 0, st, N, g->P, g->ldP, wr, wi, shift,
-                           x, b, y);
+        launch_timed(k_zmatvec64<4, 1>, dim3(dep_cdiv(N, 4)), dim3(64), st, ea, eb, N, (const double*)g->P, g->ldP,
+                     wr, wi, shift, x, b, y);
         break;
     default:
-        hipLaunchKernelGGL((k_zmatvec64<4, 4>), dim3(dep_cdiv(N, 16)), dim3(256), 0, st, N, g->P, g->ldP, wr, wi,
-                           shift, x, b, y);
+        launch_timed(k_zmatvec64<4, 4>, dim3(dep_cdiv(N, 16)), dim3(256), st, ea, eb, N, (const double*)g->P, g->ldP,
+                     wr, wi, shift, x, b, y);
     }
     return hipGetLastError();
 }
@@ -340,26 +354,28 @@ static hipError_t zmatvec64_shape(dep_geom* g, int k, const double* wr, const do
 // back-to-back timing at N = 15,044), or x read by each wave (6 or 4 rows per
 // wave); two chunks in flight in every shape
 static hipError_t zmatvec32_shape(dep_geom* g, int k, const double* wr, const double* wi, const double* shift,
-                                  const double* x, const double* b, double* y, const int* stop)
+                                  const double* x, const double* b, double* y, const int* stop,
+                                  hipEvent_t ea = nullptr, hipEvent_t eb = nullptr)
 {
     const int N = g->N;
     hipStream_t st = g->ctx->stream;
+    const float* P32 = g->P32;
     switch (k) {
     case 1:
-        hipLaunchKernelGGL((k_zmatvec32_lx<8, 4>), dim3(dep_cdiv(N, 32)), dim3(256), 0, st, N, g->P32, g->ldP, wr, wi,
-                           shift, x, b, y, stop);
+        launch_timed(k_zmatvec32_lx<8, 4>, dim3(dep_cdiv(N, 32)), dim3(256), st, ea, eb, N, P32, g->ldP, wr, wi, shift,
+                     x, b, y, stop);
         break;
     case 2:
-        hipLaunchKernelGGL((k_zmatvec<float, double, 6, 2, 1, true>), dim3(dep_cdiv(N, 6)), dim3(64), 0, st, N,
-                           g->P32, g->ldP, wr, wi, shift, x, b, y, stop);
+        launch_timed(k_zmatvec<float, double, 6, 2, 1, true>, dim3(dep_cdiv(N, 6)), dim3(64), st, ea, eb, N, P32,
+                     g->ldP, wr, wi, shift, x, b, y, stop);
         break;
     case 3:
-        hipLaunchKernelGGL((k_zmatvec<float, double, 4, 2, 1, true>), dim3(dep_cdiv(N, 4)), dim3(64), 0, st, N,
-                           g->P32, g->ldP, wr, wi, shift, x, b, y, stop);
+        launch_timed(k_zmatvec<float, double, 4, 2, 1, true>, dim3(dep_cdiv(N, 4)), dim3(64), st, ea, eb, N, P32,
+                     g->ldP, wr, wi, shift, x, b, y, stop);
         break;
     default:
-        hipLaunchKernelGGL((k_zmatvec32_lx<4, 4>), dim3(dep_cdiv(N, 16)), dim3(256), 0, st, N, g->P32, g->ldP, wr, wi,
-                           shift, x, b, y, stop);
+        launch_timed(k_zmatvec32_lx<4, 4>, dim3(dep_cdiv(N, 16)), dim3(256), st, ea, eb, N, P32, g->ldP, wr, wi, shift,
+                     x, b, y, stop);
     }
     return hipGetLastError();
 }
@@ -370,11 +386,11 @@ hipError_t launch_zmatvec_w(dep_sys* s, const double* wr, const double* wi, cons
     dep_geom* g = s->g;
     const int k = tune_choose(g->ctx, KNOB_ZMV64, g->N, true,
                               [&](int c) { return zmatvec64_shape(g, c, wr, wi, shift, x, b, y); });
-    hipEvent_t ev;
-    prof_begin(g->ctx, &ev, "zmatvec");
-    const hipError_t e = zmatvec64_shape(g, k, wr, wi, shift, x, b, y);
+    hipEvent_t ea, eb;
+    prof_pair(g->ctx, "zmatvec", &ea, &eb);
+    const hipError_t e = zmatvec64_shape(g, k, wr, wi, shift, x, b, y, ea, eb);
     const double n = g->N;
-    prof_end(g->ctx, ev, "zmatvec", 4.0 * n * n, 8.0 * n * n + 48.0 * n);
+    prof_done(g->ctx, ea, eb, "zmatvec", 4.0 * n * n, 8.0 * n * n + 48.0 * n);
     return e;
 }
 
@@ -385,11 +401,11 @@ hipError_t launch_zmatvec32_w(dep_sys* s, const double* wr, const double* wi, co
     // a launch behind a stop flag may return at once, so only unflagged launches time the shapes
     const int k = tune_choose(g->ctx, KNOB_ZMV32, g->N, stop == nullptr,
                               [&](int c) { return zmatvec32_shape(g, c, wr, wi, shift, x, b, y, nullptr); });
-    hipEvent_t ev;
-    prof_begin(g->ctx, &ev, "zmatvec32");
-    const hipError_t e = zmatvec32_shape(g, k, wr, wi, shift, x, b, y, stop);
+    hipEvent_t ea, eb;
+    prof_pair(g->ctx, "zmatvec32", &ea, &eb);
+    const hipError_t e = zmatvec32_shape(g, k, wr, wi, shift, x, b, y, stop, ea, eb);
     const double n = g->N;
-    prof_end(g->ctx, ev, "zmatvec32", 4.0 * n * n, 4.0 * n * n + 56.0 * n);
+    prof_done(g->ctx, ea, eb, "zmatvec32", 4.0 * n * n, 4.0 * n * n + 56.0 * n);
     return e;
 }
 
