@@ -637,6 +637,9 @@ extern "C" int dep_sys_destroy(dep_sys* s)
     if (!s) return DEP_OK;
     (void)hipStreamSynchronize(s->g->ctx->stream);
     if (s->trsv_exec) (void)hipGraphExecDestroy(s->trsv_exec);
+    for (hipEvent_t& e : s->gm_fev)
+        if (e) (void)hipEventDestroy(e);
+    if (s->gm_hflags) (void)hipHostFree(s->gm_hflags);
     free_all(s);
     delete s;
     return DEP_OK;
@@ -1575,16 +1578,26 @@ static int gmres_run_dev(dep_sys* s, MV matvec, bool mv_scales, const double* b,
     const bool defer = fused && mv_scales;
     const dep_defer dg{cap, h1, h2, np, H, cs, sn, g, err, sc, flags};
     constexpr int BATCH = 8;
+    if (!s->gm_hflags) {
+        DEP_HIP(c, hipHostMalloc((void**)&s->gm_hflags, 8 * sizeof(int), hipHostMallocMapped | hipHostMallocCoherent));
+        DEP_HIP(c, hipHostGetDevicePointer((void**)&s->gm_hflags_dev, s->gm_hflags, 0));
+        for (hipEvent_t& e : s->gm_fev) DEP_HIP(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
     int hf[3] = {0, 0, 1};
-    size_t mark[BATCH];   // the products' timing records (steps past the stop are dropped)
-    for (int i0 = 0; i0 < M; i0 += BATCH) {
+    std::vector<size_t> mark(M);   // the products' timing records (steps past the stop are dropped)
+    int enq = 0;                   // steps enqueued
+    // Two batches are in flight: while the device runs batch k + 1, the host
+    // waits for batch k's flags (written to mapped host memory by a kernel
+    // behind the batch) and enqueues batch k + 2,
+    // so the device does not idle between batches.  Steps behind a stop return
+    // at once; only the very first step runs without the stop flag (it runs for
+    // certain, and its product may time the product's shapes).
+    auto enqueue = [&](int i0) -> int {
         for (int i = i0; i < std::min(M, i0 + BATCH); i++) {
             double* vi = V + (size_t)i * n2;
             double* wv = V + (size_t)(i + 1) * n2;
-            mark[i - i0] = prof_mark(c);
-            // the first step of a batch runs for certain (the flags read before it said so),
-            // so its product goes without the stop flag (and may time the product's shapes)
-            DEP_HIP(c, matvec(vi, wv, i == i0 ? nullptr : flags, mv_scales ? sc : nullptr, defer ? &dg : nullptr));
+            mark[i] = prof_mark(c);
+            DEP_HIP(c, matvec(vi, wv, i == 0 ? nullptr : flags, mv_scales ? sc : nullptr, defer ? &dg : nullptr));
             DEP_HIP(c, launch_dot_partials(c, i + 2, n2, V, wv, part, flags));
             if (fused) {
                 DEP_HIP(c, launch_cgs_step(c, i, cap, n2, V, part, part2, h1, h2, defer, np, H, cs, sn, g, err, sc,
@@ -1597,13 +1610,25 @@ static int gmres_run_dev(dep_sys* s, MV matvec, bool mv_scales, const double* b,
             }
             if (!mv_scales) DEP_HIP(c, launch_scale_dev(c, n2, wv, sc, flags));
         }
+        enq = std::min(M, i0 + BATCH);
         if (defer) DEP_HIP(c, launch_gmres_finish(c, dg));
-        DEP_CHECK(download(c, hf, flags, 3));
+        const int slot = (i0 / BATCH) & 1;
+        DEP_HIP(c, launch_flags_out(c, flags, s->gm_hflags_dev + 4 * slot));
+        DEP_HIP(c, hipEventRecord(s->gm_fev[slot], c->stream));
+        return DEP_OK;
+    };
+    DEP_CHECK(enqueue(0));
+    if (BATCH < M) DEP_CHECK(enqueue(BATCH));
+    for (int i0 = 0; i0 < M; i0 += BATCH) {
+        const int slot = (i0 / BATCH) & 1;
+        DEP_HIP(c, hipEventSynchronize(s->gm_fev[slot]));
+        memcpy(hf, s->gm_hflags + 4 * slot, sizeof hf);
         if (hf[0]) {
             if (c->prof)
-                for (int i = std::max(i0, hf[1]); i < std::min(M, i0 + BATCH); i++) prof_scale(c, mark[i - i0], 0.0);
+                for (int i = hf[1]; i < enq; i++) prof_scale(c, mark[i], 0.0);
             break;
         }
+        if (i0 + 2 * BATCH < M) DEP_CHECK(enqueue(i0 + 2 * BATCH));
     }
     const int iters = hf[1], rc = hf[0] ? hf[2] : 1;
     if (rc != 2 && iters > 0) {

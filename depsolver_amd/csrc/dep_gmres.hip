@@ -265,6 +265,14 @@ __global__ void k_gmres_finish(int cap, const double* __restrict__ h1, const dou
     givens_step(p - 1, cap, h1, h2, hn2, H, cs, sn, g, err, sc, flags, lds);
 }
 
+// dst[0..2] = flags[0..2]: the stop flags written straight into mapped,
+// coherent host memory (a copy engine transfer of 12 bytes would stall the
+// stream for tens of microseconds)
+__global__ void k_flags_out(const int* __restrict__ flags, int* __restrict__ dst)
+{
+    if (threadIdx.x < 3) dst[threadIdx.x] = flags[threadIdx.x];
+}
+
 // The block-Jacobi apply of the refined GMRES with the previous step's Givens
 // step folded in (float 64 x 64 blocks, one system): workgroups 0 .. nblk-1
 // scale x = V_i by 1/|V_i| (from the norm partials np when a Givens step is
@@ -645,6 +653,12 @@ hipError_t launch_cgs_step(dep_ctx* c, int i, int cap, long n2, double* V, const
                        flags);
     hipLaunchKernelGGL(k_cgs_last, dim3(DEP_UPD_BLOCKS), dim3(256), 0, c->stream, i, cap, n2, V, part2, w, h1, h2,
                        defer, np, H, cs, sn, g, err, sc, flags);
+    return hipGetLastError();
+}
+
+hipError_t launch_flags_out(dep_ctx* c, const int* flags, int* dst)
+{
+    hipLaunchKernelGGL(k_flags_out, dim3(1), dim3(64), 0, c->stream, flags, dst);
     return hipGetLastError();
 }
 

@@ -162,6 +162,11 @@ struct dep_sys {
     double *gm_vec = nullptr, *gm_V = nullptr, *gm_red = nullptr;
     double* gm_dev = nullptr;         // device Givens state: H, cs, sn, g, err, scalars
     int* gm_flags = nullptr;          // stop, steps, rc
+    // the flags of the last two batches of device-driven GMRES steps, copied
+    // to pinned host memory behind each batch (2 x 4 ints), and their events
+    int* gm_hflags = nullptr;
+    int* gm_hflags_dev = nullptr;     // the same memory as the device addresses it
+    hipEvent_t gm_fev[2] = {nullptr, nullptr};
     // block-Jacobi preconditioner of the refined FP32 GMRES (valid for bj_asm / bj_freq)
     float* bjB = nullptr;
     double* bj_scr = nullptr;
@@ -296,6 +301,8 @@ struct dep_defer {
     double *h1, *h2, *np, *H, *cs, *sn, *g, *err, *sc;
     int* flags;
 };
+// dst[0..2] = flags[0..2], dst a device pointer to mapped host memory
+hipError_t launch_flags_out(dep_ctx* c, const int* flags, int* dst);
 // The deferred Givens step, if one is pending (ends a batch of steps).
 hipError_t launch_gmres_finish(dep_ctx* c, const dep_defer& d);
 // Block-Jacobi apply y = B (x / |x|) of one system (float blocks) that also
