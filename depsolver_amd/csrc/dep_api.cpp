@@ -1611,9 +1611,9 @@ static int gmres_run_dev(dep_sys* s, MV matvec, bool mv_scales, const double* b,
             if (!mv_scales) DEP_HIP(c, launch_scale_dev(c, n2, wv, sc, flags));
         }
         enq = std::min(M, i0 + BATCH);
-        if (defer) DEP_HIP(c, launch_gmres_finish(c, dg));
         const int slot = (i0 / BATCH) & 1;
-        DEP_HIP(c, launch_flags_out(c, flags, s->gm_hflags_dev + 4 * slot));
+        int* dst = s->gm_hflags_dev + 4 * slot;
+        DEP_HIP(c, defer ? launch_gmres_finish(c, dg, dst) : launch_flags_out(c, flags, dst));
         DEP_HIP(c, hipEventRecord(s->gm_fev[slot], c->stream));
         return DEP_OK;
     };

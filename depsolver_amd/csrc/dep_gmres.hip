@@ -252,17 +252,24 @@ __device__ inline double np_sum(const double* np)
 
 // The deferred Givens step of column flags[4] - 1, if it has not run (flags[1]
 // counts the columns done); one wave.  Ends a batch of deferred steps so that
-// the flags the host reads are complete.
+// the flags are complete, then copies flags[0..2] to dst (mapped, coherent
+// host memory) when given.
 __global__ void k_gmres_finish(int cap, const double* __restrict__ h1, const double* __restrict__ h2,
                                const double* __restrict__ np, double* __restrict__ H, double* __restrict__ cs,
                                double* __restrict__ sn, double* __restrict__ g, double* __restrict__ err,
-                               double* __restrict__ sc, int* __restrict__ flags)
+                               double* __restrict__ sc, int* __restrict__ flags, int* __restrict__ dst)
 {
     const int p = flags[4];
-    if (flags[0] || p <= flags[1]) return;
-    __shared__ double lds[3 * (UPD_MAXV + 1)];
-    const double hn2 = np_sum(np);
-    givens_step(p - 1, cap, h1, h2, hn2, H, cs, sn, g, err, sc, flags, lds);
+    if (!flags[0] && p > flags[1]) {   // every thread has read the flags before the step's barrier
+        __shared__ double lds[3 * (UPD_MAXV + 1)];
+        const double hn2 = np_sum(np);
+        givens_step(p - 1, cap, h1, h2, hn2, H, cs, sn, g, err, sc, flags, lds);
+    }
+    if (threadIdx.x == 0 && dst) {   // thread 0 made the step's flag writes
+        dst[0] = flags[0];
+        dst[1] = flags[1];
+        dst[2] = flags[2];
+    }
 }
 
 // dst[0..2] = flags[0..2]: the stop flags written straight into mapped,
@@ -662,10 +669,10 @@ hipError_t launch_flags_out(dep_ctx* c, const int* flags, int* dst)
     return hipGetLastError();
 }
 
-hipError_t launch_gmres_finish(dep_ctx* c, const dep_defer& d)
+hipError_t launch_gmres_finish(dep_ctx* c, const dep_defer& d, int* dst)
 {
     hipLaunchKernelGGL(k_gmres_finish, dim3(1), dim3(64), 0, c->stream, d.cap, d.h1, d.h2, d.np, d.H, d.cs, d.sn,
-                       d.g, d.err, d.sc, d.flags);
+                       d.g, d.err, d.sc, d.flags, dst);
     return hipGetLastError();
 }
 

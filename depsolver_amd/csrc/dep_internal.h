@@ -303,8 +303,9 @@ struct dep_defer {
 };
 // dst[0..2] = flags[0..2], dst a device pointer to mapped host memory
 hipError_t launch_flags_out(dep_ctx* c, const int* flags, int* dst);
-// The deferred Givens step, if one is pending (ends a batch of steps).
-hipError_t launch_gmres_finish(dep_ctx* c, const dep_defer& d);
+// The deferred Givens step, if one is pending (ends a batch of steps), then
+// dst[0..2] = flags[0..2] if dst is given (as launch_flags_out).
+hipError_t launch_gmres_finish(dep_ctx* c, const dep_defer& d, int* dst);
 // Block-Jacobi apply y = B (x / |x|) of one system (float blocks) that also
 // runs the pending Givens step; x is scaled in place.
 hipError_t launch_bj_apply_dg(dep_ctx* c, int N, const float* B, double* x, double* y, const int* stop,
