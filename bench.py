@@ -35,12 +35,12 @@ PEAKS = {  # MI355X published dense peaks
     "fp16": (2500.0, "TFLOP/s", "mfma"), "bf16": (2500.0, "TFLOP/s", "mfma"),
     "hbm": (8000.0, "GB/s", "hbm"),
 }
-# per-kernel launch times come from HIP events around every 7th launch of each
-# kernel inside the timed regions: each event record stalls the stream for a few
-# microseconds, which around every launch would add ~0.6 ms to a config-2 solve.
-# 7 is prime to the refined GMRES's 8-step batches, so the timed products are
-# spread over the positions in a batch (the first one follows the host's flag
-# read and runs a little slower)
+# per-kernel launch times come from HIP events on every 7th launch of each
+# kernel inside the timed regions (the products stamp their own dispatch's start
+# and end into the events; the other kernels are bracketed by event records).
+# Timing every launch would stall the stream for a few microseconds per launch
+# (~0.3 ms per config-2 solve).  7 is prime to the refined GMRES's 8-step
+# batches, so the timed products are spread over the positions in a batch.
 PROF_EVERY = 7
 KERNEL_PEAK = {"lu_gemm_fp64": "fp64", "lu_gemm_mixed": "fp16", "assemble_regular": "fp64",
                "points_field": "fp64", "points_potential": "fp64", "zmatvec": "hbm", "zmatvec32": "hbm",
